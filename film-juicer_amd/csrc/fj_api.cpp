@@ -1,0 +1,271 @@
+// fj_api.cpp — the drop-in C-ABI (include/fj/juicer_cuda_api.h) over HIP.
+//
+// Boundary semantics follow the reference entry points:
+//   owner lifecycle         native/juicer_cuda_api.cpp:245, :563-644
+//   span validation         native/juicer_cuda_prepared.cpp:58-158
+//   frame binding checks    native/juicer_cuda_prepared.cpp:160-222
+//   descriptor tag checks   native/juicer_cuda_descriptors.cpp:522-600
+//   render + completion     native/juicer_cuda_api.cpp:403-526
+// Device work is issued on the caller's stream; host tables are packed into
+// GPU-friendly layouts once and cached per device by content identity.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "fj_host.h"
+#include "fj_plan.h"
+
+using namespace fjx;
+
+struct FjCuda {
+    std::mutex mu;
+    bool open = true;
+    std::map<int, std::unique_ptr<DeviceState>> devices;
+};
+
+namespace {
+
+std::atomic<FjCuda*> g_owner{nullptr};
+thread_local bool t_in_call = false;
+
+FjStatus make_status(uint32_t category, uint32_t api = FJ_API_NONE, int32_t code = 0) {
+    FjStatus s;
+    s.category = category;
+    s.api = api;
+    s.native_code = api == FJ_API_NONE ? 0 : code;
+    return s;
+}
+
+void write_error(FjErrorBuffer* error, const std::string& text) {
+    if (!error) return;
+    if (error->capacity == 0 || error->data == nullptr) {
+        error->length = 0;
+        return;
+    }
+    const size_t n = std::min(text.size(), error->capacity - 1);
+    std::memcpy(error->data, text.data(), n);
+    error->data[n] = '\0';
+    error->length = n;
+}
+
+FjStatus fail(FjErrorBuffer* error, uint32_t category, const std::string& text, uint32_t api = FJ_API_NONE,
+              int32_t code = 0) {
+    write_error(error, text);
+    return make_status(category, api, code);
+}
+
+FjStatus hip_fail(FjErrorBuffer* error, hipError_t e, const char* where) {
+    std::string text = std::string("HIP failure in ") + where + ": " + hipGetErrorString(e);
+    const uint32_t category = (e == hipErrorOutOfMemory) ? FJ_STATUS_ALLOCATION_FAILURE : FJ_STATUS_CUDA_FAILURE;
+    return fail(error, category, text, FJ_API_CUDA_RUNTIME, static_cast<int32_t>(e));
+}
+
+struct CallGuard {
+    bool ok;
+    CallGuard() : ok(!t_in_call) {
+        if (ok) t_in_call = true;
+    }
+    ~CallGuard() {
+        if (ok) t_in_call = false;
+    }
+};
+
+bool image_device(const FjImage& img, int& device) {
+    if (img.address == 0) return false;
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, reinterpret_cast<const void*>(img.address)) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged) return false;
+    device = attr.device;
+    return true;
+}
+
+FjStatus inspect_frame(const FjFrame* frame, FjCudaContext* out, FjErrorBuffer* error) {
+    if (!frame) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "frame is null");
+    int src_dev = -1, dst_dev = -1;
+    if (!image_device(frame->source, src_dev) || !image_device(frame->destination, dst_dev))
+        return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "frame images are not HIP device memory");
+    if (src_dev != dst_dev)
+        return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "source and destination live on different devices");
+    if (out) {
+        out->device_id = src_dev;
+        out->context = static_cast<uintptr_t>(src_dev) + 1u;
+    }
+    return make_status(FJ_STATUS_SUCCESS);
+}
+
+}  // namespace
+
+extern "C" {
+
+FjStatus fj_cuda_create(FjCuda** out_cuda, FjErrorBuffer* error) {
+    try {
+        if (!out_cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "out_cuda is null");
+        *out_cuda = nullptr;
+        auto owner = std::make_unique<FjCuda>();
+        FjCuda* expected = nullptr;
+        if (!g_owner.compare_exchange_strong(expected, owner.get()))
+            return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "a film render runtime is already active");
+        *out_cuda = owner.release();
+        write_error(error, "");
+        return make_status(FJ_STATUS_SUCCESS);
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fj_cuda_create");
+    }
+}
+
+FjStatus fj_cuda_check_terminal(FjCuda* cuda, FjErrorBuffer* error) {
+    if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
+    if (t_in_call) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
+    std::lock_guard<std::mutex> lock(cuda->mu);
+    if (!cuda->open) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+    write_error(error, "");
+    return make_status(FJ_STATUS_SUCCESS);
+}
+
+FjStatus fj_cuda_inspect(FjCuda* cuda, const FjFrame* frame, FjCudaContext* out_context, FjErrorBuffer* error) {
+    try {
+        if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
+        CallGuard guard;
+        if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
+        std::lock_guard<std::mutex> lock(cuda->mu);
+        if (!cuda->open) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+        if (!out_context) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "out_context is null");
+        FjCudaContext ctx{};
+        const FjStatus s = inspect_frame(frame, &ctx, error);
+        if (s.category == FJ_STATUS_SUCCESS) {
+            *out_context = ctx;
+            write_error(error, "");
+        }
+        return s;
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fj_cuda_inspect");
+    }
+}
+
+FjRenderOutcome fj_cuda_render(FjCuda* cuda, const FjCudaContext* context, const FjFrame* frame,
+                               const FjSubmission* submission, const FjPreparedHostData* prepared,
+                               FjAbortCallback abort_callback, FjErrorBuffer* error) {
+    FjRenderOutcome outcome{};
+    try {
+        if (!cuda || g_owner.load() != cuda) {
+            outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
+            return outcome;
+        }
+        CallGuard guard;
+        if (!guard.ok) {
+            outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
+            return outcome;
+        }
+        std::lock_guard<std::mutex> lock(cuda->mu);
+        if (!cuda->open) {
+            outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+            return outcome;
+        }
+        if (!context || !frame || !submission || !prepared) {
+            outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null render argument");
+            return outcome;
+        }
+        FjCudaContext current{};
+        FjStatus s = inspect_frame(frame, &current, error);
+        if (s.category != FJ_STATUS_SUCCESS) {
+            outcome.status = s;
+            return outcome;
+        }
+        if (current.device_id != context->device_id || current.context != context->context) {
+            outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "render context differs from the frame's device");
+            return outcome;
+        }
+        hipError_t he = hipSetDevice(current.device_id);
+        if (he != hipSuccess) {
+            outcome.status = hip_fail(error, he, "hipSetDevice");
+            return outcome;
+        }
+        auto& slot = cuda->devices[current.device_id];
+        if (!slot) {
+            slot = std::make_unique<DeviceState>();
+            std::string diag;
+            he = slot->init(current.device_id);
+            if (he != hipSuccess) {
+                slot.reset();
+                outcome.status = hip_fail(error, he, "device state init");
+                return outcome;
+            }
+        }
+        outcome = render_frame(*slot, *frame, *submission, *prepared, abort_callback, error);
+        if (outcome.status.category == FJ_STATUS_SUCCESS) write_error(error, "");
+        return outcome;
+    } catch (const std::bad_alloc&) {
+        outcome.status = fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
+        return outcome;
+    } catch (...) {
+        outcome.status = fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fj_cuda_render");
+        return outcome;
+    }
+}
+
+FjStatus fj_cuda_retire_instance(FjCuda* cuda, uint64_t instance_token, FjErrorBuffer* error) {
+    if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
+    if (instance_token == 0) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "instance token is zero");
+    CallGuard guard;
+    if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
+    std::lock_guard<std::mutex> lock(cuda->mu);
+    // Static grain state is keyed by table content, not by instance: nothing
+    // instance-owned remains after a render returns.
+    write_error(error, "");
+    return make_status(FJ_STATUS_SUCCESS);
+}
+
+FjStatus fj_cuda_shutdown(FjCuda* cuda, FjErrorBuffer* error) {
+    if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
+    CallGuard guard;
+    if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
+    std::lock_guard<std::mutex> lock(cuda->mu);
+    FjStatus result = make_status(FJ_STATUS_SUCCESS);
+    for (auto& kv : cuda->devices) {
+        if (!kv.second) continue;
+        const hipError_t e = kv.second->release();
+        if (e != hipSuccess && result.category == FJ_STATUS_SUCCESS) result = hip_fail(error, e, "shutdown");
+    }
+    cuda->devices.clear();
+    cuda->open = false;
+    if (result.category == FJ_STATUS_SUCCESS) write_error(error, "");
+    return result;
+}
+
+FjStatus fj_cuda_destroy(FjCuda* cuda, FjErrorBuffer* error) {
+    if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
+    if (t_in_call) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
+    FjStatus s = make_status(FJ_STATUS_SUCCESS);
+    {
+        std::lock_guard<std::mutex> lock(cuda->mu);
+        for (auto& kv : cuda->devices) {
+            if (kv.second) {
+                const hipError_t e = kv.second->release();
+                if (e != hipSuccess && s.category == FJ_STATUS_SUCCESS) s = hip_fail(error, e, "destroy");
+            }
+        }
+        cuda->devices.clear();
+        cuda->open = false;
+    }
+    FjCuda* expected = cuda;
+    g_owner.compare_exchange_strong(expected, nullptr);
+    delete cuda;
+    if (s.category == FJ_STATUS_SUCCESS) write_error(error, "");
+    return s;
+}
+
+}  // extern "C"

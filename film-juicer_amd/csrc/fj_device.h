@@ -1,0 +1,100 @@
+// fj_device.h — device-visible parameter blocks of the MI355X render path.
+//
+// Everything here is plain data passed by value as kernel arguments; tables
+// live in device memory (HBM, L2/MALL-resident) and are staged into LDS by the
+// kernels that search them. Channel order on device follows the reference:
+// film layers B/G/R, dyes C/M/Y (JuicerCudaFilmPayloads.cpp:157-190).
+#pragma once
+
+#include <stdint.h>
+
+namespace fjx {
+
+constexpr int kTcExtent = 192;     // TC LUT side (reference kFilmTcLutExtent)
+constexpr int kSpectralSamples = 81;
+constexpr int kGamutL = 64;
+constexpr int kGamutH = 720;
+constexpr int kMaxLdsCurve = 512;  // curves longer than this are read from global memory
+
+// Input decode + film exposure.
+struct ExposureParams {
+    const float4* tc_lut;   // [192*192] (R, G, B, pad)
+    float rgb_to_xyz[9];    // input_xyz_adapt * input_rgb_to_xyz when adaptation is on
+    int decode_cctf;        // 0, or the FJ_INPUT_* space to decode
+    float scale;            // exp2(EV) * route correction (validated > 0)
+};
+
+// Film development: [x | yB | yG | yR | cxB | cxG | cxR] each n floats.
+struct DevelopParams {
+    const float* curves;
+    int n;
+    float gamma_bgr[3];
+    int dir_mode;           // FJ_DIR_*
+    float M[9];             // [donor BGR][receiver BGR]
+    float dmax[3], dref[3], donor_k[3], recv_cref[3], recv_kr[3];
+};
+
+// Print exposure + development.
+// spec: k pairs of float4 {e_c, e_m, e_y, e_base} (already scaled by -log2 10)
+// and {s_c*illum, s_m*illum, s_y*illum, 0}. curves: [x | yC | yM | yY].
+struct PrintParams {
+    const float4* spec;
+    int k;
+    float normalizer;
+    float preflash[3];      // preflash_raw * preflash_exposure (0 when inactive)
+    float outer_scale;      // print exposure * scanner exposure scale
+    const float* curves;
+    int n;
+};
+
+// Scanner LUT (interleaved PCHIP nodes) and colour.
+// nodes: 3 float4 per node {v0 v1 v2 sc0}{sc1 sc2 sm0 sm1}{sm2 sy0 sy1 sy2}
+// cells: 2 float4 per cell {min0 min1 min2 max0}{max1 max2 - -}
+struct ScanParams {
+    const float4* nodes;
+    const float4* cells;
+    int res;
+    float dmin[3];
+    float inv_span[3];
+    int correction;
+    float corr_slope, corr_offset;
+    float xyz_to_rgb[9];    // xyz_to_rgb * cat02
+};
+
+struct ColorParams {
+    int gamut;
+    const float* cmax;      // [64][720]
+    float native_to_xyz[9];
+    float xyz_to_native[9];
+    float lknee[3];
+    float cknee[3];
+    int encode;             // apply CCTF
+    int matrix;             // apply dwg_to_output
+    float to_output[9];
+    int cctf;
+    float gamma, a, b, c, d, cutoff;
+};
+
+// A frame window: pixels [x0, x0+w) x [y0, y0+h) in absolute coordinates.
+struct FrameIO {
+    const char* src;        // address of pixel (x0, y0)
+    long long src_row_bytes;
+    char* dst;
+    long long dst_row_bytes;
+    int width, height, comps;
+};
+
+struct PixelParams {
+    FrameIO io;
+    ExposureParams ex;
+    DevelopParams dev;
+    PrintParams pr;
+    ScanParams sc;
+    ColorParams co;
+    unsigned int* error_word;  // bit 0 scan error; bits 8.. DIR failure
+    int print;
+    int linear_out;            // write linear planes instead of encoded RGBA
+    float* linear_planes;      // 3 planes of width*height when linear_out
+};
+
+}  // namespace fjx

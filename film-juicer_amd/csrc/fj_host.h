@@ -1,0 +1,39 @@
+// fj_host.h — internal host-side declarations of the MI355X render library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fj/juicer_cuda_api.h"
+#include "fj_device.h"
+
+namespace fjx {
+
+struct ScannerLutInput {
+    const float *eps_c, *eps_m, *eps_y;
+    const float *weighted_x, *weighted_y, *weighted_z;
+    const float* baseline;  // may be null
+    int sample_count;
+    float inv_yn;
+    float range_min_cmy[3];
+    float inverse_span_cmy[3];
+    int medium;             // 0 film (direct routes), 1 print
+    int resolution;
+};
+
+struct ScannerLut {
+    int resolution = 0;
+    std::vector<float> log2_xyz, slope_c, slope_m, slope_y, cell_min, cell_max;
+};
+
+bool build_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& diagnostic);
+
+// kernels (fj_pixel.hip)
+hipError_t launch_fused_pixel(const PixelParams& p, int grid_limit, hipStream_t stream);
+hipError_t launch_clear_word(unsigned int* word, hipStream_t stream);
+size_t fused_lds_bytes(const PixelParams& p);
+
+}  // namespace fjx

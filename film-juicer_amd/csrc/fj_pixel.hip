@@ -1,0 +1,147 @@
+// fj_pixel.hip — fused per-pixel spectral chain for gfx950.
+//
+// One persistent launch walks the render window in 256-pixel tiles. Each block
+// first stages the density curves and the compacted print spectra into LDS
+// (binary searches and the 81-λ loop then hit LDS, not L2), then every lane
+// runs the whole chain for one pixel in registers:
+//   input decode -> XYZ -> TC-LUT Mitchell bicubic -> exposure -> film curves
+//   -> DIR couplers -> [81-λ print exposure -> print curves] -> scanner PCHIP
+//   LUT -> exp2 -> correction -> CAT02/XYZ->RGB -> OkLch gamut -> encode.
+// Reference semantics: src/Cuda/JuicerCudaDeviceHelpers.cuh (a1-a6, a11-a15)
+// and run_scan_pixel (src/Cuda/Scan/JuicerCudaScanPipeline.cu:1700-1859).
+#include <hip/hip_runtime.h>
+
+#include "fj_device.h"
+#include "fj_pixel_math.h"
+
+namespace fjx {
+
+struct LdsLayout {
+    float* film;        // 4n or 7n floats
+    float* print;       // 4n' floats
+    float4* spec;       // 2k float4
+};
+
+template <bool kPrint>
+__device__ __forceinline__ LdsLayout stage_tables(const PixelParams& p, float* lds) {
+    LdsLayout L;
+    const int film_count = (p.dev.dir_mode ? 7 : 4) * p.dev.n;
+    const int print_count = kPrint ? 4 * p.pr.n : 0;
+    const int spec_count = kPrint ? 2 * p.pr.k : 0;
+    L.spec = reinterpret_cast<float4*>(lds);
+    L.film = lds + 4 * spec_count;
+    L.print = L.film + film_count;
+    for (int i = threadIdx.x; i < spec_count; i += blockDim.x) L.spec[i] = p.pr.spec[i];
+    for (int i = threadIdx.x; i < film_count; i += blockDim.x) L.film[i] = p.dev.curves[i];
+    for (int i = threadIdx.x; i < print_count; i += blockDim.x) L.print[i] = p.pr.curves[i];
+    __syncthreads();
+    return L;
+}
+
+template <bool kPrint, bool kLds>
+__global__ __launch_bounds__(256) void fused_pixel_kernel(PixelParams p, int tiles) {
+    extern __shared__ float4 lds_raw[];
+    float* lds = reinterpret_cast<float*>(lds_raw);
+    LdsLayout L;
+    if constexpr (kLds) {
+        L = stage_tables<kPrint>(p, lds);
+    } else {
+        L.film = const_cast<float*>(p.dev.curves);
+        L.print = const_cast<float*>(p.pr.curves);
+        L.spec = const_cast<float4*>(p.pr.spec);
+    }
+    const int width = p.io.width;
+    const long long total = static_cast<long long>(width) * p.io.height;
+    unsigned int errors = 0u;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const long long idx = static_cast<long long>(tile) * blockDim.x + threadIdx.x;
+        if (idx >= total) continue;
+        const int y = static_cast<int>(idx / width);
+        const int x = static_cast<int>(idx - static_cast<long long>(y) * width);
+        const float* sp = reinterpret_cast<const float*>(p.io.src + y * p.io.src_row_bytes) + x * p.io.comps;
+        float in[4];
+        if (p.io.comps == 4) {
+            const float4 v = *reinterpret_cast<const float4*>(sp);
+            in[0] = v.x; in[1] = v.y; in[2] = v.z; in[3] = v.w;
+        } else {
+            in[0] = sp[0]; in[1] = sp[1]; in[2] = sp[2]; in[3] = 1.0f;
+        }
+        float E[3];
+        camera_exposure(p.ex, in, E);
+        float D[3];
+        errors |= develop_film(p.dev, L.film, E, D);
+        if constexpr (kPrint) {
+            print_develop(p.pr, L.spec, L.print, D);
+        }
+        float rgb[3];
+        const bool ok = scan_colour(p.sc, p.co, D, rgb);
+        if (!ok) {
+            errors |= 1u;
+            rgb[0] = rgb[1] = rgb[2] = 0.0f;
+        }
+        if (p.linear_out) {
+            const long long plane = total;
+            p.linear_planes[idx] = rgb[0];
+            p.linear_planes[plane + idx] = rgb[1];
+            p.linear_planes[2 * plane + idx] = rgb[2];
+            continue;
+        }
+        if (ok) encode_output(p.co, rgb);
+        float* dp = reinterpret_cast<float*>(p.io.dst + y * p.io.dst_row_bytes) + x * p.io.comps;
+        if (p.io.comps == 4) {
+            *reinterpret_cast<float4*>(dp) = make_float4(rgb[0], rgb[1], rgb[2], in[3]);
+        } else {
+            dp[0] = rgb[0]; dp[1] = rgb[1]; dp[2] = rgb[2];
+        }
+    }
+    if (errors && p.error_word) atomicOr(p.error_word, errors);
+}
+
+__global__ void clear_word_kernel(unsigned int* word) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *word = 0u;
+}
+
+}  // namespace fjx
+
+// ---------------------------------------------------------------------------
+// launchers (C++ linkage inside the library)
+// ---------------------------------------------------------------------------
+namespace fjx {
+
+size_t fused_lds_bytes(const PixelParams& p) {
+    const size_t film = static_cast<size_t>(p.dev.dir_mode ? 7 : 4) * p.dev.n;
+    const size_t prn = p.print ? static_cast<size_t>(4) * p.pr.n : 0;
+    const size_t spec = p.print ? static_cast<size_t>(8) * p.pr.k : 0;
+    return (film + prn + spec) * sizeof(float);
+}
+
+hipError_t launch_clear_word(unsigned int* word, hipStream_t stream) {
+    hipLaunchKernelGGL(clear_word_kernel, dim3(1), dim3(64), 0, stream, word);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused_pixel(const PixelParams& p, int grid_limit, hipStream_t stream) {
+    const long long total = static_cast<long long>(p.io.width) * p.io.height;
+    if (total <= 0) return hipSuccess;
+    const int block = 256;
+    const long long tiles_ll = (total + block - 1) / block;
+    if (tiles_ll > 0x7fffffffLL) return hipErrorInvalidValue;
+    const int tiles = static_cast<int>(tiles_ll);
+    const size_t lds = fused_lds_bytes(p);
+    const bool use_lds = lds <= 64 * 1024 && p.dev.n <= kMaxLdsCurve && (!p.print || p.pr.n <= kMaxLdsCurve);
+    const int grid = tiles < grid_limit ? tiles : grid_limit;
+    if (p.print) {
+        if (use_lds)
+            hipLaunchKernelGGL((fused_pixel_kernel<true, true>), dim3(grid), dim3(block), lds, stream, p, tiles);
+        else
+            hipLaunchKernelGGL((fused_pixel_kernel<true, false>), dim3(grid), dim3(block), 0, stream, p, tiles);
+    } else {
+        if (use_lds)
+            hipLaunchKernelGGL((fused_pixel_kernel<false, true>), dim3(grid), dim3(block), lds, stream, p, tiles);
+        else
+            hipLaunchKernelGGL((fused_pixel_kernel<false, false>), dim3(grid), dim3(block), 0, stream, p, tiles);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace fjx

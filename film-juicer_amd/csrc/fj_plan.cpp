@@ -1,0 +1,708 @@
+// fj_plan.cpp — validation of one render call, packing of prepared tables into
+// device layouts, and the frame schedule on the caller's stream.
+#include "fj_plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "fj_host.h"
+
+namespace fjx {
+
+// ---------------------------------------------------------------------------
+// hashing: 64-bit words, multiply-xorshift mixing (content identity only)
+// ---------------------------------------------------------------------------
+uint64_t hash_bytes(const void* data, size_t bytes, uint64_t seed) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    uint64_t h = seed ^ (0x9E3779B97F4A7C15ull * (bytes + 1));
+    size_t i = 0;
+    for (; i + 8 <= bytes; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + i, 8);
+        h ^= w * 0xBF58476D1CE4E5B9ull;
+        h = (h << 27 | h >> 37) * 0x94D049BB133111EBull;
+    }
+    uint64_t tail = 0;
+    for (size_t j = 0; i + j < bytes; ++j) tail |= static_cast<uint64_t>(p[i + j]) << (8 * j);
+    h ^= tail * 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 31;
+    h *= 0xD6E8FEB86659FD93ull;
+    h ^= h >> 32;
+    return h ? h : 1;
+}
+
+// ---------------------------------------------------------------------------
+// device state
+// ---------------------------------------------------------------------------
+hipError_t DeviceState::init(int dev) {
+    device = dev;
+    hipError_t e = hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    e = hipMalloc(reinterpret_cast<void**>(&error_words), sizeof(unsigned int) * kRing);
+    if (e != hipSuccess) return e;
+    e = hipHostMalloc(reinterpret_cast<void**>(&error_host), sizeof(unsigned int) * kRing, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    for (int i = 0; i < kRing; ++i) {
+        error_host[i] = 0;
+        e = hipEventCreateWithFlags(&error_events[i], hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t DeviceState::release() {
+    hipError_t first = hipSuccess;
+    auto keep = [&](hipError_t e) {
+        if (first == hipSuccess && e != hipSuccess) first = e;
+    };
+    keep(hipSetDevice(device));
+    keep(hipDeviceSynchronize());
+    for (auto& kv : tables) keep(hipFree(kv.second.ptr));
+    tables.clear();
+    lru.clear();
+    table_bytes = 0;
+    if (workspace) keep(hipFree(workspace));
+    workspace = nullptr;
+    workspace_bytes = 0;
+    for (int i = 0; i < kRing; ++i)
+        if (error_events[i]) keep(hipEventDestroy(error_events[i]));
+    if (error_words) keep(hipFree(error_words));
+    if (error_host) keep(hipHostFree(error_host));
+    error_words = nullptr;
+    error_host = nullptr;
+    return first;
+}
+
+hipError_t DeviceState::table(uint64_t key, const void* host, size_t bytes, const void** out) {
+    auto it = tables.find(key);
+    if (it != tables.end()) {
+        lru.remove(key);
+        lru.push_front(key);
+        *out = it->second.ptr;
+        return hipSuccess;
+    }
+    constexpr size_t kBudget = size_t(1) << 30;  // device bytes kept for cached tables
+    constexpr size_t kMaxEntries = 256;
+    while (!lru.empty() && (table_bytes + bytes > kBudget || tables.size() >= kMaxEntries)) {
+        const uint64_t victim = lru.back();
+        lru.pop_back();
+        auto v = tables.find(victim);
+        if (v != tables.end()) {
+            // hipFree synchronizes the device, so no in-flight frame still reads it.
+            const hipError_t e = hipFree(v->second.ptr);
+            table_bytes -= v->second.bytes;
+            tables.erase(v);
+            if (e != hipSuccess) return e;
+        }
+    }
+    DeviceTable t;
+    t.bytes = bytes;
+    hipError_t e = hipMalloc(&t.ptr, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) return e;
+    // Synchronous copy: the prepared spans expire when the call returns.
+    e = hipMemcpy(t.ptr, host, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(t.ptr);
+        return e;
+    }
+    tables.emplace(key, t);
+    lru.push_front(key);
+    table_bytes += bytes;
+    *out = t.ptr;
+    return hipSuccess;
+}
+
+hipError_t DeviceState::ensure_workspace(size_t bytes) {
+    if (workspace_bytes >= bytes) return hipSuccess;
+    if (workspace) {
+        const hipError_t e = hipFree(workspace);
+        workspace = nullptr;
+        workspace_bytes = 0;
+        if (e != hipSuccess) return e;
+    }
+    const hipError_t e = hipMalloc(&workspace, bytes);
+    if (e == hipSuccess) workspace_bytes = bytes;
+    return e;
+}
+
+unsigned int DeviceState::drain_errors(bool wait_all) {
+    unsigned int bits = 0u;
+    for (int i = 0; i < kRing; ++i) {
+        if (!error_pending[i]) continue;
+        if (wait_all) {
+            if (hipEventSynchronize(error_events[i]) != hipSuccess) continue;
+        } else if (hipEventQuery(error_events[i]) != hipSuccess) {
+            (void)hipGetLastError();
+            continue;
+        }
+        bits |= error_host[i];
+        error_pending[i] = false;
+    }
+    return bits;
+}
+
+// ---------------------------------------------------------------------------
+// validation
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Diag {
+    std::string text;
+    bool malformed(const char* field) {
+        text = std::string("MalformedPreparedHostData field=") + field;
+        return false;
+    }
+    bool unsupported(const char* what) {
+        text = std::string("UnsupportedInput ") + what;
+        return false;
+    }
+};
+
+template <typename Span>
+bool exact(const Span& s, size_t count, const char* field, Diag& d) {
+    return (s.count == count && ((count == 0) == (s.data == nullptr))) || d.malformed(field);
+}
+
+bool is_print_route(uint32_t route) { return route == FJ_ROUTE_NEGATIVE_PRINT || route == FJ_ROUTE_POSITIVE_PRINT; }
+
+// native/juicer_cuda_descriptors.cpp:522-600 tag checks and
+// native/juicer_cuda_prepared.cpp:58-158 span checks.
+bool validate_prepared(const FjPreparedHostData& p, Diag& d) {
+    if (p.route > FJ_ROUTE_POSITIVE_PRINT || p.capture_polarity > FJ_POLARITY_POSITIVE)
+        return d.malformed("route_or_polarity");
+    const uint32_t expected_polarity = p.route <= FJ_ROUTE_NEGATIVE_PRINT ? FJ_POLARITY_NEGATIVE : FJ_POLARITY_POSITIVE;
+    if (p.capture_polarity != expected_polarity) return d.malformed("route_polarity");
+    if ((p.scanner_correction.flags & ~FJ_CORRECTION_ACTIVE) != 0 || (p.scanner_post.flags & ~FJ_POST_GLARE) != 0)
+        return d.malformed("scanner.flags");
+    const FjOutputColor& oc = p.output_color;
+    const uint32_t color_flags = FJ_COLOR_ENCODE_CCTF | FJ_COLOR_INPUT_IS_OUTPUT_SPACE | FJ_COLOR_GAMUT_COMPRESSION;
+    if (oc.color_space > FJ_OUTPUT_REC709 || (oc.flags & ~color_flags) != 0) return d.malformed("output_color.tags");
+    if (oc.encoding.schema != FJ_OUTPUT_ENCODING_SCHEMA || oc.encoding.cctf_kind > FJ_CCTF_DWG_INTERMEDIATE)
+        return d.malformed("output_color.encoding");
+
+    const bool print = is_print_route(p.route);
+    const bool layers = (p.grain.flags & FJ_GRAIN_SUBLAYERS) != 0;
+    const bool grain = (p.grain.flags & FJ_GRAIN_ACTIVE) != 0;
+    const bool gamut = (oc.flags & FJ_COLOR_GAMUT_COMPRESSION) != 0;
+    const FjFilmExposure& ex = p.film_exposure;
+    const FjFilmDevelopment& dv = p.film_development;
+    const FjDirCouplers& dir = p.dir_couplers;
+    if (ex.method > FJ_RAW_ARCTIC_2026_BETA04 || ex.input_color_space > FJ_INPUT_SRGB_REC709 ||
+        (ex.flags & ~(FJ_FILM_DECODE_CCTF | FJ_FILM_ADAPT_INPUT | FJ_FILM_AUTO_EXPOSURE)) != 0 ||
+        dir.mode > FJ_DIR_POSITIVE_RECEIVER_LANGMUIR || p.auto_exposure.method > FJ_METER_HIGHLIGHT_WEIGHTED ||
+        (p.print.flags & ~FJ_PRINT_PREFLASH) != 0)
+        return d.malformed("film_or_print_tag");
+    if ((!print && p.print.flags != 0) || (dir.mode == FJ_DIR_INACTIVE && (dir.hash != 0 || dir.compensated_axes_hash != 0)) ||
+        (!layers && dv.density_layers_hash != 0))
+        return d.malformed("inactive_family");
+    if ((p.film_profile_key.count != 0) != (p.film_profile_key.data != nullptr)) return d.malformed("film_profile_key");
+    if ((p.print_profile_key.count != 0) != (p.print_profile_key.data != nullptr)) return d.malformed("print_profile_key");
+    const bool mallett = ex.method == FJ_RAW_MALLETT_2019;
+    if (!exact(ex.sensitivity_rgb, 81 * 3, "film_exposure.sensitivity_rgb", d) ||
+        !exact(ex.tc_lut_rgba, mallett ? 0 : size_t(192) * 192 * 4, "film_exposure.tc_lut_rgba", d) ||
+        !exact(ex.mallett_illuminant, mallett ? 81 : 0, "film_exposure.mallett_illuminant", d) ||
+        !exact(ex.mallett_basis, mallett ? 81 * 3 : 0, "film_exposure.mallett_basis", d))
+        return false;
+    const size_t n = dv.log_exposure.count;
+    if (n == 0 || n > static_cast<size_t>(std::numeric_limits<int>::max()) / 3) return d.malformed("film_development.log_exposure.count");
+    if (!exact(dv.log_exposure, n, "film_development.log_exposure", d) ||
+        !exact(dv.density_rgb, n * 3, "film_development.density_rgb", d))
+        return false;
+    for (const auto& layer : dv.density_layers)
+        for (const auto& ch : layer)
+            if (!exact(ch, layers ? n : 0, "film_development.density_layers", d)) return false;
+    for (const auto& axis : dir.compensated_axes_rgb)
+        if (!exact(axis, dir.mode == FJ_DIR_INACTIVE ? 0 : n, "dir_couplers.compensated_axes_rgb", d)) return false;
+    const FjScannerSpectra& sp = p.scanner_spectra;
+    for (const FjFloatSpan* s : {&sp.dye_c, &sp.dye_m, &sp.dye_y, &sp.weighted_x, &sp.weighted_y, &sp.weighted_z})
+        if (!exact(*s, 81, "scanner_spectra", d)) return false;
+    if (!exact(sp.base_density, sp.base_density.count == 0 ? 0 : 81, "scanner_spectra.base_density", d) ||
+        !exact(oc.gamut_cmax, gamut ? size_t(64) * 720 : 0, "output_color.gamut_cmax", d))
+        return false;
+    const FjPrint& pr = p.print;
+    const size_t pn = print ? pr.log_exposure.count : 0;
+    if (print && (pn == 0 || pn > static_cast<size_t>(std::numeric_limits<int>::max()) / 3))
+        return d.malformed("print.log_exposure.count");
+    const bool preflash = print && (pr.flags & FJ_PRINT_PREFLASH) != 0;
+    if (!exact(pr.film_density_cmy, print ? 81 * 3 : 0, "print.film_density_cmy", d) ||
+        !exact(pr.film_base_density, print ? 81 : 0, "print.film_base_density", d) ||
+        !exact(pr.sensitivity_cmy, print ? 81 * 3 : 0, "print.sensitivity_cmy", d) ||
+        !exact(pr.log_exposure, pn, "print.log_exposure", d) || !exact(pr.density_cmy, pn * 3, "print.density_cmy", d) ||
+        !exact(pr.main_illuminant, print ? 81 : 0, "print.main_illuminant", d) ||
+        !exact(pr.preflash_illuminant, preflash ? 81 : 0, "print.preflash_illuminant", d))
+        return false;
+    const FjStaticNoise& nz = p.noise;
+    if (!grain && (nz.stbn_width || nz.stbn_height || nz.stbn_frames || nz.wang_width || nz.wang_height ||
+                   nz.wang_tile_count || nz.wang_colors))
+        return d.malformed("inactive_noise");
+    size_t stbn = 0, wang = 0, lut = 0;
+    if (grain) {
+        if (nz.stbn_width <= 0 || nz.stbn_height <= 0 || nz.stbn_frames <= 0 || nz.wang_width <= 0 ||
+            nz.wang_height <= 0 || nz.wang_colors <= 0 || nz.wang_tile_count == 0 || nz.wang_colors > 255 ||
+            nz.wang_tile_count > static_cast<size_t>(std::numeric_limits<int>::max()))
+            return d.malformed("noise.dimensions");
+        stbn = size_t(nz.stbn_width) * size_t(nz.stbn_height) * size_t(nz.stbn_frames);
+        wang = size_t(nz.wang_width) * size_t(nz.wang_height) * nz.wang_tile_count;
+        lut = size_t(nz.wang_colors) * nz.wang_colors * nz.wang_colors * nz.wang_colors;
+    }
+    return exact(nz.stbn, stbn, "noise.stbn", d) && exact(nz.wang_tiles, wang, "noise.wang_tiles", d) &&
+           exact(nz.wang_lut, lut, "noise.wang_lut", d);
+}
+
+uint64_t norm_key(uint64_t v) { return v ? v : 1; }
+
+// native/juicer_cuda_prepared.cpp:160-222 (identity subset; auto exposure is
+// rejected separately).
+bool validate_binding(const FjPreparedHostData& p, const FjFrame& f, const FjSubmission& s, Diag& d) {
+    const double floor_t = std::floor(f.time_frames);
+    if (p.recipe_hash == 0 || s.instance_token == 0 || s.submission_id == 0 ||
+        s.upload_core_hash != norm_key(p.recipe_hash) || s.dir_hash != norm_key(p.dir_couplers.hash) ||
+        s.scanner_hash == 0 || s.auto_exposure_hash != norm_key(p.auto_exposure.hash) || !std::isfinite(f.time_frames) ||
+        f.time_frames < -0x1p63 || f.time_frames >= 0x1p63 ||
+        s.frame_token != static_cast<uint64_t>(static_cast<int64_t>(floor_t)))
+        return d.malformed("submission_identity");
+    if (p.scanner_bounds.hash == 0 || p.scanner_lut.hash == 0 || p.scanner_lut.density_bounds_hash != p.scanner_bounds.hash)
+        return d.malformed("scanner_bounds.identity");
+    return true;
+}
+
+bool rect_inside(const FjRect& inner, const FjRect& outer) {
+    return inner.x1 >= outer.x1 && inner.y1 >= outer.y1 && inner.x2 <= outer.x2 && inner.y2 <= outer.y2;
+}
+
+bool validate_frame(const FjFrame& f, Diag& d) {
+    const FjImage& s = f.source;
+    const FjImage& o = f.destination;
+    if (s.depth != FJ_DEPTH_FLOAT32 || o.depth != FJ_DEPTH_FLOAT32) return d.unsupported("image depth (float32 only)");
+    if ((s.components != FJ_COMPONENTS_RGB && s.components != FJ_COMPONENTS_RGBA) || o.components != s.components)
+        return d.unsupported("image components (RGB or RGBA, equal on both images)");
+    const FjRect& w = f.render_window;
+    if (w.x2 <= w.x1 || w.y2 <= w.y1) return d.unsupported("empty render window");
+    if (!rect_inside(w, s.bounds) || !rect_inside(w, o.bounds)) return d.unsupported("render window outside image bounds");
+    const long long pixel_bytes = 4LL * s.components;
+    if (s.row_bytes < pixel_bytes * (s.bounds.x2 - s.bounds.x1) || o.row_bytes < pixel_bytes * (o.bounds.x2 - o.bounds.x1) ||
+        s.row_bytes % 4 != 0 || o.row_bytes % 4 != 0)
+        return d.unsupported("image row_bytes");
+    if (s.components == 4 && ((s.address | o.address | static_cast<uintptr_t>(s.row_bytes) |
+                               static_cast<uintptr_t>(o.row_bytes)) & 15u) != 0)
+        return d.unsupported("RGBA images must be 16-byte aligned");
+    return true;
+}
+
+// Families this build renders; everything else is rejected loudly.
+bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
+    if (p.film_exposure.method == FJ_RAW_MALLETT_2019) return d.unsupported("Mallett 2019 reconstruction");
+    if (p.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE) return d.unsupported("auto exposure metering");
+    if (p.scanner_post.flags & FJ_POST_GLARE) return d.unsupported("scanner glare");
+    if (p.scanner_post.hash != 0 && p.scanner_post.lens_blur_sigma_px > 0.0f) return d.unsupported("scanner lens blur");
+    if (p.scanner_post.hash != 0 && p.scanner_post.unsharp_amount != 0.0f && p.scanner_post.unsharp_sigma_px > 0.0f)
+        return d.unsupported("scanner unsharp mask");
+    if (p.optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_SCATTER_HALATION |
+                          FJ_OPTICS_HIGHLIGHT_BOOST))
+        return d.unsupported("spatial optics");
+    if (p.spatial_dir.component_count != 0 || p.spatial_dir.hash != 0) return d.unsupported("spatial DIR");
+    if (p.grain.flags & FJ_GRAIN_ACTIVE) return d.unsupported("visual grain");
+    if (p.effects.flags != 0 || p.effects.hash != 0) return d.unsupported("film/gate effects");
+    (void)f;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// packing
+// ---------------------------------------------------------------------------
+void mat_mul(const float* a, const float* b, float* out) {  // out = a * b (double accumulation)
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += static_cast<double>(a[r * 3 + k]) * b[k * 3 + c];
+            out[r * 3 + c] = static_cast<float>(s);
+        }
+}
+
+// Film curves [x | yB | yG | yR | cxB | cxG | cxR] (JuicerCudaResources.cpp:3724-3842).
+std::vector<float> pack_film_curves(const FjPreparedHostData& p) {
+    const FjFilmDevelopment& dv = p.film_development;
+    const size_t n = dv.log_exposure.count;
+    const bool dir = p.dir_couplers.mode != FJ_DIR_INACTIVE;
+    std::vector<float> out((dir ? 7 : 4) * n);
+    std::memcpy(out.data(), dv.log_exposure.data, n * sizeof(float));
+    for (size_t i = 0; i < n; ++i)
+        for (int b = 0; b < 3; ++b) out[(1 + b) * n + i] = dv.density_rgb.data[i * 3 + (2 - b)];
+    if (dir)
+        for (int b = 0; b < 3; ++b) std::memcpy(&out[(4 + b) * n], p.dir_couplers.compensated_axes_rgb[2 - b].data, n * sizeof(float));
+    return out;
+}
+
+// Compacted print spectra (JuicerCudaDeviceHelpers.cuh:866-933): wavelengths
+// whose transmitted light is NaN for every density (NaN extinction, baseline
+// or illuminant) or whose three sensitivities are NaN contribute nothing and
+// are dropped; the -log2(10) factor and the illuminant are folded in.
+std::vector<float> pack_print_spectra(const FjPrint& pr, int& k_out) {
+    std::vector<float> out;
+    out.reserve(81 * 8);
+    const double kNegLog2_10 = -3.32192809488736234787;
+    int k = 0;
+    for (int i = 0; i < 81; ++i) {
+        const float* e = pr.film_density_cmy.data + i * 3;
+        const float* s = pr.sensitivity_cmy.data + i * 3;
+        const float base = pr.film_base_density.data[i];
+        const float ill = pr.main_illuminant.data[i];
+        const bool any_sens = !std::isnan(s[0]) || !std::isnan(s[1]) || !std::isnan(s[2]);
+        if (!any_sens || std::isnan(e[0]) || std::isnan(e[1]) || std::isnan(e[2]) || std::isnan(base) || std::isnan(ill))
+            continue;
+        out.push_back(static_cast<float>(e[0] * kNegLog2_10));
+        out.push_back(static_cast<float>(e[1] * kNegLog2_10));
+        out.push_back(static_cast<float>(e[2] * kNegLog2_10));
+        out.push_back(static_cast<float>(base * kNegLog2_10));
+        for (int c = 0; c < 3; ++c) out.push_back(std::isnan(s[c]) ? 0.0f : static_cast<float>(double(s[c]) * ill));
+        out.push_back(0.0f);
+        ++k;
+    }
+    k_out = k;
+    if (out.empty()) out.resize(8, 0.0f);
+    return out;
+}
+
+std::vector<float> pack_print_curves(const FjPrint& pr) {
+    const size_t n = pr.log_exposure.count;
+    std::vector<float> out(4 * n);
+    std::memcpy(out.data(), pr.log_exposure.data, n * sizeof(float));
+    for (size_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) out[(1 + c) * n + i] = pr.density_cmy.data[i * 3 + c];
+    return out;
+}
+
+struct PackedLut {
+    std::vector<float> nodes;  // 12 floats per node
+    std::vector<float> cells;  // 8 floats per cell
+    int res = 0;
+};
+
+bool pack_scanner_lut(const FjPreparedHostData& p, PackedLut& out, std::string& diag) {
+    const bool print = is_print_route(p.route);
+    ScannerLutInput in{};
+    const FjScannerSpectra& s = p.scanner_spectra;
+    in.eps_c = s.dye_c.data;
+    in.eps_m = s.dye_m.data;
+    in.eps_y = s.dye_y.data;
+    in.weighted_x = s.weighted_x.data;
+    in.weighted_y = s.weighted_y.data;
+    in.weighted_z = s.weighted_z.data;
+    in.baseline = s.base_density.count ? s.base_density.data : nullptr;
+    in.sample_count = 81;
+    in.inv_yn = s.inverse_y_normalization;
+    // Scanner medium and range as packed by JuicerCudaResources.cpp:3914-3931.
+    for (int c = 0; c < 3; ++c) {
+        in.range_min_cmy[c] = print ? p.scanner_bounds.min_cmy[c] : -p.scanner_bounds.min_cmy[c];
+        in.inverse_span_cmy[c] = p.scanner_bounds.inv_span_cmy[c];
+    }
+    in.medium = print ? 1 : 0;
+    in.resolution = static_cast<int>(p.scanner_lut.resolution);
+    ScannerLut lut;
+    if (!build_scanner_lut(in, lut, diag)) return false;
+    const int r = lut.resolution;
+    const size_t nodes = size_t(r) * r * r;
+    out.res = r;
+    out.nodes.resize(nodes * 12);
+    for (size_t i = 0; i < nodes; ++i) {
+        float* d = &out.nodes[i * 12];
+        for (int o = 0; o < 3; ++o) {
+            d[o] = lut.log2_xyz[i * 3 + o];
+            d[3 + o] = lut.slope_c[i * 3 + o];
+            d[6 + o] = lut.slope_m[i * 3 + o];
+            d[9 + o] = lut.slope_y[i * 3 + o];
+        }
+    }
+    const size_t cells = size_t(r - 1) * (r - 1) * (r - 1);
+    out.cells.assign(cells * 8, 0.0f);
+    for (size_t i = 0; i < cells; ++i) {
+        float* d = &out.cells[i * 8];
+        for (int o = 0; o < 3; ++o) {
+            d[o] = lut.cell_min[i * 3 + o];
+            d[3 + o] = lut.cell_max[i * 3 + o];
+        }
+    }
+    return true;
+}
+
+uint64_t span_hash(const FjFloatSpan& s, uint64_t seed) {
+    return s.count ? hash_bytes(s.data, s.count * sizeof(float), seed) : hash_bytes(nullptr, 0, seed);
+}
+
+// ---------------------------------------------------------------------------
+// frame parameter assembly
+// ---------------------------------------------------------------------------
+enum TableKind : uint64_t { kTcLut = 1, kFilmCurves, kPrintSpec, kPrintCurves, kScanNodes, kScanCells, kGamut };
+
+uint64_t table_key(TableKind kind, uint64_t content) { return hash_bytes(&content, sizeof(content), 0xF11F0000ull + kind); }
+
+hipError_t assemble(DeviceState& dev, const FjPreparedHostData& p, PixelParams& pp, Diag& d, bool& ok) {
+    ok = false;
+    const FjFilmExposure& ex = p.film_exposure;
+    // Exposure: one input matrix (adaptation folded in) and one scale.
+    if (ex.flags & FJ_FILM_ADAPT_INPUT) {
+        mat_mul(ex.input_xyz_adapt, ex.input_rgb_to_xyz, pp.ex.rgb_to_xyz);
+    } else {
+        std::memcpy(pp.ex.rgb_to_xyz, ex.input_rgb_to_xyz, sizeof(pp.ex.rgb_to_xyz));
+    }
+    pp.ex.decode_cctf = (ex.flags & FJ_FILM_DECODE_CCTF) && (ex.input_color_space == FJ_INPUT_BT2020 ||
+                                                          ex.input_color_space == FJ_INPUT_SRGB_REC709)
+                            ? static_cast<int>(ex.input_color_space)
+                            : 0;
+    // JuicerCudaFilmPayloads.cpp:102-112 validates the manual scale.
+    const float manual = std::exp2(ex.manual_exposure_ev);
+    if (!std::isfinite(ex.manual_exposure_ev) || !std::isfinite(manual) || !(manual > 0.0f) ||
+        !std::isfinite(ex.route_correction_scale) || !(ex.route_correction_scale > 0.0f)) {
+        d.malformed("film_exposure_scale");
+        return hipSuccess;
+    }
+    float scale = manual * ex.route_correction_scale;
+    pp.ex.scale = std::isfinite(scale) && scale > 0.0f ? scale : 1.0f;
+    const void* ptr = nullptr;
+    const uint64_t tc_id = ex.tc_lut_hash ? ex.tc_lut_hash : span_hash(ex.tc_lut_rgba, 11);
+    hipError_t e = dev.table(table_key(kTcLut, tc_id), ex.tc_lut_rgba.data, ex.tc_lut_rgba.count * sizeof(float), &ptr);
+    if (e != hipSuccess) return e;
+    pp.ex.tc_lut = static_cast<const float4*>(ptr);
+
+    // Development.
+    const FjFilmDevelopment& dv = p.film_development;
+    std::vector<float> film = pack_film_curves(p);
+    e = dev.table(table_key(kFilmCurves, hash_bytes(film.data(), film.size() * 4, 21)), film.data(), film.size() * 4, &ptr);
+    if (e != hipSuccess) return e;
+    pp.dev.curves = static_cast<const float*>(ptr);
+    pp.dev.n = static_cast<int>(dv.log_exposure.count);
+    for (int b = 0; b < 3; ++b) pp.dev.gamma_bgr[b] = dv.gamma_rgb[2 - b];
+    const FjDirCouplers& dir = p.dir_couplers;
+    pp.dev.dir_mode = static_cast<int>(dir.mode);
+    for (int db = 0; db < 3; ++db) {
+        for (int rb = 0; rb < 3; ++rb) pp.dev.M[db * 3 + rb] = dir.matrix_rgb[(2 - db) * 3 + (2 - rb)];
+        pp.dev.dmax[db] = dir.density_max_rgb[2 - db];
+        pp.dev.dref[db] = dir.density_ref_rgb[2 - db];
+        pp.dev.donor_k[db] = dir.donor_k_rgb[2 - db];
+        pp.dev.recv_cref[db] = dir.receiver_c_ref_rgb[2 - db];
+        pp.dev.recv_kr[db] = dir.receiver_kr_rgb[2 - db];
+    }
+
+    // Print (JuicerCudaResources.cpp:4331-4420 scale checks).
+    pp.print = is_print_route(p.route) ? 1 : 0;
+    if (pp.print) {
+        const FjPrint& pr = p.print;
+        const float route_scale = p.scanner_correction.exposure_scale;
+        if (!std::isfinite(pr.exposure) || pr.exposure < 0.0f || !std::isfinite(pr.normalizer) || !(pr.normalizer > 0.0f) ||
+            !std::isfinite(route_scale) || !(route_scale > 0.0f)) {
+            d.malformed("print_exposure_scale");
+            return hipSuccess;
+        }
+        const bool preflash = (pr.flags & FJ_PRINT_PREFLASH) != 0;
+        if (preflash) {
+            if (!std::isfinite(pr.preflash_exposure) || pr.preflash_exposure < 0.0f) {
+                d.malformed("preflash_exposure");
+                return hipSuccess;
+            }
+            for (float raw : pr.preflash_raw_cmy)
+                if (!std::isfinite(raw) || raw < 0.0f) {
+                    d.malformed("preflash_raw");
+                    return hipSuccess;
+                }
+        }
+        int k = 0;
+        std::vector<float> spec = pack_print_spectra(pr, k);
+        e = dev.table(table_key(kPrintSpec, hash_bytes(spec.data(), spec.size() * 4, 31)), spec.data(), spec.size() * 4, &ptr);
+        if (e != hipSuccess) return e;
+        pp.pr.spec = static_cast<const float4*>(ptr);
+        pp.pr.k = k;
+        pp.pr.normalizer = pr.normalizer;
+        const float pf = preflash && pr.preflash_exposure > 0.0f ? pr.preflash_exposure : 0.0f;
+        for (int c = 0; c < 3; ++c) pp.pr.preflash[c] = pf > 0.0f ? pr.preflash_raw_cmy[c] * pf : 0.0f;
+        pp.pr.outer_scale = pr.exposure * route_scale;
+        std::vector<float> pc = pack_print_curves(pr);
+        e = dev.table(table_key(kPrintCurves, hash_bytes(pc.data(), pc.size() * 4, 41)), pc.data(), pc.size() * 4, &ptr);
+        if (e != hipSuccess) return e;
+        pp.pr.curves = static_cast<const float*>(ptr);
+        pp.pr.n = static_cast<int>(pr.log_exposure.count);
+    }
+
+    // Scanner LUT: built natively, cached by the prepared scanner identity.
+    uint64_t lut_id = p.scanner_lut.hash;
+    {
+        const FjScannerSpectra& s = p.scanner_spectra;
+        uint64_t h = lut_id;
+        for (const FjFloatSpan* sp : {&s.dye_c, &s.dye_m, &s.dye_y, &s.weighted_x, &s.weighted_y, &s.weighted_z, &s.base_density})
+            h = span_hash(*sp, h);
+        h = hash_bytes(&p.scanner_bounds, sizeof(float) * 6, h);
+        const uint32_t extra[3] = {p.scanner_lut.resolution, static_cast<uint32_t>(is_print_route(p.route)),
+                                   *reinterpret_cast<const uint32_t*>(&s.inverse_y_normalization)};
+        lut_id = hash_bytes(extra, sizeof(extra), h);
+    }
+    const uint64_t node_key = table_key(kScanNodes, lut_id), cell_key = table_key(kScanCells, lut_id);
+    if (dev.tables.count(node_key) == 0 || dev.tables.count(cell_key) == 0) {
+        PackedLut lut;
+        std::string diag;
+        if (!pack_scanner_lut(p, lut, diag)) {
+            d.text = "PreparationFailure " + diag;
+            return hipSuccess;
+        }
+        e = dev.table(node_key, lut.nodes.data(), lut.nodes.size() * 4, &ptr);
+        if (e != hipSuccess) return e;
+        e = dev.table(cell_key, lut.cells.data(), lut.cells.size() * 4, &ptr);
+        if (e != hipSuccess) return e;
+    }
+    e = dev.table(node_key, nullptr, 0, &ptr);
+    if (e != hipSuccess) return e;
+    pp.sc.nodes = static_cast<const float4*>(ptr);
+    e = dev.table(cell_key, nullptr, 0, &ptr);
+    if (e != hipSuccess) return e;
+    pp.sc.cells = static_cast<const float4*>(ptr);
+    pp.sc.res = static_cast<int>(p.scanner_lut.resolution);
+    for (int c = 0; c < 3; ++c) {
+        pp.sc.dmin[c] = p.scanner_bounds.min_cmy[c];
+        pp.sc.inv_span[c] = p.scanner_bounds.inv_span_cmy[c];
+    }
+    pp.sc.correction = (p.scanner_correction.flags & FJ_CORRECTION_ACTIVE) != 0;
+    pp.sc.corr_slope = p.scanner_correction.xyz_slope;
+    pp.sc.corr_offset = p.scanner_correction.xyz_offset;
+    const FjOutputColor& oc = p.output_color;
+    mat_mul(oc.xyz_to_rgb, oc.cat02, pp.sc.xyz_to_rgb);
+
+    // Colour.
+    pp.co.gamut = (oc.flags & FJ_COLOR_GAMUT_COMPRESSION) != 0;
+    if (pp.co.gamut) {
+        const uint64_t g_id = oc.gamut_table_hash ? oc.gamut_table_hash : span_hash(oc.gamut_cmax, 51);
+        e = dev.table(table_key(kGamut, g_id), oc.gamut_cmax.data, oc.gamut_cmax.count * sizeof(float), &ptr);
+        if (e != hipSuccess) return e;
+        pp.co.cmax = static_cast<const float*>(ptr);
+    }
+    std::memcpy(pp.co.native_to_xyz, oc.native_rgb_to_d65_xyz, sizeof(pp.co.native_to_xyz));
+    std::memcpy(pp.co.xyz_to_native, oc.d65_xyz_to_native_rgb, sizeof(pp.co.xyz_to_native));
+    std::memcpy(pp.co.lknee, oc.lightness_knee, sizeof(pp.co.lknee));
+    std::memcpy(pp.co.cknee, oc.chroma_knee, sizeof(pp.co.cknee));
+    pp.co.encode = (oc.flags & FJ_COLOR_ENCODE_CCTF) != 0;
+    pp.co.matrix = (oc.flags & FJ_COLOR_INPUT_IS_OUTPUT_SPACE) == 0;
+    std::memcpy(pp.co.to_output, oc.encoding.dwg_to_output, sizeof(pp.co.to_output));
+    pp.co.cctf = static_cast<int>(oc.encoding.cctf_kind);
+    pp.co.gamma = oc.encoding.gamma;
+    pp.co.a = oc.encoding.a;
+    pp.co.b = oc.encoding.b;
+    pp.co.c = oc.encoding.c;
+    pp.co.d = oc.encoding.d;
+    pp.co.cutoff = oc.encoding.linear_cutoff;
+    ok = true;
+    return hipSuccess;
+}
+
+FjStatus status_of(uint32_t category, uint32_t api = FJ_API_NONE, int32_t code = 0) {
+    FjStatus s{category, api, api == FJ_API_NONE ? 0 : code};
+    return s;
+}
+
+void put_error(FjErrorBuffer* error, const std::string& text) {
+    if (!error || error->capacity == 0 || !error->data) {
+        if (error) error->length = 0;
+        return;
+    }
+    const size_t n = std::min(text.size(), error->capacity - 1);
+    std::memcpy(error->data, text.data(), n);
+    error->data[n] = '\0';
+    error->length = n;
+}
+
+FjRenderOutcome outcome_hip(hipError_t e, const char* where, FjErrorBuffer* error) {
+    FjRenderOutcome o{};
+    o.status = status_of(e == hipErrorOutOfMemory ? FJ_STATUS_ALLOCATION_FAILURE : FJ_STATUS_CUDA_FAILURE,
+                         FJ_API_CUDA_RUNTIME, static_cast<int32_t>(e));
+    put_error(error, std::string("HIP failure in ") + where + ": " + hipGetErrorString(e));
+    return o;
+}
+
+FjRenderOutcome outcome_diag(uint32_t category, const std::string& text, FjErrorBuffer* error) {
+    FjRenderOutcome o{};
+    o.status = status_of(category);
+    put_error(error, text);
+    return o;
+}
+
+bool abort_requested(const FjAbortCallback& cb, bool& invalid) {
+    invalid = false;
+    if (!cb.query) return false;
+    const uint32_t r = cb.query(cb.user);
+    if (r == FJ_ABORT_CONTINUE) return false;
+    if (r != FJ_ABORT_REQUESTED) invalid = true;
+    return true;
+}
+
+}  // namespace
+
+FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSubmission& submission,
+                             const FjPreparedHostData& prepared, FjAbortCallback abort_callback, FjErrorBuffer* error) {
+    Diag d;
+    if (!validate_prepared(prepared, d) || !validate_binding(prepared, frame, submission, d))
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
+    if (!validate_frame(frame, d) || !check_supported(prepared, frame, d))
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
+    bool invalid = false;
+    if (abort_requested(abort_callback, invalid))
+        return outcome_diag(invalid ? FJ_STATUS_INTERNAL_FAILURE : FJ_STATUS_CANCELLED,
+                            invalid ? "abort query returned an invalid value" : "render aborted", error);
+
+    PixelParams pp{};
+    bool ok = false;
+    hipError_t e = assemble(dev, prepared, pp, d, ok);
+    if (e != hipSuccess) return outcome_hip(e, "table upload", error);
+    if (!ok) return outcome_diag(d.text.rfind("Preparation", 0) == 0 ? FJ_STATUS_PREPARATION_FAILURE : FJ_STATUS_UNSUPPORTED_INPUT,
+                                 d.text, error);
+
+    const FjRect& w = frame.render_window;
+    const FjImage& s = frame.source;
+    const FjImage& o = frame.destination;
+    const int comps = static_cast<int>(s.components);
+    pp.io.src = reinterpret_cast<const char*>(s.address) + (w.y1 - s.bounds.y1) * s.row_bytes +
+                static_cast<ptrdiff_t>(w.x1 - s.bounds.x1) * comps * 4;
+    pp.io.dst = reinterpret_cast<char*>(o.address) + (w.y1 - o.bounds.y1) * o.row_bytes +
+                static_cast<ptrdiff_t>(w.x1 - o.bounds.x1) * comps * 4;
+    pp.io.src_row_bytes = s.row_bytes;
+    pp.io.dst_row_bytes = o.row_bytes;
+    pp.io.width = w.x2 - w.x1;
+    pp.io.height = w.y2 - w.y1;
+    pp.io.comps = comps;
+
+    const bool has_stream = (frame.flags & FJ_FRAME_STREAM_PRESENT) != 0;
+    hipStream_t stream = has_stream ? reinterpret_cast<hipStream_t>(frame.stream) : nullptr;
+
+    // Earlier asynchronous frames may have reported DIR failures.
+    unsigned int earlier = dev.drain_errors(false);
+
+    const int slot = dev.error_next;
+    dev.error_next = (dev.error_next + 1) % DeviceState::kRing;
+    if (dev.error_pending[slot]) {
+        e = hipEventSynchronize(dev.error_events[slot]);
+        if (e != hipSuccess) return outcome_hip(e, "error ring", error);
+        earlier |= dev.error_host[slot];
+        dev.error_pending[slot] = false;
+    }
+    pp.error_word = dev.error_words + slot;
+    e = launch_clear_word(pp.error_word, stream);
+    if (e != hipSuccess) return outcome_hip(e, "clear error word", error);
+    e = launch_fused_pixel(pp, dev.compute_units * 8, stream);
+    if (e != hipSuccess) return outcome_hip(e, "fused pixel kernel", error);
+    e = hipMemcpyAsync(dev.error_host + slot, pp.error_word, sizeof(unsigned int), hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return outcome_hip(e, "error readback", error);
+    e = hipEventRecord(dev.error_events[slot], stream);
+    if (e != hipSuccess) return outcome_hip(e, "error event", error);
+    dev.error_pending[slot] = true;
+
+    FjRenderOutcome out{};
+    out.status = status_of(FJ_STATUS_SUCCESS);
+    if (!has_stream) {
+        e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return outcome_hip(e, "stream synchronize", error);
+        earlier |= dev.drain_errors(false);
+    }
+    if (earlier & (1u << 8)) out.flags |= FJ_RENDER_DEFERRED_DIR_ERROR;
+    return out;
+}
+
+}  // namespace fjx

@@ -1,0 +1,58 @@
+// fj_plan.h — per-device state and the frame executor.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <list>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/fj/juicer_cuda_api.h"
+#include "fj_device.h"
+
+namespace fjx {
+
+// Device tables packed from prepared spans, cached by content identity.
+struct DeviceTable {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct DeviceState {
+    int device = -1;
+    int compute_units = 0;
+    std::unordered_map<uint64_t, DeviceTable> tables;
+    std::list<uint64_t> lru;  // most recent at front
+    size_t table_bytes = 0;
+
+    // Error word ring: each frame clears one device word, its kernels OR scan
+    // and DIR failure bits into it, and the word is copied to pinned memory.
+    static constexpr int kRing = 8;
+    unsigned int* error_words = nullptr;       // device [kRing]
+    unsigned int* error_host = nullptr;        // pinned [kRing]
+    hipEvent_t error_events[kRing] = {};
+    bool error_pending[kRing] = {};
+    int error_next = 0;
+
+    // Frame workspace for staged (spatial) schedules.
+    void* workspace = nullptr;
+    size_t workspace_bytes = 0;
+
+    hipError_t init(int dev);
+    hipError_t release();
+    // Returns the device copy of `bytes` host bytes under `key`, uploading once.
+    hipError_t table(uint64_t key, const void* host, size_t bytes, const void** out);
+    hipError_t ensure_workspace(size_t bytes);
+    // Collects completed error words of earlier frames; returns OR of their bits.
+    unsigned int drain_errors(bool wait_all);
+};
+
+FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSubmission& submission,
+                             const FjPreparedHostData& prepared, FjAbortCallback abort_callback,
+                             FjErrorBuffer* error);
+
+uint64_t hash_bytes(const void* data, size_t bytes, uint64_t seed);
+
+}  // namespace fjx

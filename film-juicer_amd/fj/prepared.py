@@ -1,0 +1,226 @@
+"""Builds FjPreparedHostData records from prepared-product case files.
+
+A case file pair (`<case>.json` scalars + `<case>.npz` arrays, large tables in
+`shared_tables.npz`) holds the complete prepared products of one recipe, as the
+Rust preparation layer hands them to the native boundary. `Prepared` owns the
+numpy storage every span borrows, so a record stays valid while the object is
+alive. Field mapping follows the reference's projection of prepared state into
+`FjPreparedHostData` (native/juicer_cuda_prepared.cpp:267-374).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+DEFAULT_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                           "tests", "golden", "prepared")
+
+
+def _load_case(name: str, directory: str):
+    with open(os.path.join(directory, f"{name}.json")) as fh:
+        meta = json.load(fh)
+    small = np.load(os.path.join(directory, f"{name}.npz"))
+    arrays = {k: small[k] for k in small.files}
+    if meta["large"]:
+        shared = np.load(os.path.join(directory, "shared_tables.npz"))
+        for k, key in meta["large"].items():
+            arrays[k] = shared[key]
+    return meta["scalars"], arrays
+
+
+@dataclass
+class Prepared:
+    """An FjPreparedHostData plus the storage its spans borrow."""
+
+    case: str
+    scalars: dict
+    arrays: dict
+    record: abi.FjPreparedHostData = field(default_factory=abi.FjPreparedHostData)
+    keep: list = field(default_factory=list)
+
+    # -- span helpers ---------------------------------------------------
+    def fspan(self, values) -> abi.FjFloatSpan:
+        if values is None or len(values) == 0:
+            return abi.FjFloatSpan(None, 0)
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        self.keep.append(a)
+        return abi.FjFloatSpan(a.ctypes.data_as(C.POINTER(C.c_float)), a.size)
+
+    def bspan(self, values) -> abi.FjByteSpan:
+        if values is None or len(values) == 0:
+            return abi.FjByteSpan(None, 0)
+        a = np.ascontiguousarray(values, dtype=np.uint8)
+        self.keep.append(a)
+        return abi.FjByteSpan(a.ctypes.data_as(C.POINTER(C.c_uint8)), a.size)
+
+    def ptr(self):
+        return C.byref(self.record)
+
+    @property
+    def is_print(self) -> bool:
+        return self.record.route in (abi.FJ_ROUTE_NEGATIVE_PRINT, abi.FJ_ROUTE_POSITIVE_PRINT)
+
+
+def _set(dst, values):
+    for i, v in enumerate(np.asarray(values).ravel()):
+        dst[i] = float(v)
+
+
+def load(case: str, directory: str = DEFAULT_DIR) -> Prepared:
+    s, a = _load_case(case, directory)
+    p = Prepared(case, s, a)
+    r = p.record
+    route = int(s["route"])
+    is_print = route in (abi.FJ_ROUTE_NEGATIVE_PRINT, abi.FJ_ROUTE_POSITIVE_PRINT)
+    r.route = route
+    r.capture_polarity = int(s["polarity"])
+    for attr, key in (("film_profile_key", "film_profile_key"), ("print_profile_key", "print_profile_key")):
+        text = s.get(key, "").encode()
+        buf = C.create_string_buffer(text, len(text) + 1)
+        p.keep.append(buf)
+        setattr(r, attr, abi.FjStringView(C.cast(buf, C.c_char_p) if text else None, len(text)))
+    r.film_profile_asset_version = int(s["film_profile_asset_version"])
+    r.print_profile_asset_version = int(s["print_profile_asset_version"])
+    r.recipe_hash = int(s["recipe_hash"])
+
+    fe = r.film_exposure
+    fe.input_color_space = int(s["film_exposure.input_color_space"])
+    fe.method = int(s["film_exposure.method"])
+    fe.flags = int(s["film_exposure.flags"]) & ~abi.FJ_FILM_AUTO_EXPOSURE
+    fe.manual_exposure_ev = float(s["film_exposure.manual_exposure_ev"])
+    fe.route_correction_scale = float(s["film_exposure.route_correction_scale"])
+    fe.mallett_green_midgray_scale = float(s["film_exposure.mallett_green_midgray_scale"])
+    _set(fe.input_rgb_to_xyz, a["film_exposure.input_rgb_to_xyz"])
+    _set(fe.input_xyz_adapt, a["film_exposure.input_xyz_adapt"])
+    _set(fe.xyz_to_linear_srgb, a["film_exposure.xyz_to_linear_srgb"])
+    fe.sensitivity_rgb = p.fspan(a["film_exposure.sensitivity_rgb"])
+    fe.tc_lut_rgba = p.fspan(a["film_exposure.tc_lut_rgba"])
+    fe.mallett_illuminant = p.fspan(a["film_exposure.mallett_illuminant"])
+    fe.mallett_basis = p.fspan(a["film_exposure.mallett_basis"])
+    fe.sensitivity_hash = int(s["film_exposure.sensitivity_hash"])
+    fe.tc_lut_hash = int(s["film_exposure.tc_lut_hash"])
+    fe.hash = int(s["film_exposure.hash"])
+
+    fd = r.film_development
+    fd.log_exposure = p.fspan(a["film_development.log_exposure"])
+    fd.density_rgb = p.fspan(a["film_development.density_rgb"])
+    for layer in range(3):
+        for ch in range(3):
+            fd.density_layers[layer][ch] = p.fspan(a[f"film_development.density_layers[{layer * 3 + ch}]"])
+    _set(fd.gamma_rgb, a["film_development.gamma_rgb"])
+    fd.density_curves_hash = int(s["film_development.density_curves_hash"])
+    fd.density_layers_hash = int(s["film_development.density_layers_hash"])
+    fd.hash = int(s["film_development.hash"])
+
+    dc = r.dir_couplers
+    dc.mode = int(s["dir_recipe.mode"])
+    if dc.mode != abi.FJ_DIR_INACTIVE:
+        _set(dc.matrix_rgb, a["dir_recipe.matrix_rgb"])
+        _set(dc.density_max_rgb, a["dir_recipe.density_max_rgb"])
+        _set(dc.density_ref_rgb, a["dir_recipe.density_ref_rgb"])
+        _set(dc.donor_k_rgb, a["dir_recipe.donor_k_rgb"])
+        _set(dc.receiver_c_ref_rgb, a["dir_recipe.receiver_c_ref_rgb"])
+        _set(dc.receiver_kr_rgb, a["dir_recipe.receiver_kr_rgb"])
+        for ch in range(3):
+            dc.compensated_axes_rgb[ch] = p.fspan(a[f"dir_recipe.compensated_axes_rgb[{ch}]"])
+        dc.compensated_axes_hash = int(s["dir_recipe.compensated_axes_hash"])
+        dc.hash = int(s["dir_recipe.hash"])
+
+    for dst, key in ((r.scanner_bounds, "scanner_bounds"), (r.enlarger_film_bounds, "enlarger_film_bounds")):
+        _set(dst.min_cmy, a[f"{key}.minimum_cmy"])
+        _set(dst.inv_span_cmy, a[f"{key}.inv_span_cmy"])
+        dst.hash = int(s[f"{key}.hash"])
+
+    ss = r.scanner_spectra
+    ss.dye_c = p.fspan(a["scanner_tables.dyes_cmy[0]"])
+    ss.dye_m = p.fspan(a["scanner_tables.dyes_cmy[1]"])
+    ss.dye_y = p.fspan(a["scanner_tables.dyes_cmy[2]"])
+    ss.weighted_x = p.fspan(a["scanner_tables.weighted_xyz[0]"])
+    ss.weighted_y = p.fspan(a["scanner_tables.weighted_xyz[1]"])
+    ss.weighted_z = p.fspan(a["scanner_tables.weighted_xyz[2]"])
+    if int(s["scanner_tables.has_baseline"]):
+        ss.base_density = p.fspan(a["scanner_tables.baseline_min"])
+    ss.inverse_y_normalization = float(s["scanner_tables.inv_yn"])
+
+    r.scanner_lut.density_bounds_hash = int(s["scanner_lut.density_bounds_hash"])
+    r.scanner_lut.resolution = int(s["scanner_lut.resolution"])
+    r.scanner_lut.hash = int(s["scanner_lut.hash"])
+
+    sc = r.scanner_correction
+    sc.flags = abi.FJ_CORRECTION_ACTIVE if int(s["scanner_correction.active"]) else 0
+    sc.xyz_slope = float(s["scanner_correction.xyz_slope"])
+    sc.xyz_offset = float(s["scanner_correction.xyz_offset"])
+    sc.exposure_scale = float(s["scanner_correction.exposure_scale"])
+
+    sp = r.scanner_post
+    sp.flags = 0  # scanner glare is not part of the render path built here
+    sp.lens_blur_sigma_px = float(s["scanner_post_effects.lens_blur_sigma_px"])
+    sp.lens_blur_radius = int(s["scanner_post_effects.lens_blur_radius"])
+    sp.unsharp_sigma_px = float(s["scanner_post_effects.unsharp_sigma_px"])
+    sp.unsharp_radius = int(s["scanner_post_effects.unsharp_radius"])
+    sp.unsharp_amount = float(s["scanner_post_effects.unsharp_amount"])
+    sp.hash = int(s["scanner_post_effects.hash"])
+
+    oc = r.output_color
+    for name in ("cat02", "xyz_to_rgb", "illuminant_xyz", "native_rgb_to_d65_xyz",
+                 "d65_xyz_to_native_rgb", "lightness_knee", "chroma_knee"):
+        _set(getattr(oc, name), a[f"output_color.{name}"])
+    oc.color_space = int(s["output_color.color_space"])
+    oc.flags = int(s["output_color.flags"])
+    if oc.flags & abi.FJ_COLOR_GAMUT_COMPRESSION:
+        oc.gamut_cmax = p.fspan(a["output_color.gamut_cmax"])
+    oc.gamut_table_hash = int(s["output_color.gamut_table_hash"])
+    oc.gamut_recipe_hash = int(s["output_color.gamut_recipe_hash"])
+    enc = oc.encoding
+    enc.schema = int(s["output_color.encoding.schema"])
+    enc.cctf_kind = int(s["output_color.encoding.cctf_kind"])
+    for name in ("gamma", "a", "b", "c", "d", "linear_cutoff"):
+        setattr(enc, name, float(s[f"output_color.encoding.{name}"]))
+    _set(enc.dwg_to_output, a["output_color.encoding.dwg_to_output"])
+
+    if is_print:
+        pr = r.print
+        pr.film_density_cmy = p.fspan(a["print.film_density_cmy"])
+        pr.film_base_density = p.fspan(a["print.film_base_density"])
+        pr.sensitivity_cmy = p.fspan(a["print.sensitivity_cmy"])
+        pr.log_exposure = p.fspan(a["print.log_exposure"])
+        pr.density_cmy = p.fspan(a["print.density_cmy"])
+        pr.main_illuminant = p.fspan(a["print.main_illuminant"])
+        pr.preflash_illuminant = p.fspan(a["print.preflash_illuminant"])
+        pr.flags = int(s["print.flags"])
+        _set(pr.main_cc_cmy, a["print.main_cc_cmy"])
+        _set(pr.preflash_cc_cmy, a["print.preflash_cc_cmy"])
+        pr.exposure = float(s["print.exposure"])
+        pr.preflash_exposure = float(s["print.preflash_exposure"])
+        _set(pr.preflash_raw_cmy, a["print.preflash_raw_cmy"])
+        pr.normalizer = float(s["print.normalizer"])
+        for name in ("film_density_hash", "profile_tables_hash", "main_illuminant_hash",
+                     "preflash_illuminant_hash", "preflash_raw_hash", "balance_hash", "hash"):
+            setattr(pr, name, int(s[f"print.{name}"]))
+    return p
+
+
+def simple_view(p: Prepared, *, encode: bool = True, gamut: bool = True, unsharp: bool = False,
+                dir_spatial: bool = False) -> Prepared:
+    """Strips per-frame spatial families so only the per-pixel chain runs."""
+    r = p.record
+    r.optics = abi.FjOptics()
+    r.spatial_dir = abi.FjSpatialDir()
+    r.grain = abi.FjVisualGrain()
+    r.noise = abi.FjStaticNoise()
+    r.effects = abi.FjFilmEffects()
+    if not unsharp:
+        r.scanner_post.unsharp_amount = 0.0
+        r.scanner_post.hash = 0
+    if not encode:
+        r.output_color.flags &= ~abi.FJ_COLOR_ENCODE_CCTF
+    if not gamut:
+        r.output_color.flags &= ~abi.FJ_COLOR_GAMUT_COMPRESSION
+        r.output_color.gamut_cmax = abi.FjFloatSpan(None, 0)
+    return p

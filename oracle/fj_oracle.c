@@ -1,0 +1,810 @@
+/*
+ * fj_oracle.c — FP64 CPU oracle of the per-frame render path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see fj_oracle.h). Double-precision restatement of
+ * the reference method; citations are relative to the reference repository.
+ *
+ * Stage order follows execute_print_schedule / execute_direct_schedule
+ * (src/Cuda/JuicerCudaExecutor.cpp:2561-3141): camera exposure -> film
+ * development (+DIR) -> [print exposure -> print development] -> scanner LUT
+ * -> scanner colour -> output gamut -> [post effects] -> output encoding.
+ * Every stage works on whole double planes so spatial operators can sit
+ * between stages exactly where the reference schedule puts them.
+ */
+#include "fj_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define LOG2_10 3.32192809488736234787
+#define NSPEC 81
+
+/* ------------------------------------------------------------------------ */
+/* scanner LUT                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* scanner/lut.rs:108-138 (Fritsch–Carlson harmonic-mean PCHIP slopes with the
+ * one-sided three-point endpoint rule). */
+void fjo_pchip_slopes(const double* v, int n, double* out) {
+    double d[512];
+    double* deltas = n - 1 <= 512 ? d : (double*)malloc(sizeof(double) * (size_t)(n - 1));
+    for (int i = 0; i < n - 1; ++i) deltas[i] = v[i + 1] - v[i];
+    if (n == 2) {
+        out[0] = out[1] = deltas[0];
+    } else {
+        for (int side = 0; side < 2; ++side) {
+            const double a = side == 0 ? deltas[0] : deltas[n - 2];
+            const double b = side == 0 ? deltas[1] : deltas[n - 3];
+            const double s = 0.5 * (3.0 * a - b);
+            double e;
+            if (s * a <= 0.0) e = 0.0;
+            else if (a * b < 0.0 && fabs(s) > fabs(3.0 * a)) e = 3.0 * a;
+            else e = s;
+            out[side == 0 ? 0 : n - 1] = e;
+        }
+        for (int i = 1; i < n - 1; ++i) {
+            const double p = deltas[i - 1], q = deltas[i];
+            out[i] = (p == 0.0 || q == 0.0 || p * q <= 0.0) ? 0.0 : 2.0 * p * q / (p + q);
+        }
+    }
+    if (deltas != d) free(deltas);
+}
+
+/* scanner/lut.rs:140-183: density of a normalized LUT node, 81-λ
+ * transmittance integration against illuminant-weighted CMFs, log10. */
+static void lut_log_xyz(const float* ec, const float* em, const float* ey, const float* wx,
+                        const float* wy, const float* wz, const float* base, int k, float inv_yn,
+                        const float rmin[3], const float ispan[3], int medium, const double nrm[3],
+                        double out[3]) {
+    double dens[3];
+    for (int c = 0; c < 3; ++c) {
+        const double scaled = nrm[c] / (double)ispan[c];
+        dens[c] = medium == 0 ? scaled - (double)rmin[c] : scaled + (double)rmin[c];
+    }
+    double xyz[3] = {0.0, 0.0, 0.0};
+    const float* w[3] = {wx, wy, wz};
+    for (int s = 0; s < k; ++s) {
+        const double b = base ? (double)base[s] : 0.0;
+        const double spectral = dens[0] * (double)ec[s] + dens[1] * (double)em[s] +
+                                dens[2] * (double)ey[s] + b;
+        const double t = pow(10.0, -spectral);
+        for (int c = 0; c < 3; ++c) {
+            const double weight = (double)w[c][s];
+            if (isfinite(weight)) {
+                const double prod = t * weight;
+                if (!isnan(prod)) xyz[c] += prod;
+            }
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        const double sum = xyz[c] * (double)inv_yn;
+        const double clamped = isnan(sum) ? 0.0 : (sum > 0.0 ? sum : 0.0);
+        out[c] = log10(clamped + 1e-10);
+    }
+}
+
+int fjo_scanner_lut_build(const float* ec, const float* em, const float* ey, const float* wx,
+                          const float* wy, const float* wz, const float* base, int k,
+                          float inv_yn, const float rmin[3], const float ispan[3], int medium,
+                          int r, float* log2_xyz, float* sc, float* sm, float* sy, float* cmin,
+                          float* cmax, uint64_t* fail_index, uint32_t* fail_channel) {
+    if (r < 2 || !log2_xyz || !sc || !sm || !sy || !cmin || !cmax) return 2;
+    const size_t count = (size_t)r * r * r * 3;
+    double* nodes = (double*)malloc(sizeof(double) * count);
+    if (!nodes) return 2;
+#define IDX(c, m, y, ch) (((((size_t)(c) * r + (m)) * r + (y)) * 3) + (ch))
+    for (int c = 0; c < r; ++c)
+        for (int m = 0; m < r; ++m)
+            for (int y = 0; y < r; ++y) {
+                const double nrm[3] = {(double)c / (r - 1), (double)m / (r - 1), (double)y / (r - 1)};
+                double lx[3] = {NAN, NAN, NAN};
+                if (k > 0)
+                    lut_log_xyz(ec, em, ey, wx, wy, wz, base, k, inv_yn, rmin, ispan, medium, nrm, lx);
+                for (int ch = 0; ch < 3; ++ch) {
+                    if (!isfinite(lx[ch])) {
+                        if (fail_index) *fail_index = ((uint64_t)c * r + m) * r + y;
+                        if (fail_channel) *fail_channel = (uint32_t)ch;
+                        free(nodes);
+                        return 1;
+                    }
+                    const size_t i = IDX(c, m, y, ch);
+                    nodes[i] = lx[ch] * LOG2_10;
+                    log2_xyz[i] = (float)nodes[i];
+                }
+            }
+    double* line = (double*)malloc(sizeof(double) * (size_t)r * 2);
+    double* slopes = line + r;
+    float* outs[3] = {sc, sm, sy};
+    for (int axis = 0; axis < 3; ++axis)
+        for (int a = 0; a < r; ++a)
+            for (int b = 0; b < r; ++b)
+                for (int ch = 0; ch < 3; ++ch) {
+                    for (int i = 0; i < r; ++i) {
+                        const size_t p = axis == 0 ? IDX(i, a, b, ch) : axis == 1 ? IDX(a, i, b, ch) : IDX(a, b, i, ch);
+                        line[i] = nodes[p];
+                    }
+                    fjo_pchip_slopes(line, r, slopes);
+                    for (int i = 0; i < r; ++i) {
+                        const size_t p = axis == 0 ? IDX(i, a, b, ch) : axis == 1 ? IDX(a, i, b, ch) : IDX(a, b, i, ch);
+                        outs[axis][p] = (float)slopes[i];
+                    }
+                }
+    free(line);
+    const int s = r - 1;
+    for (int c = 0; c < s; ++c)
+        for (int m = 0; m < s; ++m)
+            for (int y = 0; y < s; ++y)
+                for (int ch = 0; ch < 3; ++ch) {
+                    double lo = nodes[IDX(c, m, y, ch)], hi = lo;
+                    for (int dc = 0; dc < 2; ++dc)
+                        for (int dm = 0; dm < 2; ++dm)
+                            for (int dy = 0; dy < 2; ++dy) {
+                                const double v = nodes[IDX(c + dc, m + dm, y + dy, ch)];
+                                if (v < lo) lo = v;
+                                if (hi < v) hi = v;
+                            }
+                    const size_t i = ((((size_t)c * s + m) * s + y) * 3) + ch;
+                    cmin[i] = (float)lo;
+                    cmax[i] = (float)hi;
+                }
+#undef IDX
+    free(nodes);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* plan                                                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct Curve {
+    const float* x;
+    const float* y; /* strided */
+    int n;
+    int stride;
+} Curve;
+
+struct FjoPlan {
+    FjPreparedHostData p; /* shallow copy; spans must outlive the plan */
+    int print;
+    int res;
+    float* lut; /* one allocation holding the six LUT arrays */
+    float *l2, *sc, *sm, *sy, *cmin, *cmax;
+    double manual_scale;
+};
+
+static void set_message(char* msg, size_t cap, const char* text) {
+    if (msg && cap) {
+        snprintf(msg, cap, "%s", text);
+    }
+}
+
+FjoPlan* fjo_plan_create(const FjPreparedHostData* src, char* msg, size_t cap) {
+    if (!src) return NULL;
+    if (src->film_exposure.method == FJ_RAW_MALLETT_2019) {
+        set_message(msg, cap, "oracle: Mallett reconstruction not restated");
+        return NULL;
+    }
+    if (src->film_exposure.tc_lut_rgba.count != 192u * 192u * 4u) {
+        set_message(msg, cap, "oracle: TC LUT missing");
+        return NULL;
+    }
+    FjoPlan* plan = (FjoPlan*)calloc(1, sizeof(FjoPlan));
+    plan->p = *src;
+    plan->print = src->route == FJ_ROUTE_NEGATIVE_PRINT || src->route == FJ_ROUTE_POSITIVE_PRINT;
+    const int r = (int)src->scanner_lut.resolution;
+    plan->res = r;
+    const size_t nodes = (size_t)r * r * r * 3, cells = (size_t)(r - 1) * (r - 1) * (r - 1) * 3;
+    plan->lut = (float*)malloc(sizeof(float) * (nodes * 4 + cells * 2));
+    plan->l2 = plan->lut;
+    plan->sc = plan->l2 + nodes;
+    plan->sm = plan->sc + nodes;
+    plan->sy = plan->sm + nodes;
+    plan->cmin = plan->sy + nodes;
+    plan->cmax = plan->cmin + cells;
+    /* Medium and range follow JuicerCudaResources.cpp:3914-3931. */
+    float rmin[3], ispan[3];
+    for (int c = 0; c < 3; ++c) {
+        rmin[c] = plan->print ? src->scanner_bounds.min_cmy[c] : -src->scanner_bounds.min_cmy[c];
+        ispan[c] = src->scanner_bounds.inv_span_cmy[c];
+    }
+    const FjScannerSpectra* s = &src->scanner_spectra;
+    uint64_t fi = 0;
+    uint32_t fc = 0;
+    const int rc = fjo_scanner_lut_build(s->dye_c.data, s->dye_m.data, s->dye_y.data, s->weighted_x.data,
+                                         s->weighted_y.data, s->weighted_z.data,
+                                         s->base_density.count ? s->base_density.data : NULL, NSPEC,
+                                         s->inverse_y_normalization, rmin, ispan, plan->print ? 1 : 0, r,
+                                         plan->l2, plan->sc, plan->sm, plan->sy, plan->cmin, plan->cmax, &fi, &fc);
+    if (rc != 0) {
+        set_message(msg, cap, "oracle: scanner LUT build failed");
+        fjo_plan_destroy(plan);
+        return NULL;
+    }
+    /* JuicerCudaFilmPayloads.cpp:102: the manual scale is exp2 of the EV. */
+    plan->manual_scale = exp2((double)src->film_exposure.manual_exposure_ev);
+    return plan;
+}
+
+void fjo_plan_destroy(FjoPlan* plan) {
+    if (!plan) return;
+    free(plan->lut);
+    free(plan);
+}
+
+int fjo_plan_scanner_lut(const FjoPlan* plan, int* res, const float** l2, const float** sc,
+                         const float** sm, const float** sy, const float** cmin, const float** cmax) {
+    if (!plan) return 1;
+    *res = plan->res;
+    *l2 = plan->l2;
+    *sc = plan->sc;
+    *sm = plan->sm;
+    *sy = plan->sy;
+    *cmin = plan->cmin;
+    *cmax = plan->cmax;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* per-pixel helpers                                                        */
+/* ------------------------------------------------------------------------ */
+
+static double fin0(double v) { return isfinite(v) ? v : 0.0; }
+
+static void mat3(const float m[9], const double v[3], double o[3]) {
+    o[0] = (double)m[0] * v[0] + (double)m[1] * v[1] + (double)m[2] * v[2];
+    o[1] = (double)m[3] * v[0] + (double)m[4] * v[1] + (double)m[5] * v[2];
+    o[2] = (double)m[6] * v[0] + (double)m[7] * v[1] + (double)m[8] * v[2];
+}
+
+/* JuicerCudaDeviceHelpers.cuh:179-225 (CCTF decode for BT.2020 / sRGB). */
+static double decode_cctf(int space, double v) {
+    const double x = fmax(0.0, fin0(v));
+    if (space == FJ_INPUT_BT2020) {
+        const double a = 1.09929681, th = 0.0812428791;
+        return x < th ? x / 4.5 : pow((x + (a - 1.0)) / a, 1.0 / 0.45);
+    }
+    if (space == FJ_INPUT_SRGB_REC709) return x <= 0.04045 ? x / 12.92 : pow((x + 0.055) / 1.055, 2.4);
+    return fin0(v);
+}
+
+/* JuicerCudaDeviceHelpers.cuh:266-282 (Mitchell–Netravali, B = C = 1/3). */
+static double mitchell(double t) {
+    const double B = 1.0 / 3.0, C = 1.0 / 3.0, x = fabs(t);
+    if (x < 1.0)
+        return ((12.0 - 9.0 * B - 6.0 * C) * x * x * x + (-18.0 + 12.0 * B + 6.0 * C) * x * x + (6.0 - 2.0 * B)) / 6.0;
+    if (x < 2.0)
+        return ((-B - 6.0 * C) * x * x * x + (6.0 * B + 30.0 * C) * x * x + (-12.0 * B - 48.0 * C) * x + (8.0 * B + 24.0 * C)) / 6.0;
+    return 0.0;
+}
+
+static int reflect_idx(int i, int n) {
+    if (n <= 1) return 0;
+    if (i < 0) return -i;
+    if (i >= n) return 2 * (n - 1) - i;
+    return i;
+}
+
+/* JuicerCudaDeviceHelpers.cuh:284-327 (bicubic TC-LUT sample, normalized by
+ * the weight sum). */
+static void tc_sample(const float* lut, double qc, double qm, double out[3]) {
+    const int N = 192;
+    int base[2];
+    double frac[2];
+    const double q[2] = {qc, qm};
+    for (int a = 0; a < 2; ++a) {
+        const double coord = fmin(1.0, fmax(0.0, q[a])) * (N - 1);
+        if (coord >= N - 1) {
+            base[a] = N - 2;
+            frac[a] = 1.0;
+        } else {
+            base[a] = (int)floor(coord);
+            frac[a] = coord - base[a];
+        }
+    }
+    double wc[4], wm[4];
+    for (int i = 0; i < 4; ++i) {
+        wc[i] = mitchell(frac[0] + 1.0 - i);
+        wm[i] = mitchell(frac[1] + 1.0 - i);
+    }
+    double acc[3] = {0, 0, 0}, wsum = 0.0;
+    for (int dc = 0; dc < 4; ++dc) {
+        const int c = reflect_idx(base[0] - 1 + dc, N);
+        for (int dm = 0; dm < 4; ++dm) {
+            const int m = reflect_idx(base[1] - 1 + dm, N);
+            const double w = wc[dc] * wm[dm];
+            wsum += w;
+            const float* t = lut + ((size_t)c * N + m) * 4;
+            for (int ch = 0; ch < 3; ++ch) acc[ch] += w * (double)t[ch];
+        }
+    }
+    for (int ch = 0; ch < 3; ++ch) out[ch] = wsum != 0.0 ? acc[ch] / wsum : 0.0;
+}
+
+/* JuicerCudaDeviceHelpers.cuh:34-89: clamped linear lookup on an ascending
+ * axis after scaling logE by the channel gamma. */
+static double curve_at(const Curve* cv, double logE, double gamma) {
+    if (!isfinite(logE)) return NAN;
+    const double xq = logE * gamma;
+    const int n = cv->n;
+#define CY(i) ((double)cv->y[(size_t)(i) * cv->stride])
+    const double xmin = cv->x[0], xmax = cv->x[n - 1];
+    if (!(xmax >= xmin)) return CY(0);
+    if (xq <= xmin) return CY(0);
+    if (xq >= xmax) return CY(n - 1);
+    int lo = 1, hi = n - 1, i1 = n;
+    while (lo <= hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (xq < (double)cv->x[mid]) {
+            i1 = mid;
+            hi = mid - 1;
+        } else {
+            lo = mid + 1;
+        }
+    }
+    if (i1 > n - 1) return CY(n - 1);
+    const double x0 = cv->x[i1 - 1], x1 = cv->x[i1], y0 = CY(i1 - 1), y1 = CY(i1);
+    const double den = x1 - x0;
+    if (!(den > 0.0) || !isfinite(den)) return y0;
+    return y0 + (xq - x0) / den * (y1 - y0);
+#undef CY
+}
+
+/* JuicerCudaDeviceHelpers.cuh:91-104. */
+static double sanitize_inf_logE(double logE, const Curve* cv) {
+    if (isfinite(logE) || isnan(logE)) return logE;
+    const double xmin = cv->x[0], xmax = cv->x[cv->n - 1];
+    if (!(xmax >= xmin)) return logE;
+    return logE > 0.0 ? xmax : xmin;
+}
+
+/* JuicerCudaDeviceHelpers.cuh:1073-1123 (guarded Langmuir v·(k+ref)/(k+v)). */
+static int langmuir(double v, double k, double ref, double* out) {
+    *out = NAN;
+    if (!isfinite(v) || !isfinite(k) || !(k > 0.0) || !isfinite(ref)) return 0;
+    const double den = k + v;
+    if (den == 0.0) return 0;
+    *out = v * (k + ref) / den;
+    return isfinite(*out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* stage kernels over planes                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct Frame {
+    const FjoPlan* plan;
+    const float* src;
+    int w, h, comps;
+    ptrdiff_t src_stride;
+    float* dst;
+    ptrdiff_t dst_stride;
+    double* plane[9]; /* scratch planes */
+    double* linear;   /* [h][w][3] */
+    uint32_t errors;
+    pthread_mutex_t lock;
+} Frame;
+
+/* Camera film linear exposure, B/G/R order
+ * (JuicerCudaDeviceHelpers.cuh:509-617 and :709-763). */
+static void px_exposure(const FjoPlan* pl, const float* px, double E[3]) {
+    const FjFilmExposure* fe = &pl->p.film_exposure;
+    double rgb[3];
+    const int decode = (fe->flags & FJ_FILM_DECODE_CCTF) != 0;
+    for (int c = 0; c < 3; ++c) rgb[c] = decode ? decode_cctf((int)fe->input_color_space, px[c]) : fin0(px[c]);
+    double xyz[3], w[3];
+    mat3(fe->input_rgb_to_xyz, rgb, xyz);
+    if (fe->flags & FJ_FILM_ADAPT_INPUT) {
+        mat3(fe->input_xyz_adapt, xyz, w);
+    } else {
+        memcpy(w, xyz, sizeof(w));
+    }
+    for (int c = 0; c < 3; ++c) w[c] = fin0(w[c]);
+    const double sum = w[0] + w[1] + w[2];
+    const double den = fmax(sum, 1e-10);
+    const double x = w[0] / den, y = w[1] / den;
+    /* tri2quad (JuicerCudaDeviceHelpers.cuh:244-252) */
+    const double qx = fmin(1.0, fmax(0.0, (1.0 - x) * (1.0 - x)));
+    const double qy = fmin(1.0, fmax(0.0, y / fmax(1.0 - x, 1e-10)));
+    double rgbL[3];
+    tc_sample(fe->tc_lut_rgba.data, qx, qy, rgbL);
+    E[0] = fin0(sum * fin0(rgbL[2]));
+    E[1] = fin0(sum * fin0(rgbL[1]));
+    E[2] = fin0(sum * fin0(rgbL[0]));
+    double scale = pl->manual_scale * (double)fe->route_correction_scale;
+    if (!(isfinite(scale) && scale > 0.0)) scale = 1.0;
+    for (int c = 0; c < 3; ++c) E[c] = fmax(0.0, E[c] * scale);
+}
+
+static void film_curves(const FjoPlan* pl, Curve dens[3], Curve comp[3]) {
+    const FjFilmDevelopment* fd = &pl->p.film_development;
+    const int n = (int)fd->log_exposure.count;
+    for (int b = 0; b < 3; ++b) {
+        const int rgb = 2 - b; /* B/G/R device order reads R,G,B columns reversed */
+        dens[b].x = fd->log_exposure.data;
+        dens[b].y = fd->density_rgb.data + rgb;
+        dens[b].n = n;
+        dens[b].stride = 3;
+        comp[b] = dens[b];
+        if (pl->p.dir_couplers.mode != FJ_DIR_INACTIVE) comp[b].x = pl->p.dir_couplers.compensated_axes_rgb[rgb].data;
+    }
+}
+
+/* Film development with non-spatial DIR: log raw -> layer densities ->
+ * DIR corrections -> compensated curves (JuicerCudaScanPipeline.cu:1624-1697,
+ * JuicerCudaDeviceHelpers.cuh:1125-1253). Output D in C/M/Y order. */
+static uint32_t px_develop(const FjoPlan* pl, const double E[3], double Dcmy[3]) {
+    const FjFilmDevelopment* fd = &pl->p.film_development;
+    const FjDirCouplers* dir = &pl->p.dir_couplers;
+    Curve dens[3], comp[3];
+    film_curves(pl, dens, comp);
+    const double gamma[3] = {fd->gamma_rgb[2], fd->gamma_rgb[1], fd->gamma_rgb[0]};
+    double logE[3], layer[3];
+    for (int b = 0; b < 3; ++b) {
+        logE[b] = sanitize_inf_logE(log10(fmax(E[b], 0.0) + 1e-10), &dens[b]);
+        layer[b] = curve_at(&dens[b], logE[b], gamma[b]);
+    }
+    if (dir->mode == FJ_DIR_INACTIVE) {
+        Dcmy[0] = layer[2];
+        Dcmy[1] = layer[1];
+        Dcmy[2] = layer[0];
+        return 0;
+    }
+    /* BGR packing of the DIR record (JuicerCudaFilmPayloads.cpp:172-190). */
+    double M[9], dmax[3], dref[3], dk[3], cref[3], kr[3];
+    for (int d = 0; d < 3; ++d) {
+        for (int r = 0; r < 3; ++r) M[d * 3 + r] = dir->matrix_rgb[(2 - d) * 3 + (2 - r)];
+        dmax[d] = dir->density_max_rgb[2 - d];
+        dref[d] = dir->density_ref_rgb[2 - d];
+        dk[d] = dir->donor_k_rgb[2 - d];
+        cref[d] = dir->receiver_c_ref_rgb[2 - d];
+        kr[d] = dir->receiver_kr_rgb[2 - d];
+    }
+    uint32_t fail = 0;
+    double donors[3];
+    for (int c = 0; c < 3; ++c) {
+        donors[c] = dmax[c] - layer[c];
+        if (dir->mode == FJ_DIR_NEGATIVE_DONOR_LANGMUIR && !langmuir(layer[c], dk[c], dref[c], &donors[c])) fail |= 1u << c;
+    }
+    double corr[3];
+    for (int r = 0; r < 3; ++r) {
+        corr[r] = M[0 * 3 + r] * donors[0] + M[1 * 3 + r] * donors[1] + M[2 * 3 + r] * donors[2];
+        if (!isfinite(corr[r])) fail |= 1u << r;
+    }
+    if (dir->mode == FJ_DIR_POSITIVE_RECEIVER_LANGMUIR) {
+        for (int c = 0; c < 3; ++c) {
+            if (cref[c] == 0.0) {
+                corr[c] = 0.0;
+                continue;
+            }
+            double v;
+            if (!langmuir(corr[c], kr[c], cref[c], &v)) fail |= 1u << c;
+            corr[c] = v;
+        }
+    }
+    if (fail == 0) {
+        for (int c = 0; c < 3; ++c) {
+            if (isnan(logE[c])) {
+                fail |= 1u << c;
+                continue;
+            }
+            const double o = logE[c];
+            logE[c] = o - corr[c];
+            if (!isfinite(logE[c]) && isfinite(o)) fail |= 1u << c;
+        }
+    }
+    if (fail) {
+        Dcmy[0] = Dcmy[1] = Dcmy[2] = NAN;
+        return (1u << 8) | ((fail & 7u) << 9);
+    }
+    Dcmy[2] = curve_at(&comp[0], logE[0], gamma[0]);
+    Dcmy[1] = curve_at(&comp[1], logE[1], gamma[1]);
+    Dcmy[0] = curve_at(&comp[2], logE[2], gamma[2]);
+    return 0;
+}
+
+/* Print exposure (JuicerCudaDeviceHelpers.cuh:866-964) and print development
+ * (:1034-1071): 81-λ transmittance x filtered illuminant x paper sensitivity,
+ * balance normalizer, preflash, print exposure, log10, print curves. */
+static void px_print(const FjoPlan* pl, double D[3]) {
+    const FjPrint* pr = &pl->p.print;
+    const float* eps = pr->film_density_cmy.data;
+    const float* base = pr->film_base_density.data;
+    const float* sens = pr->sensitivity_cmy.data;
+    const float* ill = pr->main_illuminant.data;
+    double acc[3] = {0, 0, 0};
+    for (int i = 0; i < NSPEC; ++i) {
+        const double s[3] = {sens[i * 3], sens[i * 3 + 1], sens[i * 3 + 2]};
+        if (isnan(s[0]) && isnan(s[1]) && isnan(s[2])) continue;
+        const double dl = D[0] * eps[i * 3] + D[1] * eps[i * 3 + 1] + D[2] * eps[i * 3 + 2] + (double)base[i];
+        double e = pow(10.0, -dl) * (double)ill[i];
+        if (isnan(e)) e = 0.0;
+        for (int c = 0; c < 3; ++c)
+            if (!isnan(s[c])) acc[c] += e * s[c];
+    }
+    const double pf = pr->preflash_exposure;
+    const int preflash = (pr->flags & FJ_PRINT_PREFLASH) != 0;
+    for (int c = 0; c < 3; ++c) {
+        acc[c] *= (double)pr->normalizer;
+        if (preflash && isfinite(pf) && pf > 0.0) acc[c] += (double)pr->preflash_raw_cmy[c] * pf;
+        acc[c] *= (double)pr->exposure;
+        acc[c] *= (double)pl->p.scanner_correction.exposure_scale;
+    }
+    const int n = (int)pr->log_exposure.count;
+    for (int c = 0; c < 3; ++c) {
+        const Curve cv = {pr->log_exposure.data, pr->density_cmy.data + c, n, 3};
+        D[c] = curve_at(&cv, log10(acc[c] + 1e-10), 1.0);
+    }
+}
+
+/* Tricubic PCHIP sample of the log2 XYZ LUT (JuicerCudaDeviceHelpers.cuh:407-506). */
+static double hermite(double y0, double y1, double m0, double m1, double t) {
+    const double t2 = t * t, t3 = t2 * t;
+    return (2 * t3 - 3 * t2 + 1) * y0 + (t3 - 2 * t2 + t) * m0 + (-2 * t3 + 3 * t2) * y1 + (t3 - t2) * m1;
+}
+
+static int px_scan_lut(const FjoPlan* pl, const double nrm[3], double out[3]) {
+    const int r = pl->res;
+    if (!isfinite(nrm[0]) || !isfinite(nrm[1]) || !isfinite(nrm[2])) return 0;
+    int b[3];
+    double t[3];
+    for (int a = 0; a < 3; ++a) {
+        double coord = fmin((double)(r - 1), fmax(0.0, nrm[a] * (r - 1)));
+        if (coord >= r - 1) {
+            b[a] = r - 2;
+            t[a] = 1.0;
+        } else {
+            b[a] = (int)floor(coord);
+            t[a] = coord - b[a];
+        }
+    }
+#define N(arr, c, m, y, o) ((double)(arr)[((((size_t)(c) * r + (m)) * r + (y)) * 3) + (o)])
+    const int c = b[0], m = b[1], y = b[2], s = r - 1;
+    for (int o = 0; o < 3; ++o) {
+        double v[2][2];
+        for (int dm = 0; dm < 2; ++dm)
+            for (int dy = 0; dy < 2; ++dy)
+                v[dm][dy] = hermite(N(pl->l2, c, m + dm, y + dy, o), N(pl->l2, c + 1, m + dm, y + dy, o),
+                                    N(pl->sc, c, m + dm, y + dy, o), N(pl->sc, c + 1, m + dm, y + dy, o), t[0]);
+        double smv[2][2];
+        for (int dm = 0; dm < 2; ++dm)
+            for (int dy = 0; dy < 2; ++dy) {
+                const double a0 = N(pl->sm, c, m + dm, y + dy, o), a1 = N(pl->sm, c + 1, m + dm, y + dy, o);
+                smv[dm][dy] = a0 + t[0] * (a1 - a0);
+            }
+        const double vy0 = hermite(v[0][0], v[1][0], smv[0][0], smv[1][0], t[1]);
+        const double vy1 = hermite(v[0][1], v[1][1], smv[0][1], smv[1][1], t[1]);
+        double syv[2];
+        for (int dy = 0; dy < 2; ++dy) {
+            const double a00 = N(pl->sy, c, m, y + dy, o), a10 = N(pl->sy, c + 1, m, y + dy, o);
+            const double a01 = N(pl->sy, c, m + 1, y + dy, o), a11 = N(pl->sy, c + 1, m + 1, y + dy, o);
+            const double l0 = a00 + t[0] * (a10 - a00), l1 = a01 + t[0] * (a11 - a01);
+            syv[dy] = l0 + t[1] * (l1 - l0);
+        }
+        const double val = hermite(vy0, vy1, syv[0], syv[1], t[2]);
+        const size_t ci = ((((size_t)c * s + m) * s + y) * 3) + o;
+        out[o] = fmin((double)pl->cmax[ci], fmax((double)pl->cmin[ci], val));
+    }
+#undef N
+    return 1;
+}
+
+/* Output gamut compression in OkLch (src/GamutCompression.h:117-213). */
+static const double OK_XYZ_LMS[9] = {0.8189330101, 0.3618667424, -0.1288597137, 0.0329845436, 0.9293118715,
+                                     0.0361456387, 0.0482003018, 0.2643662691, 0.6338517070};
+static const double OK_LMS_XYZ[9] = {1.2270138511, -0.5577999807, 0.2812561490, -0.0405801784, 1.1122568696,
+                                     -0.0716766787, -0.0763812845, -0.4214819784, 1.5861632204};
+static const double OK_LMSR_LAB[9] = {0.2104542553, 0.7936177850, -0.0040720468, 1.9779984951, -2.4285922050,
+                                      0.4505937099, 0.0259040371, 0.7827717662, -0.8086757660};
+static const double OK_LAB_LMSR[9] = {1.0, 0.3963377774, 0.2158037573, 1.0, -0.1055613458,
+                                      -0.0638541728, 1.0, -0.0894841775, -1.2914855480};
+
+static void mat3d(const double m[9], const double v[3], double o[3]) {
+    o[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+    o[1] = m[3] * v[0] + m[4] * v[1] + m[5] * v[2];
+    o[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
+}
+
+static double knee(double v, const float k[3]) {
+    const double th = k[0], lim = k[1], pw = k[2];
+    if (!(v > th)) return v;
+    const double span = lim - th, n = (v - th) / span;
+    return th + span * n / pow(1.0 + pow(n, pw), 1.0 / pw);
+}
+
+static double boundary(const float* cmax, double L, double hue) {
+    const double pi = 3.14159265358979323846;
+    const double cl = fmin(fmax(L, 0.02), 1.0);
+    const double lp = (cl - 0.02) / (1.0 - 0.02) * 63.0;
+    const int l0 = (int)floor(lp), l1 = l0 + 1 < 64 ? l0 + 1 : l0;
+    const double lf = lp - l0;
+    double wh = fmod(hue + pi, 2.0 * pi);
+    if (wh < 0.0) wh += 2.0 * pi;
+    const double hp = wh / (2.0 * pi) * 720.0;
+    const int h0 = ((int)floor(hp)) % 720, h1 = (h0 + 1) % 720;
+    const double hf = hp - floor(hp);
+    const double a = cmax[l0 * 720 + h0] + hf * ((double)cmax[l0 * 720 + h1] - cmax[l0 * 720 + h0]);
+    const double b = cmax[l1 * 720 + h0] + hf * ((double)cmax[l1 * 720 + h1] - cmax[l1 * 720 + h0]);
+    return a + lf * (b - a);
+}
+
+static void gamut_compress(const FjOutputColor* oc, double rgb[3]) {
+    double xyz[3], lms[3], lab[3];
+    mat3(oc->native_rgb_to_d65_xyz, rgb, xyz);
+    mat3d(OK_XYZ_LMS, xyz, lms);
+    for (int i = 0; i < 3; ++i) lms[i] = cbrt(lms[i]);
+    mat3d(OK_LMSR_LAB, lms, lab);
+    lab[0] = knee(lab[0], oc->lightness_knee);
+    const double chroma = hypot(lab[1], lab[2]);
+    if (chroma > 0.0) {
+        const double hue = atan2(lab[2], lab[1]);
+        const double bnd = fmax(boundary(oc->gamut_cmax.data, lab[0], hue), 1.0e-9);
+        const double comp = knee(chroma / bnd, oc->chroma_knee) * bnd;
+        lab[1] *= comp / chroma;
+        lab[2] *= comp / chroma;
+    }
+    mat3d(OK_LAB_LMSR, lab, lms);
+    for (int i = 0; i < 3; ++i) lms[i] = lms[i] * lms[i] * lms[i];
+    mat3d(OK_LMS_XYZ, lms, xyz);
+    mat3(oc->d65_xyz_to_native_rgb, xyz, rgb);
+}
+
+/* Scanner: normalize -> PCHIP LUT -> exp2 -> correction -> CAT02 -> RGB ->
+ * gamut (JuicerCudaScanPipeline.cu:1735-1811). Returns 0 on scan error. */
+static int px_scan(const FjoPlan* pl, const double D[3], double rgb[3]) {
+    const FjDensityBounds* sb = &pl->p.scanner_bounds;
+    double nrm[3], l2[3];
+    for (int c = 0; c < 3; ++c) nrm[c] = (D[c] - (double)sb->min_cmy[c]) * (double)sb->inv_span_cmy[c];
+    if (!px_scan_lut(pl, nrm, l2)) return 0;
+    double xyz[3] = {exp2(l2[0]), exp2(l2[1]), exp2(l2[2])};
+    const FjScannerCorrection* cr = &pl->p.scanner_correction;
+    if (cr->flags & FJ_CORRECTION_ACTIVE) {
+        const double cy = fmin(fmax((double)cr->xyz_slope * xyz[1] + (double)cr->xyz_offset, 0.0), 1.0);
+        const double sc = cy / (xyz[1] + 1e-10);
+        for (int c = 0; c < 3; ++c) xyz[c] *= sc;
+    }
+    double ad[3];
+    mat3(pl->p.output_color.cat02, xyz, ad);
+    mat3(pl->p.output_color.xyz_to_rgb, ad, rgb);
+    if (!isfinite(rgb[0]) || !isfinite(rgb[1]) || !isfinite(rgb[2])) return 0;
+    if (pl->p.output_color.flags & FJ_COLOR_GAMUT_COMPRESSION) {
+        gamut_compress(&pl->p.output_color, rgb);
+        if (!isfinite(rgb[0]) || !isfinite(rgb[1]) || !isfinite(rgb[2])) return 0;
+    }
+    return 1;
+}
+
+/* Output encoding (JuicerCudaScanPipeline.cu:99-190). */
+static double encode_channel(const FjOutputEncoding* e, double v) {
+    switch (e->cctf_kind) {
+        case FJ_CCTF_LINEAR: return v;
+        case FJ_CCTF_GAMMA: return copysign(pow(fabs(v), (double)e->gamma), v);
+        case FJ_CCTF_SRGB: return v <= 0.0031308 ? 12.92 * v : 1.055 * pow(v, 1.0 / 2.4) - 0.055;
+        case FJ_CCTF_BT2020: return v < (double)e->b ? v * 4.5 : (double)e->a * pow(v, 0.45) - ((double)e->a - 1.0);
+        case FJ_CCTF_PROPHOTO: return v < (double)e->linear_cutoff ? v * 16.0 : pow(v, (double)e->gamma);
+        case FJ_CCTF_DWG_INTERMEDIATE: {
+            const double l = fmax(0.0, v);
+            return l <= (double)e->linear_cutoff ? l * (double)e->d : (log2(l + (double)e->a) + (double)e->b) * (double)e->c;
+        }
+        default: return fmin(1.0, fmax(0.0, v));
+    }
+}
+
+static void px_encode(const FjOutputColor* oc, double rgb[3]) {
+    double lin[3];
+    if (oc->flags & FJ_COLOR_INPUT_IS_OUTPUT_SPACE) {
+        memcpy(lin, rgb, sizeof(lin));
+    } else {
+        mat3(oc->encoding.dwg_to_output, rgb, lin);
+    }
+    for (int c = 0; c < 3; ++c) {
+        double v = (oc->flags & FJ_COLOR_ENCODE_CCTF) ? encode_channel(&oc->encoding, lin[c]) : lin[c];
+        rgb[c] = v <= 0.0 ? 0.0 : (v >= 1.0 ? 1.0 : v);
+        if (isnan(v)) rgb[c] = v; /* clamp01d keeps NaN out: NaN compares false */
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* schedule                                                                 */
+/* ------------------------------------------------------------------------ */
+
+typedef void (*RowFn)(Frame* f, int y0, int y1);
+
+typedef struct Job {
+    Frame* f;
+    RowFn fn;
+    int y0, y1;
+} Job;
+
+static void* job_main(void* arg) {
+    Job* j = (Job*)arg;
+    j->fn(j->f, j->y0, j->y1);
+    return NULL;
+}
+
+static void run_rows(Frame* f, RowFn fn, int threads) {
+    if (threads <= 1 || f->h < 2 * threads) {
+        fn(f, 0, f->h);
+        return;
+    }
+    pthread_t tid[256];
+    Job jobs[256];
+    if (threads > 256) threads = 256;
+    for (int t = 0; t < threads; ++t) {
+        jobs[t].f = f;
+        jobs[t].fn = fn;
+        jobs[t].y0 = (int)((long)f->h * t / threads);
+        jobs[t].y1 = (int)((long)f->h * (t + 1) / threads);
+        pthread_create(&tid[t], NULL, job_main, &jobs[t]);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+}
+
+static void add_errors(Frame* f, uint32_t bits) {
+    if (!bits) return;
+    pthread_mutex_lock(&f->lock);
+    f->errors |= bits;
+    pthread_mutex_unlock(&f->lock);
+}
+
+/* Fused per-pixel chain for frames without spatial stages. */
+static void rows_chain(Frame* f, int y0, int y1) {
+    const FjoPlan* pl = f->plan;
+    uint32_t errs = 0;
+    for (int y = y0; y < y1; ++y) {
+        for (int x = 0; x < f->w; ++x) {
+            const float* px = f->src + y * f->src_stride + (ptrdiff_t)x * f->comps;
+            double E[3], D[3], rgb[3];
+            px_exposure(pl, px, E);
+            errs |= px_develop(pl, E, D);
+            if (pl->print) px_print(pl, D);
+            const size_t idx = (size_t)y * f->w + x;
+            int ok = px_scan(pl, D, rgb);
+            if (!ok) {
+                errs |= 1u;
+                rgb[0] = rgb[1] = rgb[2] = 0.0;
+            }
+            if (f->linear) memcpy(f->linear + idx * 3, rgb, sizeof(rgb));
+            if (f->dst) {
+                float* o = f->dst + y * f->dst_stride + (ptrdiff_t)x * f->comps;
+                if (ok) px_encode(&pl->p.output_color, rgb);
+                o[0] = (float)rgb[0];
+                o[1] = (float)rgb[1];
+                o[2] = (float)rgb[2];
+                if (f->comps == 4) o[3] = px[3];
+            }
+        }
+    }
+    add_errors(f, errs);
+}
+
+int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int comps,
+               ptrdiff_t src_stride, float* dst, ptrdiff_t dst_stride, double* linear, int threads,
+               uint32_t* error_flags) {
+    if (!plan || !src || width <= 0 || height <= 0 || (comps != 3 && comps != 4)) return 2;
+    const FjPreparedHostData* p = &plan->p;
+    const int spatial = (p->optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION |
+                                            FJ_OPTICS_SCATTER_HALATION)) != 0 ||
+                        p->spatial_dir.component_count != 0 || (p->grain.flags & FJ_GRAIN_ACTIVE) != 0 ||
+                        (p->scanner_post.hash != 0 && p->scanner_post.unsharp_amount != 0.0f);
+    if (spatial) return 3;
+    Frame f;
+    memset(&f, 0, sizeof(f));
+    f.plan = plan;
+    f.src = src;
+    f.w = width;
+    f.h = height;
+    f.comps = comps;
+    f.src_stride = src_stride;
+    f.dst = dst;
+    f.dst_stride = dst_stride;
+    f.linear = linear;
+    pthread_mutex_init(&f.lock, NULL);
+    run_rows(&f, rows_chain, threads);
+    pthread_mutex_destroy(&f.lock);
+    if (error_flags) *error_flags = f.errors;
+    return 0;
+}

@@ -1,0 +1,68 @@
+/*
+ * fj_oracle.h — FP64 CPU oracle of Film-Juicer's per-frame render path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker the HIP product is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg load it. The product never links or calls it.
+ *
+ * Every function restates the reference algorithm in double precision from
+ * the same prepared FP32 tables the product consumes (FjPreparedHostData);
+ * each definition cites the reference file:line it follows.
+ */
+#ifndef FJ_ORACLE_H
+#define FJ_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/fj/juicer_cuda_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* PCHIP slopes of one line (rust/film-juicer-core/src/scanner/lut.rs:108-138). */
+void fjo_pchip_slopes(const double* values, int n, double* out);
+
+/* Scanner PCHIP LUT build (scanner/lut.rs:140-299). Outputs are f32 words:
+ * log2_xyz/slope_* hold res^3*3, cell_min/max hold (res-1)^3*3 floats.
+ * Returns 0 on success, 1 on a non-finite node (fail_index/fail_channel set),
+ * 2 on bad arguments. */
+int fjo_scanner_lut_build(const float* eps_c, const float* eps_m, const float* eps_y,
+                          const float* weighted_x, const float* weighted_y, const float* weighted_z,
+                          const float* baseline_or_null, int sample_count, float inv_yn,
+                          const float range_min_cmy[3], const float inverse_span_cmy[3],
+                          int medium, int resolution,
+                          float* log2_xyz, float* slope_c, float* slope_m, float* slope_y,
+                          float* cell_min, float* cell_max,
+                          uint64_t* fail_index, uint32_t* fail_channel);
+
+typedef struct FjoPlan FjoPlan;
+
+/* Builds host state for one prepared record (scanner LUT, curves). Returns
+ * NULL and writes a message on unsupported input. */
+FjoPlan* fjo_plan_create(const FjPreparedHostData* prepared, char* message, size_t capacity);
+void fjo_plan_destroy(FjoPlan* plan);
+
+/* Exposes the plan's scanner LUT (pointers stay valid until destroy). */
+int fjo_plan_scanner_lut(const FjoPlan* plan, int* resolution, const float** log2_xyz,
+                         const float** slope_c, const float** slope_m, const float** slope_y,
+                         const float** cell_min, const float** cell_max);
+
+/* Renders one full frame in double precision.
+ *   src: host RGB/RGBA float32, row stride in floats.
+ *   dst: host float32 output of the same component count (encoded, clamped),
+ *        may be NULL.
+ *   linear_rgb: optional [height][width][3] doubles: scanner-linear output
+ *        after gamut compression and post effects, before output encoding.
+ *   pixel_size_um / seed inputs come from the prepared record.
+ * Returns 0 on success; *error_flags receives the scan/DIR error bits. */
+int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int components,
+               ptrdiff_t src_stride_floats, float* dst, ptrdiff_t dst_stride_floats,
+               double* linear_rgb, int threads, uint32_t* error_flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

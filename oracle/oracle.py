@@ -1,0 +1,106 @@
+"""ctypes wrapper of the FP64 oracle (test infrastructure only).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg to
+check the HIP product; never used by the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfj_oracle.so")
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        fp = C.POINTER(C.c_float)
+        L.fjo_pchip_slopes.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
+        L.fjo_scanner_lut_build.argtypes = [fp] * 7 + [C.c_int, C.c_float, fp, fp, C.c_int, C.c_int] + [fp] * 6 + [
+            C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
+        L.fjo_plan_create.restype = C.c_void_p
+        L.fjo_plan_create.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
+        L.fjo_plan_destroy.argtypes = [C.c_void_p]
+        L.fjo_plan_scanner_lut.argtypes = [C.c_void_p, C.POINTER(C.c_int)] + [C.POINTER(fp)] * 6
+        L.fjo_render.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t, fp, C.c_ssize_t,
+                                 C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_uint32)]
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def pchip_slopes(values) -> np.ndarray:
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    out = np.zeros_like(v)
+    lib().fjo_pchip_slopes(v.ctypes.data_as(C.POINTER(C.c_double)), v.size, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def scanner_lut(eps_c, eps_m, eps_y, wx, wy, wz, baseline, inv_yn, range_min, inv_span, medium, res,
+                sample_count=None):
+    arr = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    ins = [arr(v) for v in (eps_c, eps_m, eps_y, wx, wy, wz)]
+    base = None if baseline is None else arr(baseline)
+    if res < 2:
+        return 2, None, (0, 0)
+    k = ins[0].size if sample_count is None else sample_count
+    nodes, cells = res ** 3 * 3, (res - 1) ** 3 * 3
+    outs = [np.zeros(nodes, np.float32) for _ in range(4)] + [np.zeros(cells, np.float32) for _ in range(2)]
+    fi, fc = C.c_uint64(0), C.c_uint32(0)
+    rc = lib().fjo_scanner_lut_build(*[_fp(a) for a in ins], _fp(base), k, C.c_float(inv_yn),
+                                     _fp(arr(range_min)), _fp(arr(inv_span)), medium, res,
+                                     *[_fp(o) for o in outs], C.byref(fi), C.byref(fc))
+    return rc, outs, (fi.value, fc.value)
+
+
+class Plan:
+    """Oracle state for one prepared record (keeps the record's storage alive)."""
+
+    def __init__(self, prepared):
+        self.prepared = prepared
+        msg = C.create_string_buffer(256)
+        self.handle = lib().fjo_plan_create(C.addressof(prepared.record), msg, 256)
+        if not self.handle:
+            raise RuntimeError(msg.value.decode())
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().fjo_plan_destroy(self.handle)
+            self.handle = None
+
+    def scanner_lut(self):
+        res = C.c_int(0)
+        ptrs = [C.POINTER(C.c_float)() for _ in range(6)]
+        lib().fjo_plan_scanner_lut(self.handle, C.byref(res), *[C.byref(p) for p in ptrs])
+        r = res.value
+        sizes = [r ** 3 * 3] * 4 + [(r - 1) ** 3 * 3] * 2
+        return r, [np.ctypeslib.as_array(p, shape=(n,)).copy() for p, n in zip(ptrs, sizes)]
+
+    def render(self, src: np.ndarray, threads: int = 1, want_dst: bool = True):
+        """Returns (encoded float32 HxWxC or None, linear float64 HxWx3, error flags)."""
+        src = np.ascontiguousarray(src, dtype=np.float32)
+        h, w, c = src.shape
+        dst = np.zeros_like(src) if want_dst else None
+        lin = np.zeros((h, w, 3), np.float64)
+        flags = C.c_uint32(0)
+        rc = lib().fjo_render(self.handle, _fp(src), w, h, c, w * c, _fp(dst), w * c,
+                              lin.ctypes.data_as(C.POINTER(C.c_double)), threads, C.byref(flags))
+        if rc != 0:
+            raise RuntimeError(f"oracle render failed rc={rc}")
+        return dst, lin, flags.value

@@ -1,0 +1,181 @@
+"""Extract compact golden fixtures from the reference's own test data.
+
+Run once in the build container (it reads /root/reference); the outputs are
+committed under tests/golden/ and nothing at run time reads the reference.
+
+Sources (all data files held by the reference's tests):
+  * tests/ffi/cuda_abi_facts.inc                       -> abi_facts.json
+      x86-64 sizes/offsets/values of every FjPreparedHostData record.
+  * tests/ffi/fixtures/prepared-state-projection/linux-release.json
+      -> prepared/<case>.npz + prepared/<case>.json
+      Complete prepared products (exact f32 words) for named recipes:
+      TC LUT, film density curves, DIR axes, print spectra/curves, scanner
+      spectra, output colour and gamut cmax table.
+  * tests/scanner/lut-fixtures/native.json             -> scanner_lut_cases.npz
+      Exact scanner PCHIP LUT words for small synthetic inputs and for the
+      four production LUTs (one per route).
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import re
+import struct
+import sys
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def f32(word: int) -> float:
+    return struct.unpack("<f", struct.pack("<I", word & 0xFFFFFFFF))[0]
+
+
+def abi_facts() -> None:
+    facts = {"types": {}, "fields": [], "values": {}}
+    pat = re.compile(r"^FJ_ABI_(TYPE|FIELD|VALUE)\((.*)\)\s*$")
+    for line in open(f"{REF}/tests/ffi/cuda_abi_facts.inc"):
+        m = pat.match(line.strip())
+        if not m:
+            continue
+        kind, args = m.group(1), [a.strip() for a in m.group(2).split(",")]
+        if kind == "TYPE":
+            facts["types"][args[0]] = [int(args[1]), int(args[2])]
+        elif kind == "FIELD":
+            facts["fields"].append([args[0], args[1], int(args[2])])
+        else:
+            facts["values"][args[0]] = int(args[1], 0)
+    with open(os.path.join(HERE, "abi_facts.json"), "w") as fh:
+        json.dump(facts, fh, indent=0, sort_keys=True)
+
+
+# ---------------------------------------------------------------------------
+# prepared-state projection
+# ---------------------------------------------------------------------------
+CASES = [
+    "NegativeDirectHanatos",
+    "NegativePrintHanatos",
+    "PositivePrintHanatos",
+    "CameraAndEnlargerOptics",
+    "GammaAndGrainLayers",
+    "DirAndCompressionOff",
+    "PreflashAndNormalizationModes[norm=1,compensate=1]",
+]
+# File-system friendly names for case ids that carry brackets.
+ALIASES = {"PreflashAndNormalizationModes[norm=1,compensate=1]": "PrintPreflash"}
+
+
+def scalar(node):
+    t = node["type"]
+    if t == "f32":
+        return f32(int(node["bits"], 16))
+    if t == "f64":
+        return struct.unpack("<d", struct.pack("<Q", int(node["bits"], 16)))[0]
+    if t == "utf8":
+        return node["value"]
+    return int(node["value"])
+
+
+def array(node):
+    et = node["element_type"]
+    words = node["words"]
+    if et == "f32":
+        return np.array([int(w, 16) for w in words], dtype=np.uint32).view(np.float32)
+    if et == "f64":
+        return np.array([int(w, 16) for w in words], dtype=np.uint64).view(np.float64)
+    if et == "u32":
+        return np.array([w if isinstance(w, int) else int(w, 16) for w in words], dtype=np.uint32)
+    raise ValueError(et)
+
+
+def flatten(node, prefix, scalars, arrays):
+    if isinstance(node, dict):
+        if "words" in node and "element_type" in node:
+            arrays[prefix] = array(node)
+            return
+        if "elements" in node and node.get("element_type", "").startswith("span"):
+            for i, el in enumerate(node["elements"]):
+                arrays[f"{prefix}[{i}]"] = array(el)
+            return
+        if "type" in node and ("value" in node or "bits" in node):
+            if node["type"] in ("live-backing",):
+                return
+            scalars[prefix] = scalar(node)
+            return
+        for k, v in node.items():
+            flatten(v, f"{prefix}.{k}" if prefix else k, scalars, arrays)
+
+
+def prepared_cases() -> None:
+    out_dir = os.path.join(HERE, "prepared")
+    os.makedirs(out_dir, exist_ok=True)
+    doc = json.load(open(f"{REF}/tests/ffi/fixtures/prepared-state-projection/linux-release.json"))
+    shared = {}  # content-addressed large arrays, stored once
+    for case in doc["cases"]:
+        cid = case["case_id"]
+        if cid not in CASES:
+            continue
+        name = ALIASES.get(cid, cid)
+        scalars, arrays = {}, {}
+        flatten(case["expected"], "", scalars, arrays)
+        inputs = {}
+        flatten(case["input"], "", inputs, {})
+        refs = {}
+        small = {}
+        for k, v in arrays.items():
+            if v.size >= 4096:
+                key = f"t{v.size}_{hashlib.sha1(v.tobytes()).hexdigest()[:12]}"
+                shared[key] = v
+                refs[k] = key
+            else:
+                small[k] = v
+        np.savez_compressed(os.path.join(out_dir, f"{name}.npz"), **small)
+        with open(os.path.join(out_dir, f"{name}.json"), "w") as fh:
+            json.dump({"case_id": cid, "scalars": scalars, "large": refs, "input": inputs},
+                      fh, indent=0, sort_keys=True)
+    np.savez_compressed(os.path.join(out_dir, "shared_tables.npz"), **shared)
+
+
+def scanner_luts() -> None:
+    doc = json.load(open(f"{REF}/tests/scanner/lut-fixtures/native.json"))
+    out = {}
+    meta = []
+    for group in ("cases", "products"):
+        for case in doc[group]:
+            cid = case["id"]
+            inp = case["input"]
+            nat = case["native"]
+            entry = {"id": cid, "group": group, "success": bool(nat["success"]),
+                     "resolution": int(inp["resolution"]), "medium": int(inp["medium"]),
+                     "sample_count": int(inp["sample_count"]), "has_baseline": bool(inp["has_baseline"]),
+                     "inv_yn": int(inp["inv_yn"])}
+            if group == "products":
+                entry["descriptor_hash"] = int(case["descriptor_hash"])
+                entry["route"] = int(case["route"])
+            meta.append(entry)
+            for k in ("eps_c", "eps_m", "eps_y", "weighted_x", "weighted_y", "weighted_z",
+                      "baseline", "range_min_cmy", "inverse_span_cmy"):
+                out[f"{cid}/in/{k}"] = np.array(inp[k], dtype=np.uint32)
+            if nat["success"]:
+                for k in ("log2_xyz", "slope_c", "slope_m", "slope_y", "cell_min", "cell_max"):
+                    out[f"{cid}/out/{k}"] = np.array(nat[k], dtype=np.uint32)
+    slopes = []
+    for s in doc["slopes"]:
+        slopes.append({"id": s["id"], "input": [str(w) for w in s["input"]],
+                       "native": [str(w) for w in s["native"]]})
+    np.savez_compressed(os.path.join(HERE, "scanner_lut_cases.npz"), **out)
+    with open(os.path.join(HERE, "scanner_lut_cases.json"), "w") as fh:
+        json.dump({"cases": meta, "slopes": slopes}, fh, indent=0)
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["abi", "prepared", "lut"]
+    if "abi" in what:
+        abi_facts()
+    if "prepared" in what:
+        prepared_cases()
+    if "lut" in what:
+        scanner_luts()

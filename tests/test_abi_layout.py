@@ -1,0 +1,59 @@
+"""The boundary records keep the reference's x86-64 layout.
+
+Facts come from the reference's own ABI contract file
+(tests/ffi/cuda_abi_facts.inc), extracted to tests/golden/abi_facts.json.
+Both the C header (compiled with gcc) and the ctypes mirror are checked.
+"""
+import ctypes
+import json
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+from fj import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+FACTS = json.load(open(os.path.join(HERE, "golden", "abi_facts.json")))
+
+
+def test_ctypes_mirror_matches_reference_facts():
+    checked = 0
+    for name, (size, align) in FACTS["types"].items():
+        if name.endswith("*") or name == "FjAbortQuery":
+            continue
+        t = abi.ALL_TYPES[name]
+        assert ctypes.sizeof(t) == size, name
+        assert ctypes.alignment(t) == align, name
+        checked += 1
+    for name, field, off in FACTS["fields"]:
+        assert getattr(abi.ALL_TYPES[name], field).offset == off, (name, field)
+    assert checked == 43 and len(FACTS["fields"]) == 414
+
+
+def test_c_header_matches_reference_facts():
+    lines = ['#include <stddef.h>', '#include "fj/juicer_cuda_api.h"']
+    for name, (size, align) in FACTS["types"].items():
+        lines.append(f'_Static_assert(sizeof({name}) == {size} && _Alignof({name}) == {align}, "{name}");')
+    for name, field, off in FACTS["fields"]:
+        lines.append(f'_Static_assert(offsetof({name}, {field}) == {off}, "{name}.{field}");')
+    for name, value in FACTS["values"].items():
+        lines.append(f'_Static_assert({name} == {value}, "{name}");')
+    lines.append("int main(void) { return 0; }")
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "abi.c")
+        open(src, "w").write("\n".join(lines))
+        r = subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-c", src, "-o",
+                            os.path.join(d, "abi.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-4000:]
+
+
+def test_header_compiles_as_cpp():
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "abi.cpp")
+        open(src, "w").write('#include "fj/juicer_cuda_api.h"\nint main() { FjPreparedHostData p{}; return (int)p.route; }\n')
+        r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-c", src, "-o",
+                            os.path.join(d, "abi.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-4000:]
