@@ -18,10 +18,17 @@ constexpr int kMaxLdsCurve = 512;  // curves longer than this are read from glob
 
 // Input decode + film exposure.
 struct ExposureParams {
-    const float4* tc_lut;   // [192*192] (R, G, B, pad)
+    const float4* tc_lut;   // [192*192] (R, G, B, pad); null for Mallett
     float rgb_to_xyz[9];    // input_xyz_adapt * input_rgb_to_xyz when adaptation is on
     int decode_cctf;        // 0, or the FJ_INPUT_* space to decode
     float scale;            // exp2(EV) * route correction (validated > 0)
+    // Mallett 2019 reconstruction: k pairs {basis_r*illum, basis_g*illum,
+    // basis_b*illum, 0}, {sens_B, sens_G, sens_R, 0}; rgb_to_srgb maps the
+    // input to linear sRGB; green mid-gray scale applied to the result.
+    const float4* mallett;
+    int mallett_k;
+    float mallett_scale;
+    float rgb_to_srgb[9];
 };
 
 // Film development: [x | yB | yG | yR | cxB | cxG | cxR] each n floats.

@@ -20,6 +20,7 @@ struct LdsLayout {
     float* film;        // 4n or 7n floats
     float* print;       // 4n' floats
     float4* spec;       // 2k float4
+    float4* mallett;    // 2k float4
 };
 
 template <bool kPrint>
@@ -28,10 +29,13 @@ __device__ __forceinline__ LdsLayout stage_tables(const PixelParams& p, float* l
     const int film_count = (p.dev.dir_mode ? 7 : 4) * p.dev.n;
     const int print_count = kPrint ? 4 * p.pr.n : 0;
     const int spec_count = kPrint ? 2 * p.pr.k : 0;
+    const int mallett_count = p.ex.mallett ? 2 * p.ex.mallett_k : 0;
     L.spec = reinterpret_cast<float4*>(lds);
-    L.film = lds + 4 * spec_count;
+    L.mallett = L.spec + spec_count;
+    L.film = reinterpret_cast<float*>(L.mallett + mallett_count);
     L.print = L.film + film_count;
     for (int i = threadIdx.x; i < spec_count; i += blockDim.x) L.spec[i] = p.pr.spec[i];
+    for (int i = threadIdx.x; i < mallett_count; i += blockDim.x) L.mallett[i] = p.ex.mallett[i];
     for (int i = threadIdx.x; i < film_count; i += blockDim.x) L.film[i] = p.dev.curves[i];
     for (int i = threadIdx.x; i < print_count; i += blockDim.x) L.print[i] = p.pr.curves[i];
     __syncthreads();
@@ -49,6 +53,7 @@ __global__ __launch_bounds__(256) void fused_pixel_kernel(PixelParams p, int til
         L.film = const_cast<float*>(p.dev.curves);
         L.print = const_cast<float*>(p.pr.curves);
         L.spec = const_cast<float4*>(p.pr.spec);
+        L.mallett = const_cast<float4*>(p.ex.mallett);
     }
     const int width = p.io.width;
     const long long total = static_cast<long long>(width) * p.io.height;
@@ -67,7 +72,7 @@ __global__ __launch_bounds__(256) void fused_pixel_kernel(PixelParams p, int til
             in[0] = sp[0]; in[1] = sp[1]; in[2] = sp[2]; in[3] = 1.0f;
         }
         float E[3];
-        camera_exposure(p.ex, in, E);
+        camera_exposure(p.ex, L.mallett, in, E);
         float D[3];
         errors |= develop_film(p.dev, L.film, E, D);
         if constexpr (kPrint) {
@@ -112,7 +117,8 @@ size_t fused_lds_bytes(const PixelParams& p) {
     const size_t film = static_cast<size_t>(p.dev.dir_mode ? 7 : 4) * p.dev.n;
     const size_t prn = p.print ? static_cast<size_t>(4) * p.pr.n : 0;
     const size_t spec = p.print ? static_cast<size_t>(8) * p.pr.k : 0;
-    return (film + prn + spec) * sizeof(float);
+    const size_t mallett = p.ex.mallett ? static_cast<size_t>(8) * p.ex.mallett_k : 0;
+    return (film + prn + spec + mallett) * sizeof(float);
 }
 
 hipError_t launch_clear_word(unsigned int* word, hipStream_t stream) {

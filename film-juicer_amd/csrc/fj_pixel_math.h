@@ -63,10 +63,32 @@ __device__ __forceinline__ void tc_coord(float q, int& base, float& frac) {
     }
 }
 
+// Mallett 2019 spectral reconstruction exposure in B/G/R
+// (JuicerCudaDeviceHelpers.cuh:619-668, :699-703).
+__device__ __forceinline__ void mallett_exposure(const ExposureParams& ex, const float4* table, const float rgb[3],
+                                                 float E[3]) {
+    float s[3];
+    mat3v(ex.rgb_to_srgb, rgb, s);
+    const float r = fmaxf(0.0f, fin0(s[0])), g = fmaxf(0.0f, fin0(s[1])), b = fmaxf(0.0f, fin0(s[2]));
+    float eb = 0.0f, eg = 0.0f, er = 0.0f;
+    for (int i = 0; i < ex.mallett_k; ++i) {
+        const float4 w = table[2 * i];
+        const float4 sens = table[2 * i + 1];
+        const float spd = fmaf(r, w.x, fmaf(g, w.y, b * w.z));
+        eb = fmaf(spd, sens.x, eb);
+        eg = fmaf(spd, sens.y, eg);
+        er = fmaf(spd, sens.z, er);
+    }
+    E[0] = fmaxf(0.0f, fmaxf(0.0f, (isfinite(eb) ? fmaxf(0.0f, eb) : 0.0f) * ex.mallett_scale) * ex.scale);
+    E[1] = fmaxf(0.0f, fmaxf(0.0f, (isfinite(eg) ? fmaxf(0.0f, eg) : 0.0f) * ex.mallett_scale) * ex.scale);
+    E[2] = fmaxf(0.0f, fmaxf(0.0f, (isfinite(er) ? fmaxf(0.0f, er) : 0.0f) * ex.mallett_scale) * ex.scale);
+}
+
 // Camera film linear exposure in B/G/R (JuicerCudaDeviceHelpers.cuh:509-617,
 // :709-763): XYZ -> chromaticity -> tri2quad -> bicubic TC LUT -> x sum ->
-// x exposure scale.
-__device__ __forceinline__ void camera_exposure(const ExposureParams& ex, const float in[3], float E[3]) {
+// x exposure scale; Mallett inputs take the spectral path above.
+__device__ __forceinline__ void camera_exposure(const ExposureParams& ex, const float4* mallett_table,
+                                                const float in[3], float E[3]) {
     float rgb[3];
     if (ex.decode_cctf) {
         rgb[0] = decode_channel(ex.decode_cctf, in[0]);
@@ -74,6 +96,10 @@ __device__ __forceinline__ void camera_exposure(const ExposureParams& ex, const 
         rgb[2] = decode_channel(ex.decode_cctf, in[2]);
     } else {
         rgb[0] = fin0(in[0]); rgb[1] = fin0(in[1]); rgb[2] = fin0(in[2]);
+    }
+    if (ex.mallett) {
+        mallett_exposure(ex, mallett_table, rgb, E);
+        return;
     }
     float w[3];
     mat3v(ex.rgb_to_xyz, rgb, w);

@@ -295,7 +295,6 @@ bool validate_frame(const FjFrame& f, Diag& d) {
 
 // Families this build renders; everything else is rejected loudly.
 bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
-    if (p.film_exposure.method == FJ_RAW_MALLETT_2019) return d.unsupported("Mallett 2019 reconstruction");
     if (p.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE) return d.unsupported("auto exposure metering");
     if (p.scanner_post.flags & FJ_POST_GLARE) return d.unsupported("scanner glare");
     if (p.scanner_post.hash != 0 && p.scanner_post.lens_blur_sigma_px > 0.0f) return d.unsupported("scanner lens blur");
@@ -359,6 +358,28 @@ std::vector<float> pack_print_spectra(const FjPrint& pr, int& k_out) {
         out.push_back(static_cast<float>(e[2] * kNegLog2_10));
         out.push_back(static_cast<float>(base * kNegLog2_10));
         for (int c = 0; c < 3; ++c) out.push_back(std::isnan(s[c]) ? 0.0f : static_cast<float>(double(s[c]) * ill));
+        out.push_back(0.0f);
+        ++k;
+    }
+    k_out = k;
+    if (out.empty()) out.resize(8, 0.0f);
+    return out;
+}
+
+// Mallett table (JuicerCudaDeviceHelpers.cuh:619-668): wavelengths with a
+// non-finite basis or illuminant never contribute and are dropped; the
+// illuminant is folded into the basis, NaN sensitivities become zero.
+std::vector<float> pack_mallett(const FjFilmExposure& ex, int& k_out) {
+    std::vector<float> out;
+    int k = 0;
+    for (int i = 0; i < 81; ++i) {
+        const float* b = ex.mallett_basis.data + i * 3;
+        const float ill = ex.mallett_illuminant.data[i];
+        const float* s = ex.sensitivity_rgb.data + i * 3;
+        if (!std::isfinite(b[0]) || !std::isfinite(b[1]) || !std::isfinite(b[2]) || !std::isfinite(ill)) continue;
+        for (int c = 0; c < 3; ++c) out.push_back(static_cast<float>(double(b[c]) * ill));
+        out.push_back(0.0f);
+        for (int c = 2; c >= 0; --c) out.push_back(std::isfinite(s[c]) ? s[c] : 0.0f);
         out.push_back(0.0f);
         ++k;
     }
@@ -436,7 +457,7 @@ uint64_t span_hash(const FjFloatSpan& s, uint64_t seed) {
 // ---------------------------------------------------------------------------
 // frame parameter assembly
 // ---------------------------------------------------------------------------
-enum TableKind : uint64_t { kTcLut = 1, kFilmCurves, kPrintSpec, kPrintCurves, kScanNodes, kScanCells, kGamut };
+enum TableKind : uint64_t { kTcLut = 1, kFilmCurves, kPrintSpec, kPrintCurves, kScanNodes, kScanCells, kGamut, kMallett };
 
 uint64_t table_key(TableKind kind, uint64_t content) { return hash_bytes(&content, sizeof(content), 0xF11F0000ull + kind); }
 
@@ -463,10 +484,29 @@ hipError_t assemble(DeviceState& dev, const FjPreparedHostData& p, PixelParams& 
     float scale = manual * ex.route_correction_scale;
     pp.ex.scale = std::isfinite(scale) && scale > 0.0f ? scale : 1.0f;
     const void* ptr = nullptr;
-    const uint64_t tc_id = ex.tc_lut_hash ? ex.tc_lut_hash : span_hash(ex.tc_lut_rgba, 11);
-    hipError_t e = dev.table(table_key(kTcLut, tc_id), ex.tc_lut_rgba.data, ex.tc_lut_rgba.count * sizeof(float), &ptr);
-    if (e != hipSuccess) return e;
-    pp.ex.tc_lut = static_cast<const float4*>(ptr);
+    hipError_t e = hipSuccess;
+    if (ex.method == FJ_RAW_MALLETT_2019) {
+        // JuicerCudaFilmPayloads.cpp:113-121 validates the mid-gray scale.
+        if (!std::isfinite(ex.mallett_green_midgray_scale) || !(ex.mallett_green_midgray_scale > 0.0f)) {
+            d.malformed("mallett_midgray_scale");
+            return hipSuccess;
+        }
+        float srgb[9];
+        mat_mul(ex.xyz_to_linear_srgb, pp.ex.rgb_to_xyz, srgb);
+        std::memcpy(pp.ex.rgb_to_srgb, srgb, sizeof(srgb));
+        int k = 0;
+        std::vector<float> tab = pack_mallett(ex, k);
+        e = dev.table(table_key(kMallett, hash_bytes(tab.data(), tab.size() * 4, 13)), tab.data(), tab.size() * 4, &ptr);
+        if (e != hipSuccess) return e;
+        pp.ex.mallett = static_cast<const float4*>(ptr);
+        pp.ex.mallett_k = k;
+        pp.ex.mallett_scale = ex.mallett_green_midgray_scale;
+    } else {
+        const uint64_t tc_id = ex.tc_lut_hash ? ex.tc_lut_hash : span_hash(ex.tc_lut_rgba, 11);
+        e = dev.table(table_key(kTcLut, tc_id), ex.tc_lut_rgba.data, ex.tc_lut_rgba.count * sizeof(float), &ptr);
+        if (e != hipSuccess) return e;
+        pp.ex.tc_lut = static_cast<const float4*>(ptr);
+    }
 
     // Development.
     const FjFilmDevelopment& dv = p.film_development;

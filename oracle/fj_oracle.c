@@ -184,11 +184,9 @@ static void set_message(char* msg, size_t cap, const char* text) {
 
 FjoPlan* fjo_plan_create(const FjPreparedHostData* src, char* msg, size_t cap) {
     if (!src) return NULL;
-    if (src->film_exposure.method == FJ_RAW_MALLETT_2019) {
-        set_message(msg, cap, "oracle: Mallett reconstruction not restated");
-        return NULL;
-    }
-    if (src->film_exposure.tc_lut_rgba.count != 192u * 192u * 4u) {
+    if (src->film_exposure.method == FJ_RAW_MALLETT_2019
+            ? (src->film_exposure.mallett_basis.count != 81u * 3u || src->film_exposure.mallett_illuminant.count != 81u)
+            : src->film_exposure.tc_lut_rgba.count != 192u * 192u * 4u) {
         set_message(msg, cap, "oracle: TC LUT missing");
         return NULL;
     }
@@ -388,6 +386,8 @@ typedef struct Frame {
     pthread_mutex_t lock;
 } Frame;
 
+static void tc_exposure(const FjFilmExposure* fe, const double w[3], double E[3]);
+
 /* Camera film linear exposure, B/G/R order
  * (JuicerCudaDeviceHelpers.cuh:509-617 and :709-763). */
 static void px_exposure(const FjoPlan* pl, const float* px, double E[3]) {
@@ -402,7 +402,38 @@ static void px_exposure(const FjoPlan* pl, const float* px, double E[3]) {
     } else {
         memcpy(w, xyz, sizeof(w));
     }
-    for (int c = 0; c < 3; ++c) w[c] = fin0(w[c]);
+    if (fe->method == FJ_RAW_MALLETT_2019) {
+        /* Mallett 2019 (JuicerCudaDeviceHelpers.cuh:495-507 input to linear
+         * sRGB, :619-668 81-λ exposure, :699-703 green mid-gray scale). */
+        double s[3];
+        mat3(fe->xyz_to_linear_srgb, w, s);
+        const double r = fmax(0.0, fin0(s[0])), g = fmax(0.0, fin0(s[1])), b = fmax(0.0, fin0(s[2]));
+        const float* basis = fe->mallett_basis.data;
+        const float* ill = fe->mallett_illuminant.data;
+        const float* sens = fe->sensitivity_rgb.data;
+        double eb = 0.0, eg = 0.0, er = 0.0;
+        for (int i = 0; i < NSPEC; ++i) {
+            const double spd = (r * basis[i * 3] + g * basis[i * 3 + 1] + b * basis[i * 3 + 2]) * (double)ill[i];
+            if (!isfinite(spd)) continue;
+            if (isfinite(sens[i * 3 + 2])) eb += spd * sens[i * 3 + 2];
+            if (isfinite(sens[i * 3 + 1])) eg += spd * sens[i * 3 + 1];
+            if (isfinite(sens[i * 3 + 0])) er += spd * sens[i * 3 + 0];
+        }
+        const double ms = fe->mallett_green_midgray_scale;
+        E[0] = fmax(0.0, (isfinite(eb) ? fmax(0.0, eb) : 0.0) * ms);
+        E[1] = fmax(0.0, (isfinite(eg) ? fmax(0.0, eg) : 0.0) * ms);
+        E[2] = fmax(0.0, (isfinite(er) ? fmax(0.0, er) : 0.0) * ms);
+    } else {
+        for (int c = 0; c < 3; ++c) w[c] = fin0(w[c]);
+        tc_exposure(fe, w, E);
+    }
+    double scale = pl->manual_scale * (double)fe->route_correction_scale;
+    if (!(isfinite(scale) && scale > 0.0)) scale = 1.0;
+    for (int c = 0; c < 3; ++c) E[c] = fmax(0.0, E[c] * scale);
+}
+
+/* TC-LUT exposure (Hanatos / Arctic) from working XYZ. */
+static void tc_exposure(const FjFilmExposure* fe, const double w[3], double E[3]) {
     const double sum = w[0] + w[1] + w[2];
     const double den = fmax(sum, 1e-10);
     const double x = w[0] / den, y = w[1] / den;
@@ -414,9 +445,6 @@ static void px_exposure(const FjoPlan* pl, const float* px, double E[3]) {
     E[0] = fin0(sum * fin0(rgbL[2]));
     E[1] = fin0(sum * fin0(rgbL[1]));
     E[2] = fin0(sum * fin0(rgbL[0]));
-    double scale = pl->manual_scale * (double)fe->route_correction_scale;
-    if (!(isfinite(scale) && scale > 0.0)) scale = 1.0;
-    for (int c = 0; c < 3; ++c) E[c] = fmax(0.0, E[c] * scale);
 }
 
 static void film_curves(const FjoPlan* pl, Curve dens[3], Curve comp[3]) {
