@@ -102,6 +102,24 @@ struct PixelParams {
     int print;
     int linear_out;            // write linear planes instead of encoded RGBA
     float* linear_planes;      // 3 planes of width*height when linear_out
+    // Spatial-DIR planes (width*height each): [0..2] corrections Y/M/C
+    // (raw from the source stage, filtered into the final stage), [3..5] log
+    // raw exposure B/G/R.
+    float* dir_planes[6];
+};
+
+// Pixel stage of one launch.
+enum PixelMode : int { kFused = 0, kDirSource = 1, kDirFinal = 2 };
+
+// Post stage: unsharp mask on the three linear planes, output encoding,
+// alpha copied from the source (JuicerCudaScanPipeline.cu:2733-2878).
+struct PostParams {
+    FrameIO io;
+    ColorParams co;
+    const float* planes;       // 3 planes width*height, R/G/B
+    const float* weights;      // 2r+1 unsharp weights
+    int radius;                // 0: no unsharp
+    float amount;
 };
 
 }  // namespace fjx

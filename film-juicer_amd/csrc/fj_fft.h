@@ -1,0 +1,63 @@
+// fj_fft.h — device argument blocks of the LDS Stockham FFT / DCT kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace fjx {
+
+constexpr int kMaxFftPasses = 24;
+constexpr int kMaxDctComponents = 8;
+constexpr int kMaxLdsFft = 8192;  // 2 x 8192 float2 = 128 KiB of the 160 KiB LDS
+
+struct FftPlanDev {
+    int n = 0;
+    int passes = 0;
+    int radix[kMaxFftPasses] = {};
+};
+
+// Up to three planes transformed by one launch (blockIdx.y selects the plane).
+struct DctRowsArgs {
+    FftPlanDev plan;
+    const float2* table = nullptr;    // exp(-2 pi i e / n), e < n
+    const float2* half_tw = nullptr;  // exp(-i pi k / (2n)), k < n
+    const float* in[3] = {};
+    float* out[3] = {};
+    int rows = 0;
+    long long in_stride = 0;
+    long long out_stride = 0;
+};
+
+// Separable-sum transfer function T(row, k) = sum_c weight[c] * t_rows[c][row] * t_cols[c][k].
+struct DctTransfer {
+    int count = 0;
+    float weight[kMaxDctComponents] = {};
+    const float* t_rows[kMaxDctComponents] = {};
+    const float* t_cols[kMaxDctComponents] = {};
+};
+
+struct TransposeArgs {
+    const float* in[3] = {};
+    float* out[3] = {};
+    int width = 0;   // of the input
+    int height = 0;  // of the input
+    long long in_stride = 0;
+    long long out_stride = 0;
+};
+
+// Host-side plan: radices of n (largest-first 4, then 2, 3, 5, then other primes).
+bool make_fft_plan(int n, FftPlanDev& plan);
+// exp(-2 pi i e / n) for e < n, and exp(-i pi k / 2n) for k < n, in float2 pairs.
+std::vector<float> fft_twiddles(int n);
+std::vector<float> dct_half_twiddles(int n);
+
+size_t dct_lds_bytes(int n, bool filter);
+hipError_t launch_dct2_rows(const DctRowsArgs& a, int planes, hipStream_t s);
+hipError_t launch_dct3_rows(const DctRowsArgs& a, int planes, hipStream_t s);
+hipError_t launch_dct_filter_rows(const DctRowsArgs& a, const DctTransfer& tf, int planes, hipStream_t s);
+hipError_t launch_transpose(const TransposeArgs& t, int planes, hipStream_t s);
+hipError_t fft_kernels_init();
+
+}  // namespace fjx

@@ -32,7 +32,8 @@ struct ScannerLut {
 bool build_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& diagnostic);
 
 // kernels (fj_pixel.hip)
-hipError_t launch_fused_pixel(const PixelParams& p, int grid_limit, hipStream_t stream);
+hipError_t launch_pixel_stage(const PixelParams& p, int mode, int grid_limit, hipStream_t stream);
+hipError_t launch_post(const PostParams& p, hipStream_t stream);
 hipError_t launch_clear_word(unsigned int* word, hipStream_t stream);
 size_t fused_lds_bytes(const PixelParams& p);
 

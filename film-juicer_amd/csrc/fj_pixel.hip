@@ -42,8 +42,8 @@ __device__ __forceinline__ LdsLayout stage_tables(const PixelParams& p, float* l
     return L;
 }
 
-template <bool kPrint, bool kLds>
-__global__ __launch_bounds__(256) void fused_pixel_kernel(PixelParams p, int tiles) {
+template <bool kPrint, bool kLds, int kMode>
+__global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int tiles) {
     extern __shared__ float4 lds_raw[];
     float* lds = reinterpret_cast<float*>(lds_raw);
     LdsLayout L;
@@ -63,18 +63,40 @@ __global__ __launch_bounds__(256) void fused_pixel_kernel(PixelParams p, int til
         if (idx >= total) continue;
         const int y = static_cast<int>(idx / width);
         const int x = static_cast<int>(idx - static_cast<long long>(y) * width);
-        const float* sp = reinterpret_cast<const float*>(p.io.src + y * p.io.src_row_bytes) + x * p.io.comps;
-        float in[4];
-        if (p.io.comps == 4) {
-            const float4 v = *reinterpret_cast<const float4*>(sp);
-            in[0] = v.x; in[1] = v.y; in[2] = v.z; in[3] = v.w;
-        } else {
-            in[0] = sp[0]; in[1] = sp[1]; in[2] = sp[2]; in[3] = 1.0f;
+        float in[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+        if constexpr (kMode != kDirFinal) {
+            const float* sp = reinterpret_cast<const float*>(p.io.src + y * p.io.src_row_bytes) + x * p.io.comps;
+            if (p.io.comps == 4) {
+                const float4 v = *reinterpret_cast<const float4*>(sp);
+                in[0] = v.x; in[1] = v.y; in[2] = v.z; in[3] = v.w;
+            } else {
+                in[0] = sp[0]; in[1] = sp[1]; in[2] = sp[2];
+            }
         }
-        float E[3];
-        camera_exposure(p.ex, L.mallett, in, E);
         float D[3];
-        errors |= develop_film(p.dev, L.film, E, D);
+        if constexpr (kMode == kDirSource) {
+            float E[3], corr[3], log_raw[3];
+            camera_exposure(p.ex, L.mallett, in, E);
+            errors |= dir_source(p.dev, L.film, E, corr, log_raw);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                p.dir_planes[c][idx] = corr[c];
+                p.dir_planes[3 + c][idx] = log_raw[c];
+            }
+            continue;
+        } else if constexpr (kMode == kDirFinal) {
+            float filtered[3], log_raw[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                filtered[c] = p.dir_planes[c][idx];
+                log_raw[c] = p.dir_planes[3 + c][idx];
+            }
+            errors |= develop_dir_final(p.dev, L.film, log_raw, filtered, D);
+        } else {
+            float E[3];
+            camera_exposure(p.ex, L.mallett, in, E);
+            errors |= develop_film(p.dev, L.film, E, D);
+        }
         if constexpr (kPrint) {
             print_develop(p.pr, L.spec, L.print, D);
         }
@@ -90,6 +112,10 @@ __global__ __launch_bounds__(256) void fused_pixel_kernel(PixelParams p, int til
             p.linear_planes[plane + idx] = rgb[1];
             p.linear_planes[2 * plane + idx] = rgb[2];
             continue;
+        }
+        if constexpr (kMode == kDirFinal) {
+            const float* sp = reinterpret_cast<const float*>(p.io.src + y * p.io.src_row_bytes) + x * p.io.comps;
+            if (p.io.comps == 4) in[3] = sp[3];
         }
         if (ok) encode_output(p.co, rgb);
         float* dp = reinterpret_cast<float*>(p.io.dst + y * p.io.dst_row_bytes) + x * p.io.comps;
@@ -126,7 +152,23 @@ hipError_t launch_clear_word(unsigned int* word, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_fused_pixel(const PixelParams& p, int grid_limit, hipStream_t stream) {
+template <int kMode>
+static hipError_t launch_mode(const PixelParams& p, int grid, int tiles, size_t lds, bool use_lds, hipStream_t stream) {
+    if (p.print) {
+        if (use_lds)
+            hipLaunchKernelGGL((pixel_stage_kernel<true, true, kMode>), dim3(grid), dim3(256), lds, stream, p, tiles);
+        else
+            hipLaunchKernelGGL((pixel_stage_kernel<true, false, kMode>), dim3(grid), dim3(256), 0, stream, p, tiles);
+    } else {
+        if (use_lds)
+            hipLaunchKernelGGL((pixel_stage_kernel<false, true, kMode>), dim3(grid), dim3(256), lds, stream, p, tiles);
+        else
+            hipLaunchKernelGGL((pixel_stage_kernel<false, false, kMode>), dim3(grid), dim3(256), 0, stream, p, tiles);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pixel_stage(const PixelParams& p, int mode, int grid_limit, hipStream_t stream) {
     const long long total = static_cast<long long>(p.io.width) * p.io.height;
     if (total <= 0) return hipSuccess;
     const int block = 256;
@@ -136,18 +178,21 @@ hipError_t launch_fused_pixel(const PixelParams& p, int grid_limit, hipStream_t 
     const size_t lds = fused_lds_bytes(p);
     const bool use_lds = lds <= 64 * 1024 && p.dev.n <= kMaxLdsCurve && (!p.print || p.pr.n <= kMaxLdsCurve);
     const int grid = tiles < grid_limit ? tiles : grid_limit;
-    if (p.print) {
-        if (use_lds)
-            hipLaunchKernelGGL((fused_pixel_kernel<true, true>), dim3(grid), dim3(block), lds, stream, p, tiles);
-        else
-            hipLaunchKernelGGL((fused_pixel_kernel<true, false>), dim3(grid), dim3(block), 0, stream, p, tiles);
-    } else {
-        if (use_lds)
-            hipLaunchKernelGGL((fused_pixel_kernel<false, true>), dim3(grid), dim3(block), lds, stream, p, tiles);
-        else
-            hipLaunchKernelGGL((fused_pixel_kernel<false, false>), dim3(grid), dim3(block), 0, stream, p, tiles);
+    switch (mode) {
+        case kDirSource: {
+            // The source stage needs neither print nor scanner tables.
+            PixelParams q = p;
+            q.print = 0;
+            const size_t lq = fused_lds_bytes(q);
+            if (use_lds)
+                hipLaunchKernelGGL((pixel_stage_kernel<false, true, kDirSource>), dim3(grid), dim3(block), lq, stream, q, tiles);
+            else
+                hipLaunchKernelGGL((pixel_stage_kernel<false, false, kDirSource>), dim3(grid), dim3(block), 0, stream, q, tiles);
+            return hipGetLastError();
+        }
+        case kDirFinal: return launch_mode<kDirFinal>(p, grid, tiles, lds, use_lds, stream);
+        default: return launch_mode<kFused>(p, grid, tiles, lds, use_lds, stream);
     }
-    return hipGetLastError();
 }
 
 }  // namespace fjx

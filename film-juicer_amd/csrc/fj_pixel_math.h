@@ -265,6 +265,83 @@ __device__ __forceinline__ unsigned int develop_film(const DevelopParams& dv, co
     return 0u;
 }
 
+// Spatial-DIR source: raw donor corrections (Y/M/C) and the unsanitized log
+// raw exposure (B/G/R) of one pixel (JuicerCudaFilmPipeline.cu:124-220).
+__device__ __forceinline__ unsigned int dir_source(const DevelopParams& dv, const float* curves, const float E[3],
+                                                   float corr[3], float log_raw[3]) {
+    const int n = dv.n;
+    const float* x = curves;
+    float layer[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const float raw = log10_fast(fmaxf(E[b], 0.0f) + 1e-10f);
+        log_raw[b] = raw;
+        float l = raw;
+        if (isinf(l)) l = (x[n - 1] >= x[0]) ? (l > 0.0f ? x[n - 1] : x[0]) : l;
+        layer[b] = sample_logE(x, curves + (1 + b) * n, n, l, dv.gamma_bgr[b]);
+    }
+    unsigned int fail = 0u;
+    float donors[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        donors[c] = dv.dmax[c] - layer[c];
+        if (dv.dir_mode == 1 && !langmuir(layer[c], dv.donor_k[c], dv.dref[c], donors[c])) fail |= 1u << c;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        corr[r] = fmaf(dv.M[r], donors[0], fmaf(dv.M[3 + r], donors[1], dv.M[6 + r] * donors[2]));
+        if (!isfinite(corr[r])) fail |= 1u << r;
+    }
+    return fail ? ((1u << 8) | ((fail & 7u) << 9)) : 0u;
+}
+
+// Final development against filtered corrections
+// (JuicerCudaDeviceHelpers.cuh:1255-1329). D out in C/M/Y.
+__device__ __forceinline__ unsigned int develop_dir_final(const DevelopParams& dv, const float* curves,
+                                                          const float log_raw[3], const float filtered[3], float D[3]) {
+    const int n = dv.n;
+    const float* x = curves;
+    float corr[3] = {filtered[0], filtered[1], filtered[2]};
+    unsigned int fail = 0u;
+    if (dv.dir_mode == 2) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (dv.recv_cref[c] == 0.0f) {
+                corr[c] = 0.0f;
+                continue;
+            }
+            float v;
+            if (!langmuir(corr[c], dv.recv_kr[c], dv.recv_cref[c], v)) fail |= 1u << c;
+            corr[c] = v;
+        }
+    }
+    float l[3];
+    if (fail == 0u) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float o = log_raw[c];
+            if (isnan(o)) {
+                fail |= 1u << c;
+                continue;
+            }
+            l[c] = o - corr[c];
+            if ((!isinf(o) && !isfinite(l[c])) || (isinf(o) && (!isinf(l[c]) || signbit(o) != signbit(l[c]))))
+                fail |= 1u << c;
+        }
+    }
+    if (fail) {
+        D[0] = D[1] = D[2] = __builtin_nanf("");
+        return (1u << 8) | ((fail & 7u) << 9);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        if (isinf(l[c])) l[c] = (x[n - 1] >= x[0]) ? (l[c] > 0.0f ? x[n - 1] : x[0]) : l[c];
+    D[2] = sample_logE(curves + 4 * n, curves + 1 * n, n, l[0], dv.gamma_bgr[0]);
+    D[1] = sample_logE(curves + 5 * n, curves + 2 * n, n, l[1], dv.gamma_bgr[1]);
+    D[0] = sample_logE(curves + 6 * n, curves + 3 * n, n, l[2], dv.gamma_bgr[2]);
+    return 0u;
+}
+
 // 81-λ print exposure over the compacted active wavelengths, then the print
 // curves (JuicerCudaDeviceHelpers.cuh:866-1071). A NaN transmitted sample
 // contributes zero, as in the reference.

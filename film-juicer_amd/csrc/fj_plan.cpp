@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "fj_host.h"
+#include "fj_spatial.h"
 
 namespace fjx {
 
@@ -51,7 +52,7 @@ hipError_t DeviceState::init(int dev) {
         e = hipEventCreateWithFlags(&error_events[i], hipEventDisableTiming);
         if (e != hipSuccess) return e;
     }
-    return hipSuccess;
+    return fft_kernels_init();
 }
 
 hipError_t DeviceState::release() {
@@ -297,17 +298,37 @@ bool validate_frame(const FjFrame& f, Diag& d) {
 bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
     if (p.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE) return d.unsupported("auto exposure metering");
     if (p.scanner_post.flags & FJ_POST_GLARE) return d.unsupported("scanner glare");
-    if (p.scanner_post.hash != 0 && p.scanner_post.lens_blur_sigma_px > 0.0f) return d.unsupported("scanner lens blur");
-    if (p.scanner_post.hash != 0 && p.scanner_post.unsharp_amount != 0.0f && p.scanner_post.unsharp_sigma_px > 0.0f)
-        return d.unsupported("scanner unsharp mask");
+    if (p.scanner_post.hash != 0 && p.scanner_post.lens_blur_sigma_px > 0.0f && p.scanner_post.lens_blur_radius > 0)
+        return d.unsupported("scanner lens blur");
     if (p.optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_SCATTER_HALATION |
                           FJ_OPTICS_HIGHLIGHT_BOOST))
         return d.unsupported("spatial optics");
-    if (p.spatial_dir.component_count != 0 || p.spatial_dir.hash != 0) return d.unsupported("spatial DIR");
     if (p.grain.flags & FJ_GRAIN_ACTIVE) return d.unsupported("visual grain");
     if (p.effects.flags != 0 || p.effects.hash != 0) return d.unsupported("film/gate effects");
     (void)f;
     return true;
+}
+
+bool same_extent(const FjExtent& e, const FjRect& r) {
+    return e.x == r.x1 && e.y == r.y1 && e.width == r.x2 - r.x1 && e.height == r.y2 - r.y1;
+}
+
+// Spatial DIR binding (native/juicer_cuda_prepared.cpp:192-199).
+bool check_spatial_dir(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
+    const FjSpatialDir& sd = p.spatial_dir;
+    if (sd.hash == 0) return sd.component_count == 0 || d.malformed("spatial_dir.hash");
+    if (p.dir_couplers.mode == FJ_DIR_INACTIVE) return d.malformed("spatial_dir.inactive_dir");
+    if (sd.recipe_hash != p.dir_couplers.hash || !same_extent(sd.render_extent, f.render_window) ||
+        !same_extent(sd.render_extent, f.full_frame_extent) || !same_extent(sd.filter_domain_extent, f.full_frame_extent) ||
+        !same_extent(sd.full_frame_extent, f.full_frame_extent))
+        return d.malformed("spatial_dir.frame");
+    const int w = f.render_window.x2 - f.render_window.x1, h = f.render_window.y2 - f.render_window.y1;
+    if (w > kMaxLdsFft || h > kMaxLdsFft) return d.unsupported("spatial DIR frame side above 8192 pixels");
+    return true;
+}
+
+bool post_active(const FjScannerPostEffects& sp) {
+    return sp.hash != 0 && sp.unsharp_radius > 0 && std::isfinite(sp.unsharp_amount) && sp.unsharp_amount > 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -681,7 +702,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     Diag d;
     if (!validate_prepared(prepared, d) || !validate_binding(prepared, frame, submission, d))
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
-    if (!validate_frame(frame, d) || !check_supported(prepared, frame, d))
+    if (!validate_frame(frame, d) || !check_supported(prepared, frame, d) || !check_spatial_dir(prepared, frame, d))
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
     bool invalid = false;
     if (abort_requested(abort_callback, invalid))
@@ -726,8 +747,69 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     pp.error_word = dev.error_words + slot;
     e = launch_clear_word(pp.error_word, stream);
     if (e != hipSuccess) return outcome_hip(e, "clear error word", error);
-    e = launch_fused_pixel(pp, dev.compute_units * 8, stream);
-    if (e != hipSuccess) return outcome_hip(e, "fused pixel kernel", error);
+    const int grid_limit = dev.compute_units * 8;
+    const bool spatial_dir = prepared.spatial_dir.hash != 0;
+    const bool post = post_active(prepared.scanner_post);
+    const long long plane = static_cast<long long>(pp.io.width) * pp.io.height;
+    const int planes_needed = spatial_dir ? 12 : (post ? 3 : 0);
+    float* ws = nullptr;
+    if (planes_needed) {
+        e = dev.ensure_workspace(static_cast<size_t>(plane) * planes_needed * sizeof(float));
+        if (e != hipSuccess) return outcome_hip(e, "workspace", error);
+        ws = static_cast<float*>(dev.workspace);
+    }
+    if (post) {
+        pp.linear_out = 1;
+        pp.linear_planes = ws + (spatial_dir ? 9 : 0) * plane;
+    }
+    if (spatial_dir) {
+        for (int c = 0; c < 6; ++c) pp.dir_planes[c] = ws + c * plane;
+        e = launch_pixel_stage(pp, kDirSource, grid_limit, stream);
+        if (e != hipSuccess) return outcome_hip(e, "DIR source stage", error);
+        HostTransfer ht;
+        std::string diag;
+        if (!dir_transfer(prepared.spatial_dir, pp.io.width, pp.io.height, ht, diag))
+            return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "MalformedPreparedHostData field=" + diag, error);
+        DctFilterJob job;
+        job.width = pp.io.width;
+        job.height = pp.io.height;
+        job.planes = 3;
+        const uint64_t tkey = hash_bytes(&prepared.spatial_dir.components, sizeof(prepared.spatial_dir.components),
+                                         static_cast<uint64_t>(job.width) << 32 | static_cast<uint32_t>(job.height));
+        e = upload_transfer(dev, ht, tkey, job.transfer);
+        if (e != hipSuccess) return outcome_hip(e, "DIR transfer upload", error);
+        for (int c = 0; c < 3; ++c) {
+            job.in[c] = pp.dir_planes[c];
+            job.out[c] = pp.dir_planes[c];
+            job.tmp_a[c] = ws + (6 + c) * plane;
+            job.tmp_b[c] = ws + (9 + c) * plane;
+        }
+        e = run_dct_filter(dev, job, stream);
+        if (e != hipSuccess) return outcome_hip(e, "DIR spatial filter", error);
+        e = launch_pixel_stage(pp, kDirFinal, grid_limit, stream);
+        if (e != hipSuccess) return outcome_hip(e, "DIR final stage", error);
+    } else {
+        e = launch_pixel_stage(pp, kFused, grid_limit, stream);
+        if (e != hipSuccess) return outcome_hip(e, "fused pixel kernel", error);
+    }
+    if (post) {
+        PostParams po{};
+        po.io = pp.io;
+        po.co = pp.co;
+        po.planes = pp.linear_planes;
+        const FjScannerPostEffects& sp = prepared.scanner_post;
+        std::vector<float> w = gaussian_weights(sp.unsharp_radius, sp.unsharp_sigma_px);
+        if (w.empty() || sp.unsharp_radius > 16)
+            return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput unsharp radius/sigma", error);
+        const void* wptr = nullptr;
+        e = dev.table(hash_bytes(w.data(), w.size() * 4, 0x5057ull), w.data(), w.size() * 4, &wptr);
+        if (e != hipSuccess) return outcome_hip(e, "unsharp weights", error);
+        po.weights = static_cast<const float*>(wptr);
+        po.radius = sp.unsharp_radius;
+        po.amount = sp.unsharp_amount;
+        e = launch_post(po, stream);
+        if (e != hipSuccess) return outcome_hip(e, "post stage", error);
+    }
     e = hipMemcpyAsync(dev.error_host + slot, pp.error_word, sizeof(unsigned int), hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) return outcome_hip(e, "error readback", error);
     e = hipEventRecord(dev.error_events[slot], stream);

@@ -736,6 +736,236 @@ static void px_encode(const FjOutputColor* oc, double rgb[3]) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* spatial operators                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Gaussian FIR weights (rust optics/gaussian.rs:20-56): double samples
+ * rounded to float, normalized by the unrounded double sum. */
+int fjo_gaussian_weights(int radius, float sigma, float* out) {
+    if (radius < 1 || !(sigma > 0.0f) || !isfinite(sigma)) return 1;
+    const double s2 = (double)sigma * (double)sigma * 2.0;
+    double sum = 0.0;
+    for (int i = -radius; i <= radius; ++i) {
+        const double v = exp(-((double)i * i) / s2);
+        out[i + radius] = (float)v;
+        sum += v;
+    }
+    const double inv = sum != 0.0 ? 1.0 / sum : 0.0;
+    for (int i = 0; i <= 2 * radius; ++i) out[i] = (float)((double)out[i] * inv);
+    return 0;
+}
+
+/* Half-sample reflection with period 2n (JuicerCudaFilmPipeline.cu:82-92). */
+static int reflect_periodic(long i, int n) {
+    if (n <= 1) return 0;
+    const long period = 2L * n;
+    long w = i % period;
+    if (w < 0) w += period;
+    return (int)(w < n ? w : period - 1 - w);
+}
+
+/* FIR on a strided line with reflected borders (JuicerCudaFilmPipeline.cu:388-436). */
+static void fir_line(const double* in, double* out, int n, ptrdiff_t stride, const float* w, int r) {
+    for (int i = 0; i < n; ++i) {
+        double acc = 0.0;
+        for (int o = -r; o <= r; ++o) acc += in[reflect_periodic(i + o, n) * stride] * (double)w[o + r];
+        out[i * stride] = isfinite(acc) ? acc : 0.0;
+    }
+}
+
+static void invert3l(long double m[3][3], int size, long double inv[3][3]) {
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) inv[r][c] = r == c ? 1.0L : 0.0L;
+    for (int col = 0; col < size; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < size; ++r)
+            if (fabsl(m[r][col]) > fabsl(m[piv][col])) piv = r;
+        if (piv != col)
+            for (int c = 0; c < 3; ++c) {
+                long double t = m[piv][c]; m[piv][c] = m[col][c]; m[col][c] = t;
+                t = inv[piv][c]; inv[piv][c] = inv[col][c]; inv[col][c] = t;
+            }
+        const long double sc = m[col][col];
+        for (int c = 0; c < size; ++c) {
+            m[col][c] /= sc;
+            inv[col][c] /= sc;
+        }
+        for (int r = 0; r < size; ++r) {
+            if (r == col) continue;
+            const long double f = m[r][col];
+            for (int c = 0; c < size; ++c) {
+                m[r][c] -= f * m[col][c];
+                inv[r][c] -= f * inv[col][c];
+            }
+        }
+    }
+}
+
+int fjo_yvv_boundary(const double c[9], int n, double* weights, double* terminal) {
+    /* Extended precision (x87 long double) stands in for the reference's
+     * double-double arithmetic; the terminal solve is ill-conditioned for
+     * long kernels on short lines. */
+    const int period = 2 * n;
+    long double T[3][3] = {{c[1], c[2], c[3]}, {1, 0, 0}, {0, 1, 0}};
+    long double* pw = (long double*)malloc(sizeof(long double) * 3 * (size_t)period);
+    pw[0] = c[0];
+    pw[1] = 0;
+    pw[2] = 0;
+    for (int e = 1; e < period; ++e)
+        for (int r = 0; r < 3; ++r)
+            pw[e * 3 + r] = T[r][0] * pw[(e - 1) * 3] + T[r][1] * pw[(e - 1) * 3 + 1] + T[r][2] * pw[(e - 1) * 3 + 2];
+    long double P[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int e = 0; e < period; ++e) {
+        long double Q[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) Q[r][k] = T[r][0] * P[0][k] + T[r][1] * P[1][k] + T[r][2] * P[2][k];
+        memcpy(P, Q, sizeof(P));
+    }
+    long double S[3][3], Si[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) S[r][k] = (r == k ? 1.0L : 0.0L) - P[r][k];
+    invert3l(S, 3, Si);
+    for (int s = 0; s < n; ++s) {
+        long double rhs[3];
+        for (int r = 0; r < 3; ++r) rhs[r] = pw[s * 3 + r] + pw[(period - 1 - s) * 3 + r];
+        for (int r = 0; r < 3; ++r) weights[r * n + s] = (double)(Si[r][0] * rhs[0] + Si[r][1] * rhs[1] + Si[r][2] * rhs[2]);
+    }
+    free(pw);
+    const int size = n < 3 ? n : 3;
+    long double Ts[3][3] = {{0}}, Ti[3][3];
+    for (int r = 0; r < size; ++r) Ts[r][r] = c[4];
+    for (int r = 0; r < size; ++r) {
+        const long o = n - size + r;
+        for (int k = 0; k < 3; ++k) {
+            const int col = reflect_periodic(o + k + 1, n) - (n - size);
+            if (col >= 0 && col < size) Ts[r][col] -= c[6 + k];
+        }
+    }
+    invert3l(Ts, size, Ti);
+    for (int r = 0; r < size; ++r)
+        for (int k = 0; k < size; ++k) terminal[r * size + k] = (double)((long double)c[5] * Ti[r][k]);
+    return size;
+}
+
+/* Reflected YVV recursion on one strided line (JuicerCudaFilmPipeline.cu:478-553
+ * operator): the causal then anti-causal recursion applied to the 2n-periodic
+ * half-sample reflection of the line. The infinite extension is realized by
+ * running both recursions over M reflected samples on either side, M chosen
+ * so that the slowest pole has decayed below 1e-18 (equivalent to the
+ * reference's solved boundary states without its ill-conditioned solve). */
+void fjo_yvv_reflect_line(const double* in, double* out, int n, ptrdiff_t stride, const double c[9]) {
+    /* decay length of the recursion's impulse response */
+    double s0 = 1.0, s1 = 0.0, s2 = 0.0, peak = 1.0;
+    long M = 3;
+    for (long m = 0; m < 2000000; ++m) {
+        const double v = c[1] * s0 + c[2] * s1 + c[3] * s2;
+        s2 = s1;
+        s1 = s0;
+        s0 = v;
+        const double mag = fabs(s0) + fabs(s1) + fabs(s2);
+        if (mag > peak) peak = mag;
+        if (mag < 1e-18 * peak) {
+            M = m + 4;
+            break;
+        }
+        M = m + 4;
+    }
+    const long total = 2 * M + n;
+    double* ext = (double*)malloc(sizeof(double) * (size_t)total);
+    double* y = (double*)malloc(sizeof(double) * (size_t)total);
+    for (long i = 0; i < total; ++i) ext[i] = in[reflect_periodic(i - M, n) * stride];
+    double a = 0, b = 0, d = 0;
+    for (long i = 0; i < total; ++i) {
+        const double v = c[0] * ext[i] + c[1] * a + c[2] * b + c[3] * d;
+        y[i] = v;
+        d = b;
+        b = a;
+        a = v;
+    }
+    a = b = d = 0;
+    for (long i = total - 1; i >= 0; --i) {
+        const double v = c[0] * y[i] + c[1] * a + c[2] * b + c[3] * d;
+        if (i >= M && i < M + n) out[(i - M) * stride] = v;
+        d = b;
+        b = a;
+        a = v;
+    }
+    free(ext);
+    free(y);
+}
+
+/* Weighted sum of separable components on `planes` W x H double planes, in
+ * place (JuicerCudaFilmPipeline.cu:798-1478 schedule, FP64 throughout). */
+int fjo_dir_filter(const FjSpatialDir* sd, double* const* planes, int nplanes, int W, int H) {
+    if (!sd || sd->component_count < 1 || sd->component_count > 4) return 1;
+    const size_t N = (size_t)W * H;
+    double* acc = (double*)calloc(N, sizeof(double));
+    double* tmp = (double*)malloc(sizeof(double) * N);
+    double* tmp2 = (double*)malloc(sizeof(double) * N);
+    float w[8193];
+    for (int p = 0; p < nplanes; ++p) {
+        memset(acc, 0, sizeof(double) * N);
+        for (size_t ci = 0; ci < sd->component_count; ++ci) {
+            const FjDirGaussian* g = &sd->components[ci];
+            if (!(g->weight > 0.0f)) continue;
+            if (g->reference_operator == FJ_DIR_OPERATOR_IDENTITY) {
+                for (size_t i = 0; i < N; ++i) acc[i] += (double)g->weight * planes[p][i];
+                continue;
+            }
+            if (g->reference_operator == FJ_DIR_OPERATOR_FIR_REFLECT) {
+                if (g->radius > 4096 || fjo_gaussian_weights(g->radius, g->sigma_px, w)) return 2;
+                for (int y = 0; y < H; ++y) fir_line(planes[p] + (size_t)y * W, tmp + (size_t)y * W, W, 1, w, g->radius);
+                for (int x = 0; x < W; ++x) fir_line(tmp + x, tmp2 + x, H, W, w, g->radius);
+            } else if (g->reference_operator == FJ_DIR_OPERATOR_YVV_REFLECT) {
+                const double c[9] = {g->feedforward, g->feedback[0], g->feedback[1], g->feedback[2],
+                                     g->normalization_denominator, g->feedforward_numerator,
+                                     g->feedback_numerators[0], g->feedback_numerators[1], g->feedback_numerators[2]};
+                for (int y = 0; y < H; ++y) fjo_yvv_reflect_line(planes[p] + (size_t)y * W, tmp + (size_t)y * W, W, 1, c);
+                for (int x = 0; x < W; ++x) fjo_yvv_reflect_line(tmp + x, tmp2 + x, H, W, c);
+            } else {
+                return 3;
+            }
+            for (size_t i = 0; i < N; ++i) acc[i] += (double)g->weight * tmp2[i];
+        }
+        memcpy(planes[p], acc, sizeof(double) * N);
+    }
+    free(acc);
+    free(tmp);
+    free(tmp2);
+    return 0;
+}
+
+/* Unsharp mask on one W x H double plane (JuicerCudaScanPipeline.cu:316-452,
+ * :586-660): half-sample reflected horizontal and vertical blur, non-finite
+ * blurs replaced by zero, v + a (v - blur). */
+static int reflect_half(long i, int n) {
+    if (n <= 1) return 0;
+    while (i < 0 || i >= n) i = i < 0 ? -i - 1 : 2L * n - i - 1;
+    return (int)i;
+}
+
+static void unsharp_plane(double* v, int W, int H, const float* w, int r, double amount) {
+    const size_t N = (size_t)W * H;
+    double* h = (double*)malloc(sizeof(double) * N);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double a = 0.0;
+            for (int o = -r; o <= r; ++o) a += v[(size_t)y * W + reflect_half(x + o, W)] * (double)w[o + r];
+            h[(size_t)y * W + x] = isfinite(a) ? a : 0.0;
+        }
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double a = 0.0;
+            for (int o = -r; o <= r; ++o) a += h[(size_t)reflect_half(y + o, H) * W + x] * (double)w[o + r];
+            const double b = isfinite(a) ? a : 0.0;
+            const double v0 = v[(size_t)y * W + x];
+            const double o = v0 + amount * (v0 - b);
+            v[(size_t)y * W + x] = isfinite(o) ? o : 0.0;
+        }
+    free(h);
+}
+
+/* ------------------------------------------------------------------------ */
 /* schedule                                                                 */
 /* ------------------------------------------------------------------------ */
 
@@ -778,34 +1008,128 @@ static void add_errors(Frame* f, uint32_t bits) {
     pthread_mutex_unlock(&f->lock);
 }
 
-/* Fused per-pixel chain for frames without spatial stages. */
+/* Print + scan + colour of one developed pixel into the linear planes. */
+static uint32_t px_back(const FjoPlan* pl, Frame* f, size_t idx, double D[3]) {
+    double rgb[3];
+    if (pl->print) px_print(pl, D);
+    const int ok = px_scan(pl, D, rgb);
+    if (!ok) rgb[0] = rgb[1] = rgb[2] = 0.0;
+    memcpy(f->linear + idx * 3, rgb, sizeof(rgb));
+    return ok ? 0u : 1u;
+}
+
+/* Non-spatial chain: exposure -> development -> back end. */
 static void rows_chain(Frame* f, int y0, int y1) {
     const FjoPlan* pl = f->plan;
     uint32_t errs = 0;
-    for (int y = y0; y < y1; ++y) {
+    for (int y = y0; y < y1; ++y)
         for (int x = 0; x < f->w; ++x) {
             const float* px = f->src + y * f->src_stride + (ptrdiff_t)x * f->comps;
-            double E[3], D[3], rgb[3];
+            double E[3], D[3];
             px_exposure(pl, px, E);
             errs |= px_develop(pl, E, D);
-            if (pl->print) px_print(pl, D);
-            const size_t idx = (size_t)y * f->w + x;
-            int ok = px_scan(pl, D, rgb);
-            if (!ok) {
-                errs |= 1u;
-                rgb[0] = rgb[1] = rgb[2] = 0.0;
-            }
-            if (f->linear) memcpy(f->linear + idx * 3, rgb, sizeof(rgb));
-            if (f->dst) {
-                float* o = f->dst + y * f->dst_stride + (ptrdiff_t)x * f->comps;
-                if (ok) px_encode(&pl->p.output_color, rgb);
-                o[0] = (float)rgb[0];
-                o[1] = (float)rgb[1];
-                o[2] = (float)rgb[2];
-                if (f->comps == 4) o[3] = px[3];
-            }
+            errs |= px_back(pl, f, (size_t)y * f->w + x, D);
         }
+    add_errors(f, errs);
+}
+
+/* Spatial-DIR source (JuicerCudaFilmPipeline.cu:124-220): raw donor
+ * corrections (Y/M/C) and unsanitized log raw exposure (B/G/R) planes. */
+static void rows_dir_source(Frame* f, int y0, int y1) {
+    const FjoPlan* pl = f->plan;
+    const FjDirCouplers* dir = &pl->p.dir_couplers;
+    const FjFilmDevelopment* fd = &pl->p.film_development;
+    Curve dens[3], comp[3];
+    film_curves(pl, dens, comp);
+    const double gamma[3] = {fd->gamma_rgb[2], fd->gamma_rgb[1], fd->gamma_rgb[0]};
+    double M[9], dmax[3], dref[3], dk[3];
+    for (int d = 0; d < 3; ++d) {
+        for (int r = 0; r < 3; ++r) M[d * 3 + r] = dir->matrix_rgb[(2 - d) * 3 + (2 - r)];
+        dmax[d] = dir->density_max_rgb[2 - d];
+        dref[d] = dir->density_ref_rgb[2 - d];
+        dk[d] = dir->donor_k_rgb[2 - d];
     }
+    uint32_t errs = 0;
+    const size_t N = (size_t)f->w * f->h;
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < f->w; ++x) {
+            const float* px = f->src + y * f->src_stride + (ptrdiff_t)x * f->comps;
+            const size_t idx = (size_t)y * f->w + x;
+            double E[3], layer[3], donors[3];
+            px_exposure(pl, px, E);
+            uint32_t fail = 0;
+            for (int b = 0; b < 3; ++b) {
+                const double raw = log10(fmax(E[b], 0.0) + 1e-10);
+                f->plane[3 + b][idx] = raw;
+                layer[b] = curve_at(&dens[b], sanitize_inf_logE(raw, &dens[b]), gamma[b]);
+            }
+            for (int c = 0; c < 3; ++c) {
+                donors[c] = dmax[c] - layer[c];
+                if (dir->mode == FJ_DIR_NEGATIVE_DONOR_LANGMUIR && !langmuir(layer[c], dk[c], dref[c], &donors[c])) fail |= 1u << c;
+            }
+            for (int r = 0; r < 3; ++r) {
+                const double v = M[r] * donors[0] + M[3 + r] * donors[1] + M[6 + r] * donors[2];
+                if (!isfinite(v)) fail |= 1u << r;
+                f->plane[r][idx] = v;
+            }
+            if (fail) errs |= (1u << 8) | ((fail & 7u) << 9);
+        }
+    (void)N;
+    add_errors(f, errs);
+}
+
+/* Final development against filtered corrections
+ * (JuicerCudaDeviceHelpers.cuh:1255-1329), then the back end. */
+static void rows_dir_final(Frame* f, int y0, int y1) {
+    const FjoPlan* pl = f->plan;
+    const FjDirCouplers* dir = &pl->p.dir_couplers;
+    const FjFilmDevelopment* fd = &pl->p.film_development;
+    Curve dens[3], comp[3];
+    film_curves(pl, dens, comp);
+    const double gamma[3] = {fd->gamma_rgb[2], fd->gamma_rgb[1], fd->gamma_rgb[0]};
+    double cref[3], kr[3];
+    for (int d = 0; d < 3; ++d) {
+        cref[d] = dir->receiver_c_ref_rgb[2 - d];
+        kr[d] = dir->receiver_kr_rgb[2 - d];
+    }
+    uint32_t errs = 0;
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < f->w; ++x) {
+            const size_t idx = (size_t)y * f->w + x;
+            double corr[3], l[3], D[3];
+            uint32_t fail = 0;
+            for (int c = 0; c < 3; ++c) corr[c] = f->plane[c][idx];
+            if (dir->mode == FJ_DIR_POSITIVE_RECEIVER_LANGMUIR)
+                for (int c = 0; c < 3; ++c) {
+                    if (cref[c] == 0.0) {
+                        corr[c] = 0.0;
+                        continue;
+                    }
+                    double v;
+                    if (!langmuir(corr[c], kr[c], cref[c], &v)) fail |= 1u << c;
+                    corr[c] = v;
+                }
+            if (!fail)
+                for (int c = 0; c < 3; ++c) {
+                    const double o = f->plane[3 + c][idx];
+                    if (isnan(o)) {
+                        fail |= 1u << c;
+                        continue;
+                    }
+                    l[c] = o - corr[c];
+                    if (!isfinite(l[c]) && isfinite(o)) fail |= 1u << c;
+                }
+            if (fail) {
+                errs |= (1u << 8) | ((fail & 7u) << 9);
+                D[0] = D[1] = D[2] = NAN;
+            } else {
+                for (int c = 0; c < 3; ++c) l[c] = sanitize_inf_logE(l[c], &dens[c]);
+                D[2] = curve_at(&comp[0], l[0], gamma[0]);
+                D[1] = curve_at(&comp[1], l[1], gamma[1]);
+                D[0] = curve_at(&comp[2], l[2], gamma[2]);
+            }
+            errs |= px_back(pl, f, idx, D);
+        }
     add_errors(f, errs);
 }
 
@@ -814,11 +1138,10 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
                uint32_t* error_flags) {
     if (!plan || !src || width <= 0 || height <= 0 || (comps != 3 && comps != 4)) return 2;
     const FjPreparedHostData* p = &plan->p;
-    const int spatial = (p->optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION |
-                                            FJ_OPTICS_SCATTER_HALATION)) != 0 ||
-                        p->spatial_dir.component_count != 0 || (p->grain.flags & FJ_GRAIN_ACTIVE) != 0 ||
-                        (p->scanner_post.hash != 0 && p->scanner_post.unsharp_amount != 0.0f);
-    if (spatial) return 3;
+    if ((p->optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_SCATTER_HALATION)) ||
+        (p->grain.flags & FJ_GRAIN_ACTIVE))
+        return 3;
+    const size_t N = (size_t)width * height;
     Frame f;
     memset(&f, 0, sizeof(f));
     f.plan = plan;
@@ -829,10 +1152,51 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     f.src_stride = src_stride;
     f.dst = dst;
     f.dst_stride = dst_stride;
-    f.linear = linear;
+    f.linear = (double*)malloc(sizeof(double) * N * 3);
     pthread_mutex_init(&f.lock, NULL);
-    run_rows(&f, rows_chain, threads);
+    int rc = 0;
+    if (p->spatial_dir.hash != 0) {
+        for (int i = 0; i < 6; ++i) f.plane[i] = (double*)malloc(sizeof(double) * N);
+        run_rows(&f, rows_dir_source, threads);
+        rc = fjo_dir_filter(&p->spatial_dir, f.plane, 3, width, height);
+        if (rc == 0) run_rows(&f, rows_dir_final, threads);
+        for (int i = 0; i < 6; ++i) free(f.plane[i]);
+    } else {
+        run_rows(&f, rows_chain, threads);
+    }
+    const FjScannerPostEffects* sp = &p->scanner_post;
+    if (rc == 0 && sp->hash != 0 && sp->unsharp_radius > 0 && isfinite(sp->unsharp_amount) && sp->unsharp_amount > 0.0f) {
+        float w[8193];
+        if (sp->unsharp_radius > 4096 || fjo_gaussian_weights(sp->unsharp_radius, sp->unsharp_sigma_px, w)) {
+            rc = 4;
+        } else {
+            double* pl3 = (double*)malloc(sizeof(double) * N);
+            for (int c = 0; c < 3; ++c) {
+                for (size_t i = 0; i < N; ++i) pl3[i] = f.linear[i * 3 + c];
+                unsharp_plane(pl3, width, height, w, sp->unsharp_radius, sp->unsharp_amount);
+                for (size_t i = 0; i < N; ++i) f.linear[i * 3 + c] = pl3[i];
+            }
+            free(pl3);
+        }
+    }
+    if (rc == 0) {
+        if (linear) memcpy(linear, f.linear, sizeof(double) * N * 3);
+        if (dst)
+            for (int y = 0; y < height; ++y)
+                for (int x = 0; x < width; ++x) {
+                    double rgb[3];
+                    memcpy(rgb, f.linear + ((size_t)y * width + x) * 3, sizeof(rgb));
+                    px_encode(&p->output_color, rgb);
+                    float* o = dst + y * dst_stride + (ptrdiff_t)x * comps;
+                    const float* px = src + y * src_stride + (ptrdiff_t)x * comps;
+                    o[0] = (float)rgb[0];
+                    o[1] = (float)rgb[1];
+                    o[2] = (float)rgb[2];
+                    if (comps == 4) o[3] = px[3];
+                }
+    }
+    free(f.linear);
     pthread_mutex_destroy(&f.lock);
     if (error_flags) *error_flags = f.errors;
-    return 0;
+    return rc;
 }

@@ -37,6 +37,21 @@ int fjo_scanner_lut_build(const float* eps_c, const float* eps_m, const float* e
                           float* cell_min, float* cell_max,
                           uint64_t* fail_index, uint32_t* fail_channel);
 
+/* Gaussian FIR weights of 2*radius+1 taps (rust optics/gaussian.rs:20-56). */
+int fjo_gaussian_weights(int radius, float sigma, float* out);
+
+/* Reflected-YVV boundary solve for a line of n samples: weights[3*n]
+ * (start-state rows) and terminal[size*size]; returns size = min(n, 3). */
+int fjo_yvv_boundary(const double c[9], int n, double* weights, double* terminal);
+
+/* Reflected third-order recursive filter on one strided line of n doubles
+ * (forward + backward pass with the periodic boundary solve of rust
+ * film/dir/boundary.rs). c = {b, a1, a2, a3, D, N, n1, n2, n3}. */
+void fjo_yvv_reflect_line(const double* in, double* out, int n, ptrdiff_t stride, const double c[9]);
+
+/* Spatial DIR operator on `nplanes` W x H double planes in place. */
+int fjo_dir_filter(const FjSpatialDir* sd, double* const* planes, int nplanes, int W, int H);
+
 typedef struct FjoPlan FjoPlan;
 
 /* Builds host state for one prepared record (scanner LUT, curves). Returns

@@ -35,6 +35,11 @@ def lib():
         L.fjo_plan_create.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
         L.fjo_plan_destroy.argtypes = [C.c_void_p]
         L.fjo_plan_scanner_lut.argtypes = [C.c_void_p, C.POINTER(C.c_int)] + [C.POINTER(fp)] * 6
+        L.fjo_gaussian_weights.argtypes = [C.c_int, C.c_float, fp]
+        dp = C.POINTER(C.c_double)
+        L.fjo_yvv_reflect_line.argtypes = [dp, dp, C.c_int, C.c_ssize_t, dp]
+        L.fjo_yvv_boundary.argtypes = [dp, C.c_int, dp, dp]
+        L.fjo_dir_filter.argtypes = [C.c_void_p, C.POINTER(dp), C.c_int, C.c_int, C.c_int]
         L.fjo_render.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t, fp, C.c_ssize_t,
                                  C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_uint32)]
         _lib = L
@@ -67,6 +72,43 @@ def scanner_lut(eps_c, eps_m, eps_y, wx, wy, wz, baseline, inv_yn, range_min, in
                                      _fp(arr(range_min)), _fp(arr(inv_span)), medium, res,
                                      *[_fp(o) for o in outs], C.byref(fi), C.byref(fc))
     return rc, outs, (fi.value, fc.value)
+
+
+def gaussian_weights(radius: int, sigma: float) -> np.ndarray:
+    out = np.zeros(2 * radius + 1, np.float32)
+    if lib().fjo_gaussian_weights(radius, sigma, _fp(out)) != 0:
+        raise ValueError("invalid Gaussian kernel")
+    return out
+
+
+def yvv_reflect_line(x, coefficients) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    c = np.ascontiguousarray(coefficients, dtype=np.float64)
+    dp = C.POINTER(C.c_double)
+    lib().fjo_yvv_reflect_line(x.ctypes.data_as(dp), out.ctypes.data_as(dp), x.size, 1, c.ctypes.data_as(dp))
+    return out
+
+
+def yvv_boundary(coefficients, n: int):
+    c = np.ascontiguousarray(coefficients, dtype=np.float64)
+    w = np.zeros(3 * n, np.float64)
+    t = np.zeros(9, np.float64)
+    dp = C.POINTER(C.c_double)
+    size = lib().fjo_yvv_boundary(c.ctypes.data_as(dp), n, w.ctypes.data_as(dp), t.ctypes.data_as(dp))
+    return w.reshape(3, n), t[: size * size].reshape(size, size)
+
+
+def dir_filter(spatial_dir, planes) -> list:
+    """Applies the spatial DIR operator (FP64) to a list of HxW planes."""
+    arrs = [np.ascontiguousarray(p, dtype=np.float64).copy() for p in planes]
+    h, w = arrs[0].shape
+    dp = C.POINTER(C.c_double)
+    ptrs = (dp * len(arrs))(*[a.ctypes.data_as(dp) for a in arrs])
+    rc = lib().fjo_dir_filter(C.addressof(spatial_dir), ptrs, len(arrs), w, h)
+    if rc != 0:
+        raise RuntimeError(f"oracle dir filter rc={rc}")
+    return arrs
 
 
 class Plan:

@@ -171,11 +171,33 @@ def scanner_luts() -> None:
         json.dump({"cases": meta, "slopes": slopes}, fh, indent=0)
 
 
+def dir_fixtures() -> None:
+    """Spatial-DIR frame math and reflected-YVV boundary words
+    (tests/dir/frame-fixtures, tests/dir/boundary-fixtures of the reference)."""
+    frame = json.load(open(f"{REF}/tests/dir/frame-fixtures/linux-release.json"))
+    rows = []
+    for group in ("cases", "products"):
+        for c in frame[group]:
+            rows.append({"group": group, "input": c["input"], "native": c["native"]})
+    with open(os.path.join(HERE, "dir_frame_cases.json"), "w") as fh:
+        json.dump(rows, fh)
+    bound = json.load(open(f"{REF}/tests/dir/boundary-fixtures/linux-release.json"))
+    keep = []
+    for c in bound["cases"]:
+        if not c["success"]:
+            continue
+        keep.append({"input": c["input"], "horizontal": c["horizontal"], "vertical": c["vertical"]})
+    with open(os.path.join(HERE, "dir_boundary_cases.json"), "w") as fh:
+        json.dump(keep, fh)
+
+
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["abi", "prepared", "lut"]
+    what = sys.argv[1:] or ["abi", "prepared", "lut", "dir"]
     if "abi" in what:
         abi_facts()
     if "prepared" in what:
         prepared_cases()
     if "lut" in what:
         scanner_luts()
+    if "dir" in what:
+        dir_fixtures()
