@@ -66,6 +66,7 @@ FjStatus fail(FjErrorBuffer* error, uint32_t category, const std::string& text, 
 }
 
 FjStatus hip_fail(FjErrorBuffer* error, hipError_t e, const char* where) {
+    (void)hipGetLastError();  // do not leave a sticky error for the caller's next HIP call
     std::string text = std::string("HIP failure in ") + where + ": " + hipGetErrorString(e);
     const uint32_t category = (e == hipErrorOutOfMemory) ? FJ_STATUS_ALLOCATION_FAILURE : FJ_STATUS_CUDA_FAILURE;
     return fail(error, category, text, FJ_API_CUDA_RUNTIME, static_cast<int32_t>(e));

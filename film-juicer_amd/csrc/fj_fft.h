@@ -58,6 +58,6 @@ hipError_t launch_dct2_rows(const DctRowsArgs& a, int planes, hipStream_t s);
 hipError_t launch_dct3_rows(const DctRowsArgs& a, int planes, hipStream_t s);
 hipError_t launch_dct_filter_rows(const DctRowsArgs& a, const DctTransfer& tf, int planes, hipStream_t s);
 hipError_t launch_transpose(const TransposeArgs& t, int planes, hipStream_t s);
-hipError_t fft_kernels_init();
+hipError_t fft_kernels_init(int* max_lds_n);
 
 }  // namespace fjx

@@ -237,7 +237,6 @@ __global__ __launch_bounds__(256) void dct_filter_rows_kernel(DctRowsArgs args, 
     const int n = plan.n;
     float2* a = lds2;
     float2* b = lds2 + n;
-    float* spec = reinterpret_cast<float*>(lds2 + 2 * n);  // 2n floats: X_a, X_b
     const int row0 = 2 * blockIdx.x;
     const int plane = blockIdx.y;
     if (row0 >= args.rows) return;
@@ -256,23 +255,23 @@ __global__ __launch_bounds__(256) void dct_filter_rows_kernel(DctRowsArgs args, 
         fa[c] = tf.weight[c] * tf.t_rows[c][row0];
         fb[c] = has1 ? tf.weight[c] * tf.t_rows[c][row0 + 1] : 0.0f;
     }
-    {
-        float2* V = fft_lds(a, b, plan, args.table);
-        for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            const float2 z = V[k];
-            const float2 zc = V[k == 0 ? 0 : n - k];
-            const float2 va = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
-            const float2 vb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
-            const float2 w = args.half_tw[k];
-            float ta = 0.0f, tb = 0.0f;
-            for (int c = 0; c < tf.count; ++c) {
-                const float tk = tf.t_cols[c][k];
-                ta = fmaf(fa[c], tk, ta);
-                tb = fmaf(fb[c], tk, tb);
-            }
-            spec[k] = (w.x * va.x - w.y * va.y) * ta;
-            spec[n + k] = (w.x * vb.x - w.y * vb.y) * tb;
+    float2* V = fft_lds(a, b, plan, args.table);
+    float2* other = V == a ? b : a;
+    float* spec = reinterpret_cast<float*>(other);  // n floats X_a, then n floats X_b
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const float2 z = V[k];
+        const float2 zc = V[k == 0 ? 0 : n - k];
+        const float2 va = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+        const float2 vb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+        const float2 w = args.half_tw[k];
+        float ta = 0.0f, tb = 0.0f;
+        for (int c = 0; c < tf.count; ++c) {
+            const float tk = tf.t_cols[c][k];
+            ta = fmaf(fa[c], tk, ta);
+            tb = fmaf(fb[c], tk, tb);
         }
+        spec[k] = (w.x * va.x - w.y * va.y) * ta;
+        spec[n + k] = (w.x * vb.x - w.y * vb.y) * tb;
     }
     __syncthreads();
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -284,10 +283,10 @@ __global__ __launch_bounds__(256) void dct_filter_rows_kernel(DctRowsArgs args, 
         const float2 va = cmul(wc, make_float2(xa, -xam));
         const float2 vb = cmul(wc, make_float2(xb, -xbm));
         const float2 v = make_float2(va.x - vb.y, va.y + vb.x);
-        a[k] = make_float2(v.x, -v.y);
+        V[k] = make_float2(v.x, -v.y);
     }
     __syncthreads();
-    float2* res = fft_lds(a, b, plan, args.table);
+    float2* res = fft_lds(V, other, plan, args.table);
     const float inv = 1.0f / static_cast<float>(n);
     float* o0 = out + static_cast<long long>(row0) * args.out_stride;
     float* o1 = has1 ? out + static_cast<long long>(row0 + 1) * args.out_stride : nullptr;
@@ -320,7 +319,7 @@ __global__ __launch_bounds__(256) void transpose_kernel(TransposeArgs t) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-size_t dct_lds_bytes(int n, bool filter) { return static_cast<size_t>(n) * sizeof(float2) * 2 + (filter ? n * 2 * sizeof(float) : 0); }
+size_t dct_lds_bytes(int n, bool) { return static_cast<size_t>(n) * sizeof(float2) * 2; }
 
 hipError_t launch_dct2_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
     const dim3 grid((a.rows + 1) / 2, planes);
@@ -350,17 +349,21 @@ hipError_t launch_transpose(const TransposeArgs& t, int planes, hipStream_t s) {
 
 namespace fjx {
 
-hipError_t fft_kernels_init() {
-    // Line transforms up to kMaxLdsFft points need more than the default 64 KiB.
+hipError_t fft_kernels_init(int* max_lds_n) {
+    // Line transforms beyond 4096 points need more than the default 64 KiB of
+    // LDS; fall back to 4096-point lines if the runtime refuses the request.
     const int bytes = static_cast<int>(dct_lds_bytes(kMaxLdsFft, true));
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(dct2_rows_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(dct3_rows_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(dct_filter_rows_kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    const void* kernels[3] = {reinterpret_cast<const void*>(dct2_rows_kernel),
+                              reinterpret_cast<const void*>(dct3_rows_kernel),
+                              reinterpret_cast<const void*>(dct_filter_rows_kernel)};
+    *max_lds_n = kMaxLdsFft;
+    for (const void* k : kernels) {
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            *max_lds_n = 4096;
+        }
+    }
+    return hipSuccess;
 }
 
 }  // namespace fjx

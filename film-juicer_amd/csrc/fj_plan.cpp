@@ -52,7 +52,7 @@ hipError_t DeviceState::init(int dev) {
         e = hipEventCreateWithFlags(&error_events[i], hipEventDisableTiming);
         if (e != hipSuccess) return e;
     }
-    return fft_kernels_init();
+    return fft_kernels_init(&max_line);
 }
 
 hipError_t DeviceState::release() {
@@ -314,7 +314,7 @@ bool same_extent(const FjExtent& e, const FjRect& r) {
 }
 
 // Spatial DIR binding (native/juicer_cuda_prepared.cpp:192-199).
-bool check_spatial_dir(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
+bool check_spatial_dir(const FjPreparedHostData& p, const FjFrame& f, int max_line, Diag& d) {
     const FjSpatialDir& sd = p.spatial_dir;
     if (sd.hash == 0) return sd.component_count == 0 || d.malformed("spatial_dir.hash");
     if (p.dir_couplers.mode == FJ_DIR_INACTIVE) return d.malformed("spatial_dir.inactive_dir");
@@ -323,7 +323,7 @@ bool check_spatial_dir(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
         !same_extent(sd.full_frame_extent, f.full_frame_extent))
         return d.malformed("spatial_dir.frame");
     const int w = f.render_window.x2 - f.render_window.x1, h = f.render_window.y2 - f.render_window.y1;
-    if (w > kMaxLdsFft || h > kMaxLdsFft) return d.unsupported("spatial DIR frame side above 8192 pixels");
+    if (w > max_line || h > max_line) return d.unsupported("spatial DIR frame side above the LDS line limit");
     return true;
 }
 
@@ -672,6 +672,7 @@ void put_error(FjErrorBuffer* error, const std::string& text) {
 }
 
 FjRenderOutcome outcome_hip(hipError_t e, const char* where, FjErrorBuffer* error) {
+    (void)hipGetLastError();  // do not leave a sticky error for the caller's next HIP call
     FjRenderOutcome o{};
     o.status = status_of(e == hipErrorOutOfMemory ? FJ_STATUS_ALLOCATION_FAILURE : FJ_STATUS_CUDA_FAILURE,
                          FJ_API_CUDA_RUNTIME, static_cast<int32_t>(e));
@@ -702,7 +703,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     Diag d;
     if (!validate_prepared(prepared, d) || !validate_binding(prepared, frame, submission, d))
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
-    if (!validate_frame(frame, d) || !check_supported(prepared, frame, d) || !check_spatial_dir(prepared, frame, d))
+    if (!validate_frame(frame, d) || !check_supported(prepared, frame, d) || !check_spatial_dir(prepared, frame, dev.max_line, d))
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
     bool invalid = false;
     if (abort_requested(abort_callback, invalid))

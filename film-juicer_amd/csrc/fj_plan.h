@@ -23,6 +23,7 @@ struct DeviceTable {
 struct DeviceState {
     int device = -1;
     int compute_units = 0;
+    int max_line = 4096;  // longest DCT line the LDS transform accepts on this device
     std::unordered_map<uint64_t, DeviceTable> tables;
     std::list<uint64_t> lru;  // most recent at front
     size_t table_bytes = 0;
