@@ -23,10 +23,13 @@
 #include <unordered_map>
 #include <vector>
 
+#include "../../include/fj/fj_mi355x_ext.h"
 #include "fj_host.h"
 #include "fj_plan.h"
 
 using namespace fjx;
+
+#define FJ_EXPORT __attribute__((visibility("default")))
 
 struct FjCuda {
     std::mutex mu;
@@ -112,7 +115,7 @@ FjStatus inspect_frame(const FjFrame* frame, FjCudaContext* out, FjErrorBuffer* 
 
 extern "C" {
 
-FjStatus fj_cuda_create(FjCuda** out_cuda, FjErrorBuffer* error) {
+FJ_EXPORT FjStatus fj_cuda_create(FjCuda** out_cuda, FjErrorBuffer* error) {
     try {
         if (!out_cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "out_cuda is null");
         *out_cuda = nullptr;
@@ -128,7 +131,7 @@ FjStatus fj_cuda_create(FjCuda** out_cuda, FjErrorBuffer* error) {
     }
 }
 
-FjStatus fj_cuda_check_terminal(FjCuda* cuda, FjErrorBuffer* error) {
+FJ_EXPORT FjStatus fj_cuda_check_terminal(FjCuda* cuda, FjErrorBuffer* error) {
     if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
     if (t_in_call) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
     std::lock_guard<std::mutex> lock(cuda->mu);
@@ -137,7 +140,7 @@ FjStatus fj_cuda_check_terminal(FjCuda* cuda, FjErrorBuffer* error) {
     return make_status(FJ_STATUS_SUCCESS);
 }
 
-FjStatus fj_cuda_inspect(FjCuda* cuda, const FjFrame* frame, FjCudaContext* out_context, FjErrorBuffer* error) {
+FJ_EXPORT FjStatus fj_cuda_inspect(FjCuda* cuda, const FjFrame* frame, FjCudaContext* out_context, FjErrorBuffer* error) {
     try {
         if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
         CallGuard guard;
@@ -157,7 +160,7 @@ FjStatus fj_cuda_inspect(FjCuda* cuda, const FjFrame* frame, FjCudaContext* out_
     }
 }
 
-FjRenderOutcome fj_cuda_render(FjCuda* cuda, const FjCudaContext* context, const FjFrame* frame,
+FJ_EXPORT FjRenderOutcome fj_cuda_render(FjCuda* cuda, const FjCudaContext* context, const FjFrame* frame,
                                const FjSubmission* submission, const FjPreparedHostData* prepared,
                                FjAbortCallback abort_callback, FjErrorBuffer* error) {
     FjRenderOutcome outcome{};
@@ -218,7 +221,7 @@ FjRenderOutcome fj_cuda_render(FjCuda* cuda, const FjCudaContext* context, const
     }
 }
 
-FjStatus fj_cuda_retire_instance(FjCuda* cuda, uint64_t instance_token, FjErrorBuffer* error) {
+FJ_EXPORT FjStatus fj_cuda_retire_instance(FjCuda* cuda, uint64_t instance_token, FjErrorBuffer* error) {
     if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
     if (instance_token == 0) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "instance token is zero");
     CallGuard guard;
@@ -230,7 +233,7 @@ FjStatus fj_cuda_retire_instance(FjCuda* cuda, uint64_t instance_token, FjErrorB
     return make_status(FJ_STATUS_SUCCESS);
 }
 
-FjStatus fj_cuda_shutdown(FjCuda* cuda, FjErrorBuffer* error) {
+FJ_EXPORT FjStatus fj_cuda_shutdown(FjCuda* cuda, FjErrorBuffer* error) {
     if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
     CallGuard guard;
     if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
@@ -247,7 +250,7 @@ FjStatus fj_cuda_shutdown(FjCuda* cuda, FjErrorBuffer* error) {
     return result;
 }
 
-FjStatus fj_cuda_destroy(FjCuda* cuda, FjErrorBuffer* error) {
+FJ_EXPORT FjStatus fj_cuda_destroy(FjCuda* cuda, FjErrorBuffer* error) {
     if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
     if (t_in_call) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
     FjStatus s = make_status(FJ_STATUS_SUCCESS);
@@ -267,6 +270,34 @@ FjStatus fj_cuda_destroy(FjCuda* cuda, FjErrorBuffer* error) {
     delete cuda;
     if (s.category == FJ_STATUS_SUCCESS) write_error(error, "");
     return s;
+}
+
+FJ_EXPORT int32_t fjx_stage_timings(FjCuda* cuda, int32_t device_id, FjxStageTiming* out, int32_t capacity) {
+    try {
+        if (!cuda || g_owner.load() != cuda || capacity < 0 || (capacity > 0 && !out)) return -1;
+        std::lock_guard<std::mutex> lock(cuda->mu);
+        auto it = cuda->devices.find(device_id);
+        if (it == cuda->devices.end() || !it->second) return -1;
+        DeviceState& dev = *it->second;
+        const int n = dev.stage_count;
+        if (n <= 0) return 0;
+        if (hipEventSynchronize(dev.stage_events[n]) != hipSuccess) {
+            (void)hipGetLastError();
+            return -1;
+        }
+        for (int i = 0; i < n && i < capacity; ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, dev.stage_events[i], dev.stage_events[i + 1]) != hipSuccess) {
+                (void)hipGetLastError();
+                return -1;
+            }
+            std::snprintf(out[i].name, sizeof(out[i].name), "%s", dev.stage_names[i] ? dev.stage_names[i] : "");
+            out[i].milliseconds = ms;
+        }
+        return n;
+    } catch (...) {
+        return -1;
+    }
 }
 
 }  // extern "C"

@@ -52,6 +52,10 @@ hipError_t DeviceState::init(int dev) {
         e = hipEventCreateWithFlags(&error_events[i], hipEventDisableTiming);
         if (e != hipSuccess) return e;
     }
+    for (int i = 0; i <= kMaxStages; ++i) {
+        e = hipEventCreate(&stage_events[i]);
+        if (e != hipSuccess) return e;
+    }
     return fft_kernels_init(&max_line);
 }
 
@@ -71,6 +75,8 @@ hipError_t DeviceState::release() {
     workspace_bytes = 0;
     for (int i = 0; i < kRing; ++i)
         if (error_events[i]) keep(hipEventDestroy(error_events[i]));
+    for (int i = 0; i <= kMaxStages; ++i)
+        if (stage_events[i]) keep(hipEventDestroy(stage_events[i]));
     if (error_words) keep(hipFree(error_words));
     if (error_host) keep(hipHostFree(error_host));
     error_words = nullptr;
@@ -748,6 +754,15 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     pp.error_word = dev.error_words + slot;
     e = launch_clear_word(pp.error_word, stream);
     if (e != hipSuccess) return outcome_hip(e, "clear error word", error);
+    const bool trace = (frame.flags & FJ_FRAME_TRACE_VERBOSE) != 0;
+    int stage = 0;
+    auto mark = [&](const char* name) {
+        // Closes the previous stage (if any) and names the next one.
+        if (!trace || stage > DeviceState::kMaxStages) return;
+        (void)hipEventRecord(dev.stage_events[stage], stream);
+        if (name && stage < DeviceState::kMaxStages) dev.stage_names[stage] = name;
+        ++stage;
+    };
     const int grid_limit = dev.compute_units * 8;
     const bool spatial_dir = prepared.spatial_dir.hash != 0;
     const bool post = post_active(prepared.scanner_post);
@@ -765,6 +780,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     }
     if (spatial_dir) {
         for (int c = 0; c < 6; ++c) pp.dir_planes[c] = ws + c * plane;
+        mark("dir_source");
         e = launch_pixel_stage(pp, kDirSource, grid_limit, stream);
         if (e != hipSuccess) return outcome_hip(e, "DIR source stage", error);
         HostTransfer ht;
@@ -785,11 +801,14 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
             job.tmp_a[c] = ws + (6 + c) * plane;
             job.tmp_b[c] = ws + (9 + c) * plane;
         }
+        mark("dir_dct_filter");
         e = run_dct_filter(dev, job, stream);
         if (e != hipSuccess) return outcome_hip(e, "DIR spatial filter", error);
+        mark("pixel_chain");
         e = launch_pixel_stage(pp, kDirFinal, grid_limit, stream);
         if (e != hipSuccess) return outcome_hip(e, "DIR final stage", error);
     } else {
+        mark("pixel_chain");
         e = launch_pixel_stage(pp, kFused, grid_limit, stream);
         if (e != hipSuccess) return outcome_hip(e, "fused pixel kernel", error);
     }
@@ -808,9 +827,12 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         po.weights = static_cast<const float*>(wptr);
         po.radius = sp.unsharp_radius;
         po.amount = sp.unsharp_amount;
+        mark("post_unsharp_encode");
         e = launch_post(po, stream);
         if (e != hipSuccess) return outcome_hip(e, "post stage", error);
     }
+    mark(nullptr);
+    dev.stage_count = trace ? std::min(stage - 1, static_cast<int>(DeviceState::kMaxStages)) : dev.stage_count;
     e = hipMemcpyAsync(dev.error_host + slot, pp.error_word, sizeof(unsigned int), hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) return outcome_hip(e, "error readback", error);
     e = hipEventRecord(dev.error_events[slot], stream);

@@ -41,6 +41,12 @@ struct DeviceState {
     void* workspace = nullptr;
     size_t workspace_bytes = 0;
 
+    // Stage timing of the last traced frame (FJ_FRAME_TRACE_VERBOSE).
+    static constexpr int kMaxStages = 16;
+    hipEvent_t stage_events[kMaxStages + 1] = {};
+    const char* stage_names[kMaxStages] = {};
+    int stage_count = 0;
+
     hipError_t init(int dev);
     hipError_t release();
     // Returns the device copy of `bytes` host bytes under `key`, uploading once.

@@ -29,6 +29,8 @@ FJ_COMPONENTS_RGB = 3
 FJ_COMPONENTS_RGBA = 4
 FJ_DEPTH_FLOAT32 = 1
 FJ_FRAME_STREAM_PRESENT = 1
+FJ_FRAME_TRACE_INFO = 2
+FJ_FRAME_TRACE_VERBOSE = 4
 FJ_ABORT_CONTINUE = 0
 FJ_ABORT_REQUESTED = 1
 

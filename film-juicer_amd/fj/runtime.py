@@ -46,10 +46,16 @@ def lib():
         L.fj_cuda_retire_instance.argtypes = [C.c_void_p, C.c_uint64, eb]
         L.fj_cuda_shutdown.restype = abi.FjStatus
         L.fj_cuda_shutdown.argtypes = [C.c_void_p, eb]
+        L.fjx_stage_timings.restype = C.c_int32
+        L.fjx_stage_timings.argtypes = [C.c_void_p, C.c_int32, C.POINTER(FjxStageTiming), C.c_int32]
         L.fj_cuda_destroy.restype = abi.FjStatus
         L.fj_cuda_destroy.argtypes = [C.c_void_p, eb]
         _lib = L
     return _lib
+
+
+class FjxStageTiming(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("milliseconds", C.c_float)]
 
 
 class FjError(RuntimeError):
@@ -160,6 +166,14 @@ class Renderer:
         if out.status.category != abi.FJ_STATUS_SUCCESS:
             raise FjError(out.status, text)
         return out
+
+    def stage_timings(self, device_id: int = 0) -> list:
+        """(name, ms) of each stage of the last frame rendered with FJ_FRAME_TRACE_VERBOSE."""
+        buf = (FjxStageTiming * 16)()
+        n = lib().fjx_stage_timings(self.handle, device_id, buf, 16)
+        if n < 0:
+            raise RuntimeError("stage timings unavailable")
+        return [(buf[i].name.decode(), float(buf[i].milliseconds)) for i in range(n)]
 
     def shutdown(self):
         err = ErrorBuffer()

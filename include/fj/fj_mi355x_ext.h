@@ -1,0 +1,31 @@
+/*
+ * fj_mi355x_ext.h — extensions of the MI355X render library beyond the
+ * drop-in boundary of juicer_cuda_api.h. These entry points have no
+ * counterpart in the reference; they serve benchmarks and diagnostics.
+ */
+#ifndef FJ_MI355X_EXT_H
+#define FJ_MI355X_EXT_H
+
+#include "juicer_cuda_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One timed stage of the last traced frame. A frame is traced when its
+ * FjFrame.flags carry FJ_FRAME_TRACE_VERBOSE: every stage is bracketed by HIP
+ * events on the frame's stream. */
+typedef struct FjxStageTiming {
+    char name[32];
+    float milliseconds;
+} FjxStageTiming;
+
+/* Waits for the last traced frame on `device_id` and copies up to `capacity`
+ * stage timings into `out`. Returns the number of stages, or -1 on error. */
+int32_t fjx_stage_timings(FjCuda* cuda, int32_t device_id, FjxStageTiming* out, int32_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
