@@ -68,6 +68,7 @@ hipError_t DeviceState::release() {
     keep(hipDeviceSynchronize());
     for (auto& kv : tables) keep(hipFree(kv.second.ptr));
     tables.clear();
+    transfers.clear();
     lru.clear();
     table_bytes = 0;
     if (workspace) keep(hipFree(workspace));
@@ -103,6 +104,7 @@ hipError_t DeviceState::table(uint64_t key, const void* host, size_t bytes, cons
             const hipError_t e = hipFree(v->second.ptr);
             table_bytes -= v->second.bytes;
             tables.erase(v);
+            transfers.clear();  // they may point into the freed table
             if (e != hipSuccess) return e;
         }
     }
@@ -783,18 +785,26 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         mark("dir_source");
         e = launch_pixel_stage(pp, kDirSource, grid_limit, stream);
         if (e != hipSuccess) return outcome_hip(e, "DIR source stage", error);
-        HostTransfer ht;
-        std::string diag;
-        if (!dir_transfer(prepared.spatial_dir, pp.io.width, pp.io.height, ht, diag))
-            return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "MalformedPreparedHostData field=" + diag, error);
         DctFilterJob job;
         job.width = pp.io.width;
         job.height = pp.io.height;
         job.planes = 3;
         const uint64_t tkey = hash_bytes(&prepared.spatial_dir.components, sizeof(prepared.spatial_dir.components),
-                                         static_cast<uint64_t>(job.width) << 32 | static_cast<uint32_t>(job.height));
-        e = upload_transfer(dev, ht, tkey, job.transfer);
-        if (e != hipSuccess) return outcome_hip(e, "DIR transfer upload", error);
+                                         static_cast<uint64_t>(job.width) << 32 | static_cast<uint32_t>(job.height)) ^
+                              (static_cast<uint64_t>(prepared.spatial_dir.component_count) << 56);
+        auto cached = dev.transfers.find(tkey);
+        if (cached == dev.transfers.end()) {
+            HostTransfer ht;
+            std::string diag;
+            if (!dir_transfer(prepared.spatial_dir, pp.io.width, pp.io.height, ht, diag))
+                return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "MalformedPreparedHostData field=" + diag, error);
+            DctTransfer dt;
+            e = upload_transfer(dev, ht, tkey, dt);
+            if (e != hipSuccess) return outcome_hip(e, "DIR transfer upload", error);
+            if (dev.transfers.size() > 64) dev.transfers.clear();
+            cached = dev.transfers.emplace(tkey, dt).first;
+        }
+        job.transfer = cached->second;
         for (int c = 0; c < 3; ++c) {
             job.in[c] = pp.dir_planes[c];
             job.out[c] = pp.dir_planes[c];

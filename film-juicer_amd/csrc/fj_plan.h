@@ -11,6 +11,7 @@
 
 #include "../../include/fj/juicer_cuda_api.h"
 #include "fj_device.h"
+#include "fj_fft.h"
 
 namespace fjx {
 
@@ -26,6 +27,8 @@ struct DeviceState {
     int max_line = 4096;  // longest DCT line the LDS transform accepts on this device
     std::unordered_map<uint64_t, DeviceTable> tables;
     std::list<uint64_t> lru;  // most recent at front
+    // Device transfer functions of spatial filters, by descriptor and frame size.
+    std::unordered_map<uint64_t, DctTransfer> transfers;
     size_t table_bytes = 0;
 
     // Error word ring: each frame clears one device word, its kernels OR scan
