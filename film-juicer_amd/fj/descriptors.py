@@ -84,3 +84,238 @@ def spatial_dir(record: abi.FjPreparedHostData, size_um: float, tail_um: float, 
 def pixel_size_for(width: int, height: int, film_long_edge_mm: float = 35.0) -> float:
     """Pixel pitch on the film frame (rust film-juicer-core frame.rs:45-58)."""
     return film_long_edge_mm * 1000.0 / max(width, height)
+
+
+# ---------------------------------------------------------------------------
+# Scatter / halation (rust film-juicer-core optics.rs:50-56, :588-700 and
+# optics/halation_frame.rs:85-447).
+# ---------------------------------------------------------------------------
+SCATTER_CORE_UM = (2.2, 2.0, 1.6)
+SCATTER_TAIL_UM = (9.3, 9.7, 9.1)
+SCATTER_TAIL_WEIGHT = (0.78, 0.65, 0.67)
+HALATION_SIGMA_RATIOS = (0.5360, 1.5236, 2.7684)
+HALATION_BOUNCE_MULTIPLIERS = (1.0, 1.4142135623730950488, 1.7320508075688772935, 2.0, 2.23606797749979)
+FNV_OFFSET = 0xCBF29CE484222325
+FNV_PRIME = 0x100000001B3
+
+
+def _fnv(h: int, data: bytes) -> int:
+    for b in data:
+        h ^= b
+        h = (h * FNV_PRIME) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _f32_bytes(v: float) -> bytes:
+    import struct
+    return struct.pack("<f", v)
+
+
+def _canon(v: float) -> float:
+    return 0.0 if v == 0.0 else v
+
+
+class HalationError(ValueError):
+    pass
+
+
+def legacy_bounce_weights():
+    return tuple(_f32(w) for w in (4.0 / 7.0, 2.0 / 7.0, 1.0 / 7.0, 0.0, 0.0))
+
+
+def bounce_weights(count: int = 3, decay: float = 0.5):
+    """Normalized geometric bounce weights (optics.rs:128-138)."""
+    powers = [0.0] * 5
+    power, total = 1.0, 0.0
+    for i in range(count):
+        powers[i] = power
+        total += power
+        power *= decay
+    return tuple(_f32(w / total) for w in powers)
+
+
+def halation_recipe(*, active=True, scatter_amount=1.0, scatter_spatial_scale=1.0, halation_amount=1.0,
+                    halation_spatial_scale=1.0, first_sigma_um=(65.0, 65.0, 65.0),
+                    primary_amount=(0.015, 0.005, 0.0), legacy_reflection=True, bounce_count=3,
+                    bounce_decay=0.5, renormalize=True) -> dict:
+    """Resolved halation inputs and recipe hash for the default shape
+    (Halation::new, optics.rs:588-700; HalationFrameInput::from,
+    halation_frame.rs:50-70). Controls are f32 values."""
+    f = _f32
+    scatter_amount, scatter_spatial_scale = f(scatter_amount), f(scatter_spatial_scale)
+    halation_amount, halation_spatial_scale = f(halation_amount), f(halation_spatial_scale)
+    first_sigma_um = tuple(f(v) for v in first_sigma_um)
+    primary_amount = tuple(f(v) for v in primary_amount)
+    r = dict(scatter_active=False, back_reflection_active=False, scatter_amount=0.0, scatter_spatial_scale=0.0,
+             halation_spatial_scale=0.0, first_sigma_um=(0.0, 0.0, 0.0), total_strength=(0.0, 0.0, 0.0),
+             scatter_core_um=SCATTER_CORE_UM, scatter_tail_um=SCATTER_TAIL_UM,
+             scatter_tail_weight=tuple(f(v) for v in SCATTER_TAIL_WEIGHT), bounce_weights=(0.0,) * 5,
+             bounce_count=0, renormalize=False, legacy_scatter=True, legacy_reflection=False, recipe_hash=0)
+    if not active:
+        return r
+    r["scatter_active"] = scatter_amount > 0.0 and scatter_spatial_scale > 0.0
+    if r["scatter_active"]:
+        r["scatter_amount"] = _canon(scatter_amount)
+        r["scatter_spatial_scale"] = _canon(scatter_spatial_scale)
+    back = (halation_amount > 0.0 and halation_spatial_scale > 0.0 and any(v > 0.0 for v in primary_amount)
+            and any(v > 0.0 for v in first_sigma_um))
+    if back:
+        strength = tuple(_canon(f(pa * halation_amount)) for pa in primary_amount)
+        if any(v > 0.0 for v in strength):
+            r.update(back_reflection_active=True, halation_spatial_scale=_canon(halation_spatial_scale),
+                     first_sigma_um=first_sigma_um, total_strength=strength)
+            w = bounce_weights(bounce_count, bounce_decay)
+            r["bounce_weights"] = w
+            r["bounce_count"] = max(i + 1 for i in range(5) if w[i] > 0.0)
+            r["renormalize"] = renormalize
+            r["legacy_reflection"] = legacy_reflection and renormalize and w == legacy_bounce_weights()
+    if not r["scatter_active"] and not r["back_reflection_active"]:
+        return r
+    h = _fnv(FNV_OFFSET, bytes([1 if r["scatter_active"] else 0]))
+    if r["scatter_active"]:
+        h = _fnv(h, _f32_bytes(r["scatter_amount"]) + _f32_bytes(r["scatter_spatial_scale"]))
+    h = _fnv(h, bytes([1 if r["back_reflection_active"] else 0]))
+    if r["back_reflection_active"]:
+        h = _fnv(h, _f32_bytes(r["halation_spatial_scale"]))
+        for v in r["first_sigma_um"] + r["total_strength"]:
+            h = _fnv(h, _f32_bytes(v))
+    if r["back_reflection_active"] and not r["legacy_reflection"]:
+        h = _fnv(h, b"halation-reflection\0")
+        h = _fnv(h, bytes([r["bounce_count"], 1 if r["renormalize"] else 0]))
+        for v in r["bounce_weights"]:
+            h = _fnv(h, _f32_bytes(v))
+    r["recipe_hash"] = h if h else 1
+    return r
+
+
+def _factored(b: float, a1: float, a2: float, a3: float):
+    """Real pole + second-order section of the YVV recurrence (halation_frame.rs:89-145)."""
+    vals = (b, a1, a2, a3)
+    if not all(math.isfinite(v) for v in vals) or b <= 0.0 or a3 <= 0.0:
+        return None
+    c1 = 3.0 - 2.0 * a1 - a2
+    c2 = 3.0 - a1
+
+    def poly(hh):
+        return ((-hh + c2) * hh - c1) * hh + b
+
+    if not (poly(0.0) > 0.0 and poly(1.0) < 0.0):
+        return None
+    lo, hi = 0.0, 1.0
+    for _ in range(64):
+        mid = (lo + hi) * 0.5
+        if poly(mid) > 0.0:
+            lo = mid
+        else:
+            hi = mid
+    h = (lo + hi) * 0.5
+    pole = 1.0 - h
+    t = a3 / pole
+    g = b / h
+    pair = 1.0 + t - g
+    tol = 2.0e-10
+    if (not (0.0 < h < 1.0 and 0.0 < t < 1.0 and 0.0 < g < 2.0 * (1.0 + t) and math.isfinite(g))
+            or abs(pole + pair - a1) > tol or abs(-t - pole * pair - a2) > tol or abs(pole * t - a3) > tol):
+        return None
+    fh, fg, ft = _f32(h), _f32(g), _f32(t)
+    if not (0.0 < fh < 1.0 and fg > 0.0 and math.isfinite(fg) and 0.0 < ft < 1.0 and fg < 2.0 * (1.0 + ft)):
+        return None
+    return (fh, fg, ft, 0.0)
+
+
+def halation_gaussian(sigma_px: float, legacy: bool):
+    """(kind, radius, fir_weights[19], yvv[4]) of one Gaussian (halation_frame.rs:155-240)."""
+    if not math.isfinite(sigma_px):
+        raise HalationError(f"gaussian sigma {sigma_px}")
+    ident = (abi.FJ_HALATION_IDENTITY, 0, (0.0,) * 19, (0.0,) * 4)
+    if sigma_px <= 0.0:
+        return ident
+    if sigma_px < 3.0:
+        radius = int(3.0 * sigma_px + 0.5)
+        if radius == 0:
+            return ident
+        vals = [math.exp(-(float(i - radius) ** 2) / (2.0 * sigma_px * sigma_px)) for i in range(2 * radius + 1)]
+        total = 0.0
+        for v in vals:
+            total += v
+        w = [_canon(_f32(v / total)) for v in vals] + [0.0] * (19 - len(vals))
+        return (abi.FJ_HALATION_FIR_REFLECT, radius, tuple(w), (0.0,) * 4)
+    if not legacy and sigma_px > 1024.0:
+        raise HalationError(f"advanced radius {sigma_px}")
+    q = 0.98711 * sigma_px - 0.96330
+    q2 = q * q
+    q3 = q2 * q
+    b0 = 1.57825 + 2.44413 * q + 1.4281 * q2 + 0.422205 * q3
+    b1 = 2.44413 * q + 2.85619 * q2 + 1.26661 * q3
+    b2 = -(1.4281 * q2 + 1.26661 * q3)
+    b3 = 0.422205 * q3
+    coeff = (1.0 - (b1 + b2 + b3) / b0, b1 / b0, b2 / b0, b3 / b0)
+    if not all(math.isfinite(v) for v in coeff):
+        raise HalationError(f"gaussian sigma {sigma_px}")
+    if legacy:
+        yvv = tuple(_f32(v) for v in coeff)
+        kind = abi.FJ_HALATION_YVV_REPLICATE
+    else:
+        yvv = _factored(*coeff)
+        if yvv is None:
+            raise HalationError(f"gaussian sigma {sigma_px}")
+        kind = abi.FJ_HALATION_YVV_REPLICATE_FACTORED
+    if not all(math.isfinite(v) for v in yvv):
+        raise HalationError(f"gaussian sigma {sigma_px}")
+    return (kind, 0, (0.0,) * 19, yvv)
+
+
+def _ordered_max(v: float, floor: float) -> float:
+    return floor if v < floor else v
+
+
+def halation_frame(inp: dict, pixel_size_um: float):
+    """FjScatterHalation for one frame (HalationFrame::new, halation_frame.rs:326-447),
+    or None when the recipe is inactive."""
+    if inp["recipe_hash"] == 0:
+        return None
+    px32 = _f32(pixel_size_um)
+    if not math.isfinite(px32) or px32 <= 0.0:
+        raise HalationError("pixel size")
+    back = inp["back_reflection_active"]
+    if back and not (1 <= inp["bounce_count"] <= 5):
+        raise HalationError("bounce count")
+    out = abi.FjScatterHalation()
+    out.recipe_hash = inp["recipe_hash"]
+    out.scatter_amount = _canon(inp["scatter_amount"]) if inp["scatter_active"] else 0.0
+    count = inp["bounce_count"] if back else 0
+    out.bounce_count = count
+    for i in range(5):
+        out.bounce_weights[i] = inp["bounce_weights"][i] if back else 0.0
+    out.renormalize = 1 if (back and inp["renormalize"]) else 0
+    pixel = float(px32)
+
+    def put(dst, g):
+        kind, radius, w, yvv = g
+        dst.kind = kind
+        dst.radius = radius
+        for i in range(19):
+            dst.fir_weights[i] = w[i]
+        for i in range(4):
+            dst.yvv[i] = yvv[i]
+
+    for ch in range(3):
+        c = out.channels[ch]
+        if inp["scatter_active"]:
+            scale = float(_f32(inp["scatter_spatial_scale"]))
+            c.tail_weight = inp["scatter_tail_weight"][ch]
+            if c.tail_weight != 1.0:
+                sigma = _ordered_max(inp["scatter_core_um"][ch] * scale / pixel, 1.0e-6)
+                put(c.core, halation_gaussian(sigma, inp["legacy_scatter"]))
+            if c.tail_weight != 0.0:
+                for i in range(3):
+                    sigma = _ordered_max(inp["scatter_tail_um"][ch] * HALATION_SIGMA_RATIOS[i] * scale / pixel, 1.0e-6)
+                    put(c.tail[i], halation_gaussian(sigma, inp["legacy_scatter"]))
+        c.total_strength = _canon(inp["total_strength"][ch]) if back else 0.0
+        if back and (inp["legacy_reflection"] or c.total_strength > 0.0):
+            first = float(_f32(inp["first_sigma_um"][ch]))
+            scale = float(_f32(inp["halation_spatial_scale"]))
+            for i in range(count):
+                sigma = _ordered_max(first * HALATION_BOUNCE_MULTIPLIERS[i] * scale / pixel, 1.0e-6)
+                put(c.bounce[i], halation_gaussian(sigma, inp["legacy_reflection"]))
+    return out

@@ -381,6 +381,7 @@ typedef struct Frame {
     float* dst;
     ptrdiff_t dst_stride;
     double* plane[9]; /* scratch planes */
+    double* carrier[3]; /* camera film linear exposure B/G/R, or NULL */
     double* linear;   /* [h][w][3] */
     uint32_t errors;
     pthread_mutex_t lock;
@@ -388,9 +389,9 @@ typedef struct Frame {
 
 static void tc_exposure(const FjFilmExposure* fe, const double w[3], double E[3]);
 
-/* Camera film linear exposure, B/G/R order
- * (JuicerCudaDeviceHelpers.cuh:509-617 and :709-763). */
-static void px_exposure(const FjoPlan* pl, const float* px, double E[3]) {
+/* Reconstructed film raw exposure before scaling, B/G/R order
+ * (JuicerCudaDeviceHelpers.cuh:509-617, :619-703). */
+static void px_raw(const FjoPlan* pl, const float* px, double E[3]) {
     const FjFilmExposure* fe = &pl->p.film_exposure;
     double rgb[3];
     const int decode = (fe->flags & FJ_FILM_DECODE_CCTF) != 0;
@@ -427,9 +428,32 @@ static void px_exposure(const FjoPlan* pl, const float* px, double E[3]) {
         for (int c = 0; c < 3; ++c) w[c] = fin0(w[c]);
         tc_exposure(fe, w, E);
     }
-    double scale = pl->manual_scale * (double)fe->route_correction_scale;
+}
+
+/* Film raw exposure with the manual and route scales
+ * (compute_film_raw_device, JuicerCudaDeviceHelpers.cuh:709-763). */
+static void px_exposure(const FjoPlan* pl, const float* px, double E[3]) {
+    px_raw(pl, px, E);
+    double scale = pl->manual_scale * (double)pl->p.film_exposure.route_correction_scale;
     if (!(isfinite(scale) && scale > 0.0)) scale = 1.0;
     for (int c = 0; c < 3; ++c) E[c] = fmax(0.0, E[c] * scale);
+}
+
+/* Camera carrier exposure: manual scale only
+ * (compute_camera_film_linear_exposure_device, JuicerCudaDeviceHelpers.cuh:
+ * 709-744). */
+static void px_camera(const FjoPlan* pl, const float* px, double E[3]) {
+    px_raw(pl, px, E);
+    double scale = pl->manual_scale;
+    if (!(isfinite(scale) && scale > 0.0)) scale = 1.0;
+    for (int c = 0; c < 3; ++c) E[c] = fmax(0.0, E[c] * scale);
+}
+
+/* Route correction applied when a carrier is developed
+ * (apply_film_route_correction_device, JuicerCudaDeviceHelpers.cuh:746-755). */
+static void carrier_exposure(const FjoPlan* pl, double* const carrier[3], size_t idx, double E[3]) {
+    const double route = pl->p.film_exposure.route_correction_scale;
+    for (int c = 0; c < 3; ++c) E[c] = fmax(0.0, carrier[c][idx] * route);
 }
 
 /* TC-LUT exposure (Hanatos / Arctic) from working XYZ. */
@@ -938,6 +962,125 @@ int fjo_dir_filter(const FjSpatialDir* sd, double* const* planes, int nplanes, i
 /* Unsharp mask on one W x H double plane (JuicerCudaScanPipeline.cu:316-452,
  * :586-660): half-sample reflected horizontal and vertical blur, non-finite
  * blurs replaced by zero, v + a (v - blur). */
+/* ---- scatter / halation (JuicerCudaScatterHalation.cu) ------------------ */
+
+/* One step of the replicate-initialized third-order recursion
+ * (JuicerCudaScatterHalation.cu:183-198): legacy coefficients {B, B1, B2, B3}
+ * run the direct form, factored {h, g, t, 0} the real pole + second-order
+ * section. */
+static double yvv_step(int factored, const double c[4], double x, double s[3]) {
+    if (factored) {
+        s[0] += c[0] * (x - s[0]);
+        s[2] = c[2] * s[2] + c[1] * (s[0] - s[1]);
+        s[1] += s[2];
+        return s[1];
+    }
+    const double v = c[0] * x + c[1] * s[0] + c[2] * s[1] + c[3] * s[2];
+    s[2] = s[1];
+    s[1] = s[0];
+    s[0] = v;
+    return v;
+}
+
+static void yvv_init(int factored, double first, double s[3]) {
+    s[0] = first;
+    s[1] = first;
+    s[2] = factored ? 0.0 : first;
+}
+
+/* One halation Gaussian on a W x H plane (JuicerCudaScatterHalation.cu:
+ * 104-180 FIR vertical-then-horizontal with half-sample reflection,
+ * :200-420 YVV horizontal forward/backward then vertical bottom-up/top-down,
+ * each pass seeded with its first sample). */
+static void hal_gauss(const FjHalationGaussian* g, const double* src, double* dst, double* tmp, int W, int H) {
+    const size_t N = (size_t)W * H;
+    if (g->kind == FJ_HALATION_IDENTITY) {
+        memcpy(dst, src, sizeof(double) * N);
+        return;
+    }
+    if (g->kind == FJ_HALATION_FIR_REFLECT) {
+        const int r = g->radius;
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                double sum = 0.0;
+                for (int o = -r; o <= r; ++o) sum += src[(size_t)reflect_periodic(y - o, H) * W + x] * g->fir_weights[o + r];
+                tmp[(size_t)y * W + x] = sum;
+            }
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                double sum = 0.0;
+                for (int o = -r; o <= r; ++o) sum += tmp[(size_t)y * W + reflect_periodic(x + o, W)] * g->fir_weights[o + r];
+                dst[(size_t)y * W + x] = sum;
+            }
+        return;
+    }
+    const int fac = g->kind == FJ_HALATION_YVV_REPLICATE_FACTORED;
+    const double c[4] = {g->yvv[0], g->yvv[1], g->yvv[2], g->yvv[3]};
+    double s[3];
+    for (int y = 0; y < H; ++y) {
+        const double* in = src + (size_t)y * W;
+        double* t = tmp + (size_t)y * W;
+        yvv_init(fac, in[0], s);
+        for (int x = 0; x < W; ++x) t[x] = yvv_step(fac, c, in[x], s);
+        yvv_init(fac, t[W - 1], s);
+        for (int x = W - 1; x >= 0; --x) t[x] = yvv_step(fac, c, t[x], s);
+    }
+    for (int x = 0; x < W; ++x) {
+        yvv_init(fac, tmp[(size_t)(H - 1) * W + x], s);
+        for (int y = H - 1; y >= 0; --y) tmp[(size_t)y * W + x] = yvv_step(fac, c, tmp[(size_t)y * W + x], s);
+        yvv_init(fac, tmp[x], s);
+        for (int y = 0; y < H; ++y) dst[(size_t)y * W + x] = yvv_step(fac, c, tmp[(size_t)y * W + x], s);
+    }
+}
+
+int fjo_halation(const FjScatterHalation* d, double* const planes[3], int W, int H) {
+    static const double kTailAmplitude[3] = {0.1633f, 0.6496f, 0.1870f};
+    if (!d || W <= 0 || H <= 0) return 2;
+    const size_t N = (size_t)W * H;
+    double* acc = (double*)malloc(sizeof(double) * N);
+    double* g = (double*)malloc(sizeof(double) * N);
+    double* tmp = (double*)malloc(sizeof(double) * N);
+    if (!acc || !g || !tmp) {
+        free(acc);
+        free(g);
+        free(tmp);
+        return 2;
+    }
+    /* launch_scatter_halation (JuicerCudaScatterHalation.cu:690-852). */
+    for (int ch = 0; ch < 3; ++ch) {
+        double* x = planes[ch];
+        const FjHalationChannel* sc = &d->channels[ch];
+        if (d->scatter_amount != 0.0f) {
+            const double a = d->scatter_amount, tw = sc->tail_weight;
+            const int tail = sc->tail_weight > 0.0f;
+            if (tail)
+                for (int i = 0; i < 3; ++i) {
+                    hal_gauss(&sc->tail[i], x, g, tmp, W, H);
+                    for (size_t k = 0; k < N; ++k) acc[k] = (i ? acc[k] : 0.0) + kTailAmplitude[i] * g[k];
+                }
+            hal_gauss(&sc->core, x, g, tmp, W, H);
+            for (size_t k = 0; k < N; ++k) {
+                const double scattered = (1.0 - tw) * g[k] + tw * (tail ? acc[k] : 0.0);
+                x[k] = (1.0 - a) * x[k] + a * scattered;
+            }
+        }
+        if (sc->total_strength == 0.0f) continue;
+        for (uint32_t j = 0; j < d->bounce_count; ++j) {
+            hal_gauss(&sc->bounce[j], x, g, tmp, W, H);
+            for (size_t k = 0; k < N; ++k) acc[k] = (j ? acc[k] : 0.0) + (double)d->bounce_weights[j] * g[k];
+        }
+        const double s = sc->total_strength;
+        for (size_t k = 0; k < N; ++k) {
+            const double added = x[k] + s * (d->bounce_count ? acc[k] : 0.0);
+            x[k] = d->renormalize ? added / (1.0 + s) : added;
+        }
+    }
+    free(acc);
+    free(g);
+    free(tmp);
+    return 0;
+}
+
 static int reflect_half(long i, int n) {
     if (n <= 1) return 0;
     while (i < 0 || i >= n) i = i < 0 ? -i - 1 : 2L * n - i - 1;
@@ -1024,13 +1167,27 @@ static void rows_chain(Frame* f, int y0, int y1) {
     uint32_t errs = 0;
     for (int y = y0; y < y1; ++y)
         for (int x = 0; x < f->w; ++x) {
-            const float* px = f->src + y * f->src_stride + (ptrdiff_t)x * f->comps;
             double E[3], D[3];
-            px_exposure(pl, px, E);
+            if (f->carrier[0])
+                carrier_exposure(pl, f->carrier, (size_t)y * f->w + x, E);
+            else
+                px_exposure(pl, f->src + y * f->src_stride + (ptrdiff_t)x * f->comps, E);
             errs |= px_develop(pl, E, D);
             errs |= px_back(pl, f, (size_t)y * f->w + x, D);
         }
     add_errors(f, errs);
+}
+
+/* Camera carrier planes (camera_film_linear_exposure_kernel,
+ * JuicerCudaFilmPipeline.cu:3103-3146). */
+static void rows_carrier(Frame* f, int y0, int y1) {
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < f->w; ++x) {
+            double E[3];
+            px_camera(f->plan, f->src + y * f->src_stride + (ptrdiff_t)x * f->comps, E);
+            const size_t idx = (size_t)y * f->w + x;
+            for (int c = 0; c < 3; ++c) f->carrier[c][idx] = E[c];
+        }
 }
 
 /* Spatial-DIR source (JuicerCudaFilmPipeline.cu:124-220): raw donor
@@ -1053,10 +1210,12 @@ static void rows_dir_source(Frame* f, int y0, int y1) {
     const size_t N = (size_t)f->w * f->h;
     for (int y = y0; y < y1; ++y)
         for (int x = 0; x < f->w; ++x) {
-            const float* px = f->src + y * f->src_stride + (ptrdiff_t)x * f->comps;
             const size_t idx = (size_t)y * f->w + x;
             double E[3], layer[3], donors[3];
-            px_exposure(pl, px, E);
+            if (f->carrier[0])
+                carrier_exposure(pl, f->carrier, idx, E);
+            else
+                px_exposure(pl, f->src + y * f->src_stride + (ptrdiff_t)x * f->comps, E);
             uint32_t fail = 0;
             for (int b = 0; b < 3; ++b) {
                 const double raw = log10(fmax(E[b], 0.0) + 1e-10);
@@ -1138,9 +1297,10 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
                uint32_t* error_flags) {
     if (!plan || !src || width <= 0 || height <= 0 || (comps != 3 && comps != 4)) return 2;
     const FjPreparedHostData* p = &plan->p;
-    if ((p->optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_SCATTER_HALATION)) ||
+    if ((p->optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION)) ||
         (p->grain.flags & FJ_GRAIN_ACTIVE))
         return 3;
+    const int halation = (p->optics.flags & FJ_OPTICS_SCATTER_HALATION) != 0;
     const size_t N = (size_t)width * height;
     Frame f;
     memset(&f, 0, sizeof(f));
@@ -1155,7 +1315,16 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     f.linear = (double*)malloc(sizeof(double) * N * 3);
     pthread_mutex_init(&f.lock, NULL);
     int rc = 0;
-    if (p->spatial_dir.hash != 0) {
+    if (halation) {
+        /* expose_print_camera (JuicerCudaExecutor.cpp:1874-1989): carrier,
+         * then scatter/halation in place on it. */
+        for (int c = 0; c < 3; ++c) f.carrier[c] = (double*)malloc(sizeof(double) * N);
+        run_rows(&f, rows_carrier, threads);
+        double* const rgb_planes[3] = {f.carrier[2], f.carrier[1], f.carrier[0]};
+        rc = fjo_halation(&p->optics.scatter_halation, rgb_planes, width, height);
+    }
+    if (rc != 0) {
+    } else if (p->spatial_dir.hash != 0) {
         for (int i = 0; i < 6; ++i) f.plane[i] = (double*)malloc(sizeof(double) * N);
         run_rows(&f, rows_dir_source, threads);
         rc = fjo_dir_filter(&p->spatial_dir, f.plane, 3, width, height);
@@ -1195,6 +1364,7 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
                     if (comps == 4) o[3] = px[3];
                 }
     }
+    for (int c = 0; c < 3; ++c) free(f.carrier[c]);
     free(f.linear);
     pthread_mutex_destroy(&f.lock);
     if (error_flags) *error_flags = f.errors;

@@ -52,6 +52,10 @@ void fjo_yvv_reflect_line(const double* in, double* out, int n, ptrdiff_t stride
 /* Spatial DIR operator on `nplanes` W x H double planes in place. */
 int fjo_dir_filter(const FjSpatialDir* sd, double* const* planes, int nplanes, int W, int H);
 
+/* Scatter/halation on three W x H double planes (R, G, B) in place
+ * (JuicerCudaScatterHalation.cu:690-852). */
+int fjo_halation(const FjScatterHalation* d, double* const planes[3], int W, int H);
+
 typedef struct FjoPlan FjoPlan;
 
 /* Builds host state for one prepared record (scanner LUT, curves). Returns

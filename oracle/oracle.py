@@ -40,6 +40,7 @@ def lib():
         L.fjo_yvv_reflect_line.argtypes = [dp, dp, C.c_int, C.c_ssize_t, dp]
         L.fjo_yvv_boundary.argtypes = [dp, C.c_int, dp, dp]
         L.fjo_dir_filter.argtypes = [C.c_void_p, C.POINTER(dp), C.c_int, C.c_int, C.c_int]
+        L.fjo_halation.argtypes = [C.c_void_p, C.POINTER(dp), C.c_int, C.c_int]
         L.fjo_render.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t, fp, C.c_ssize_t,
                                  C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_uint32)]
         _lib = L
@@ -108,6 +109,18 @@ def dir_filter(spatial_dir, planes) -> list:
     rc = lib().fjo_dir_filter(C.addressof(spatial_dir), ptrs, len(arrs), w, h)
     if rc != 0:
         raise RuntimeError(f"oracle dir filter rc={rc}")
+    return arrs
+
+
+def halation(descriptor, planes) -> list:
+    """Scatter/halation (FP64) on [R, G, B] HxW planes; returns new planes."""
+    arrs = [np.ascontiguousarray(p, dtype=np.float64).copy() for p in planes]
+    h, w = arrs[0].shape
+    dp = C.POINTER(C.c_double)
+    ptrs = (dp * 3)(*[a.ctypes.data_as(dp) for a in arrs])
+    rc = lib().fjo_halation(C.addressof(descriptor), ptrs, w, h)
+    if rc != 0:
+        raise RuntimeError(f"oracle halation rc={rc}")
     return arrs
 
 

@@ -2,6 +2,7 @@ import os
 import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 for p in (os.path.join(ROOT, "film-juicer_amd"), ROOT):
     if p not in sys.path:
         sys.path.insert(0, p)

@@ -14,6 +14,13 @@ Sources (all data files held by the reference's tests):
   * tests/scanner/lut-fixtures/native.json             -> scanner_lut_cases.npz
       Exact scanner PCHIP LUT words for small synthetic inputs and for the
       four production LUTs (one per route).
+  * tests/scatter_halation/frame-fixtures/native-payload.json
+      -> halation_frame_cases.json: resolved halation inputs with the exact
+      descriptor words (kinds, radii, FIR and YVV words) the Rust frame
+      builder produced, plus its f64 exp primitives.
+  * tests/scatter_halation/fixtures/{manifest.json,scatter_halation_reference.bin}
+      -> halation_cohort.json + halation_cohort.npz: the external operator
+      cohort (source carrier planes and expected planes).
 """
 from __future__ import annotations
 
@@ -191,8 +198,54 @@ def dir_fixtures() -> None:
         json.dump(keep, fh)
 
 
+def _gauss_words(g):
+    return [g["kind"], g["radius"]] + list(g["weights"]) + list(g["yvv"])
+
+
+def halation_fixtures() -> None:
+    """Halation frame descriptors and the operator cohort."""
+    base = f"{REF}/tests/scatter_halation"
+    native = json.load(open(f"{base}/frame-fixtures/native-payload.json"))
+    rows = []
+    for group in ("cases", "products"):
+        for c in native[group]:
+            d = c["descriptor"]
+            row = {"group": group, "input": c["input"], "success": c["success"], "present": c["present"]}
+            if d is not None:
+                row["descriptor"] = {
+                    "recipe_hash": d["recipe_hash"], "scatter_amount": d["scatter_amount"],
+                    "channels": [{"strength": ch["strength"], "core": _gauss_words(ch["core"]),
+                                  "tail": [_gauss_words(g) for g in ch["tail"]],
+                                  "bounce": [_gauss_words(g) for g in ch["bounce"]]} for ch in d["channels"]]}
+            rows.append(row)
+    with open(os.path.join(HERE, "halation_frame_cases.json"), "w") as fh:
+        json.dump({"rows": rows, "primitives": native["primitives"]}, fh)
+
+    man = json.load(open(f"{base}/fixtures/manifest.json"))
+    blob = open(f"{base}/fixtures/scatter_halation_reference.bin", "rb").read()
+    arrays, meta = {}, []
+    for r in man["rows"]:
+        entry = {"id": r["id"], "category": r["category"], "comparison": r["comparison"],
+                 "recipe": r["recipeInput"]}
+        for key in ("source", "expected", "downstreamExpected"):
+            if key not in r:
+                continue
+            planes = []
+            for pl in r[key]["planes"]:
+                dt = np.dtype(pl["dtype"])
+                a = np.frombuffer(blob, dtype=dt, count=pl["elementCount"], offset=pl["byteOffset"])
+                assert hashlib.sha256(a.tobytes()).hexdigest() == pl["sha256"], (r["id"], key)
+                planes.append(a.reshape(pl["shape"]))
+            arrays[f"{r['id']}/{key}"] = np.stack(planes)
+            entry.setdefault("planes", []).append(key)
+        meta.append(entry)
+    np.savez_compressed(os.path.join(HERE, "halation_cohort.npz"), **arrays)
+    with open(os.path.join(HERE, "halation_cohort.json"), "w") as fh:
+        json.dump(meta, fh, indent=0)
+
+
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["abi", "prepared", "lut", "dir"]
+    what = sys.argv[1:] or ["abi", "prepared", "lut", "dir", "halation"]
     if "abi" in what:
         abi_facts()
     if "prepared" in what:
@@ -201,3 +254,5 @@ if __name__ == "__main__":
         scanner_luts()
     if "dir" in what:
         dir_fixtures()
+    if "halation" in what:
+        halation_fixtures()
