@@ -22,6 +22,8 @@ struct ExposureParams {
     float rgb_to_xyz[9];    // input_xyz_adapt * input_rgb_to_xyz when adaptation is on
     int decode_cctf;        // 0, or the FJ_INPUT_* space to decode
     float scale;            // exp2(EV) * route correction (validated > 0)
+    float camera_scale;     // exp2(EV) alone: the camera carrier's scale
+    float route_scale;      // route correction applied when a carrier is developed
     // Mallett 2019 reconstruction: k pairs {basis_r*illum, basis_g*illum,
     // basis_b*illum, 0}, {sens_B, sens_G, sens_R, 0}; rgb_to_srgb maps the
     // input to linear sRGB; green mid-gray scale applied to the result.
@@ -106,10 +108,14 @@ struct PixelParams {
     // (raw from the source stage, filtered into the final stage), [3..5] log
     // raw exposure B/G/R.
     float* dir_planes[6];
+    // Camera carrier (film linear exposure B/G/R, width*height each). When
+    // set, the exposure of the source/fused stages is read from it instead
+    // of computed from the source image; the carrier stage writes it.
+    float* carrier[3];
 };
 
 // Pixel stage of one launch.
-enum PixelMode : int { kFused = 0, kDirSource = 1, kDirFinal = 2 };
+enum PixelMode : int { kFused = 0, kDirSource = 1, kDirFinal = 2, kCarrier = 3 };
 
 // Post stage: unsharp mask on the three linear planes, output encoding,
 // alpha copied from the source (JuicerCudaScanPipeline.cu:2733-2878).

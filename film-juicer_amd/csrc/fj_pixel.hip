@@ -42,6 +42,19 @@ __device__ __forceinline__ LdsLayout stage_tables(const PixelParams& p, float* l
     return L;
 }
 
+// Film exposure of one pixel: from the camera carrier with the route
+// correction (JuicerCudaDeviceHelpers.cuh:746-755, :838-848) when a carrier
+// exists, else straight from the source pixel.
+__device__ __forceinline__ void film_exposure(const PixelParams& p, const float4* mallett, const float in[4],
+                                              long long idx, float E[3]) {
+    if (p.carrier[0]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) E[c] = fmaxf(0.0f, p.carrier[c][idx] * p.ex.route_scale);
+        return;
+    }
+    camera_exposure(p.ex, mallett, in, E);
+}
+
 template <bool kPrint, bool kLds, int kMode>
 __global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int tiles) {
     extern __shared__ float4 lds_raw[];
@@ -74,9 +87,17 @@ __global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int til
             }
         }
         float D[3];
-        if constexpr (kMode == kDirSource) {
-            float E[3], corr[3], log_raw[3];
+        if constexpr (kMode == kCarrier) {
+            // Camera carrier: exposure with the manual scale only
+            // (camera_film_linear_exposure_kernel, JuicerCudaFilmPipeline.cu:3103-3146).
+            float E[3];
             camera_exposure(p.ex, L.mallett, in, E);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p.carrier[c][idx] = E[c];
+            continue;
+        } else if constexpr (kMode == kDirSource) {
+            float E[3], corr[3], log_raw[3];
+            film_exposure(p, L.mallett, in, idx, E);
             errors |= dir_source(p.dev, L.film, E, corr, log_raw);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -94,7 +115,7 @@ __global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int til
             errors |= develop_dir_final(p.dev, L.film, log_raw, filtered, D);
         } else {
             float E[3];
-            camera_exposure(p.ex, L.mallett, in, E);
+            film_exposure(p, L.mallett, in, idx, E);
             errors |= develop_film(p.dev, L.film, E, D);
         }
         if constexpr (kPrint) {
@@ -188,6 +209,18 @@ hipError_t launch_pixel_stage(const PixelParams& p, int mode, int grid_limit, hi
                 hipLaunchKernelGGL((pixel_stage_kernel<false, true, kDirSource>), dim3(grid), dim3(block), lq, stream, q, tiles);
             else
                 hipLaunchKernelGGL((pixel_stage_kernel<false, false, kDirSource>), dim3(grid), dim3(block), 0, stream, q, tiles);
+            return hipGetLastError();
+        }
+        case kCarrier: {
+            // Exposure only, with the camera scale (no route correction).
+            PixelParams q = p;
+            q.print = 0;
+            q.ex.scale = p.ex.camera_scale;
+            const size_t lq = fused_lds_bytes(q);
+            if (use_lds)
+                hipLaunchKernelGGL((pixel_stage_kernel<false, true, kCarrier>), dim3(grid), dim3(block), lq, stream, q, tiles);
+            else
+                hipLaunchKernelGGL((pixel_stage_kernel<false, false, kCarrier>), dim3(grid), dim3(block), 0, stream, q, tiles);
             return hipGetLastError();
         }
         case kDirFinal: return launch_mode<kDirFinal>(p, grid, tiles, lds, use_lds, stream);

@@ -1,14 +1,17 @@
 // fj_plan.cpp — validation of one render call, packing of prepared tables into
 // device layouts, and the frame schedule on the caller's stream.
 #include "fj_plan.h"
+#include "fj_halation_host.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <limits>
 #include <string>
 #include <vector>
 
+#include "fj_halation.h"
 #include "fj_host.h"
 #include "fj_spatial.h"
 
@@ -56,7 +59,14 @@ hipError_t DeviceState::init(int dev) {
         e = hipEventCreate(&stage_events[i]);
         if (e != hipSuccess) return e;
     }
-    return fft_kernels_init(&max_line);
+    e = fft_kernels_init(&max_line);
+    if (e != hipSuccess) return e;
+    if (halation_kernels_init() != hipSuccess) {
+        // Without the larger LDS window the row pass holds shorter rows.
+        (void)hipGetLastError();
+        halation_max_line = 7680;
+    }
+    return hipSuccess;
 }
 
 hipError_t DeviceState::release() {
@@ -308,12 +318,28 @@ bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
     if (p.scanner_post.flags & FJ_POST_GLARE) return d.unsupported("scanner glare");
     if (p.scanner_post.hash != 0 && p.scanner_post.lens_blur_sigma_px > 0.0f && p.scanner_post.lens_blur_radius > 0)
         return d.unsupported("scanner lens blur");
-    if (p.optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_SCATTER_HALATION |
-                          FJ_OPTICS_HIGHLIGHT_BOOST))
-        return d.unsupported("spatial optics");
+    if (p.optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_HIGHLIGHT_BOOST))
+        return d.unsupported("camera/enlarger diffusion and highlight boost");
+    if (p.optics.flags & ~(FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_SCATTER_HALATION |
+                           FJ_OPTICS_HIGHLIGHT_BOOST))
+        return d.malformed("optics.flags");
+    if (p.optics.flags & FJ_OPTICS_SCATTER_HALATION) {
+        std::string field;
+        if (!halation_validate(p.optics.scatter_halation, field)) return d.malformed(field.c_str());
+    } else if (p.optics.scatter_halation.recipe_hash != 0) {
+        return d.malformed("optics.inactive_scatter_halation");
+    }
+    if (p.optics.flags != 0) {
+        // Optics need the complete frame domain (native/juicer_cuda_api.cpp:432-436).
+        auto same = [](const FjRect& a, const FjRect& b) {
+            return a.x1 == b.x1 && a.y1 == b.y1 && a.x2 == b.x2 && a.y2 == b.y2;
+        };
+        if (!same(f.source.bounds, f.full_frame_extent) || !same(f.destination.bounds, f.full_frame_extent) ||
+            !same(f.render_window, f.full_frame_extent))
+            return d.unsupported("optics require the complete frame domain");
+    }
     if (p.grain.flags & FJ_GRAIN_ACTIVE) return d.unsupported("visual grain");
     if (p.effects.flags != 0 || p.effects.hash != 0) return d.unsupported("film/gate effects");
-    (void)f;
     return true;
 }
 
@@ -512,6 +538,8 @@ hipError_t assemble(DeviceState& dev, const FjPreparedHostData& p, PixelParams& 
     }
     float scale = manual * ex.route_correction_scale;
     pp.ex.scale = std::isfinite(scale) && scale > 0.0f ? scale : 1.0f;
+    pp.ex.camera_scale = manual;
+    pp.ex.route_scale = ex.route_correction_scale;
     const void* ptr = nullptr;
     hipError_t e = hipSuccess;
     if (ex.method == FJ_RAW_MALLETT_2019) {
@@ -768,8 +796,25 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     const int grid_limit = dev.compute_units * 8;
     const bool spatial_dir = prepared.spatial_dir.hash != 0;
     const bool post = post_active(prepared.scanner_post);
+    const bool halation = (prepared.optics.flags & FJ_OPTICS_SCATTER_HALATION) != 0;
     const long long plane = static_cast<long long>(pp.io.width) * pp.io.height;
-    const int planes_needed = spatial_dir ? 12 : (post ? 3 : 0);
+    if (halation && pp.io.width > dev.halation_max_line)
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput halation row above the LDS line limit", error);
+
+    // Workspace planes: [DIR 12][carrier 3][halation rows 3K]; the unsharp
+    // input planes reuse DIR scratch or the halation rows once those are done.
+    HalationStages hs;
+    int hal_k = 0;
+    if (halation) {
+        halation_stages(prepared.optics.scatter_halation, pp.io.width, pp.io.height, hs);
+        for (int c = 0; c < 3; ++c) hal_k = std::max({hal_k, hs.scatter[c].count, hs.bounce[c].count});
+    }
+    const int dir_off = 0;
+    const int carrier_off = spatial_dir ? 12 : 0;
+    const int q_off = carrier_off + (halation ? 6 : 0);  // carrier + its ping-pong copy
+    int planes_needed = q_off + (halation ? 3 * hal_k : 0);
+    const int post_off = spatial_dir ? 9 : (halation && hal_k > 0 ? q_off : planes_needed);
+    if (post) planes_needed = std::max(planes_needed, post_off + 3);
     float* ws = nullptr;
     if (planes_needed) {
         e = dev.ensure_workspace(static_cast<size_t>(plane) * planes_needed * sizeof(float));
@@ -778,8 +823,69 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     }
     if (post) {
         pp.linear_out = 1;
-        pp.linear_planes = ws + (spatial_dir ? 9 : 0) * plane;
+        pp.linear_planes = ws + post_off * plane;
     }
+    if (halation) {
+        for (int c = 0; c < 3; ++c) pp.carrier[c] = ws + (carrier_off + c) * plane;
+        mark("camera_carrier");
+        e = launch_pixel_stage(pp, kCarrier, grid_limit, stream);
+        if (e != hipSuccess) return outcome_hip(e, "camera carrier stage", error);
+        HalArgs ha{};
+        // Descriptor channels are R, G, B; carrier planes are B/G/R-sensitive.
+        float* cur[3];
+        float* alt[3];
+        for (int c = 0; c < 3; ++c) {
+            cur[c] = pp.carrier[2 - c];
+            alt[c] = ws + (carrier_off + 3 + 2 - c) * plane;
+            for (int k = 0; k < hal_k; ++k) ha.q[c][k] = ws + (q_off + c * hal_k + k) * plane;
+        }
+        auto fir_only = [](const HalStageDev* st) {
+            for (int c = 0; c < 3; ++c)
+                for (int k = 0; k < st[c].count; ++k)
+                    if (st[c].g[k].kind > kHalFir) return false;
+            return true;
+        };
+        // FIR-only stages run tiled from one buffer into the other; the
+        // recursive stages run in place.
+        auto run_stage = [&](const HalStageDev* host, const HalStageDev* device) -> hipError_t {
+            ha.stage = device;
+            if (fir_only(host)) {
+                for (int c = 0; c < 3; ++c) {
+                    ha.in[c] = cur[c];
+                    ha.out[c] = alt[c];
+                }
+                hipError_t r = launch_halation_fir_stage(ha, stream);
+                for (int c = 0; c < 3; ++c) std::swap(cur[c], alt[c]);
+                return r;
+            }
+            for (int c = 0; c < 3; ++c) {
+                ha.in[c] = cur[c];
+                ha.out[c] = cur[c];
+            }
+            return launch_halation_stage(ha, stream);
+        };
+        ha.width = pp.io.width;
+        ha.height = pp.io.height;
+        ha.row_len = hs.row_len;
+        ha.col_len = hs.col_len;
+        const void* stage_ptr = nullptr;
+        const uint64_t hkey = hash_bytes(&hs, sizeof(hs), 0x4841ull);
+        e = dev.table(hkey, &hs, sizeof(hs), &stage_ptr);
+        if (e != hipSuccess) return outcome_hip(e, "halation tables", error);
+        const char* dstages = static_cast<const char*>(stage_ptr);
+        if (hs.scatter_active) {
+            mark("halation_scatter");
+            e = run_stage(hs.scatter, reinterpret_cast<const HalStageDev*>(dstages + offsetof(HalationStages, scatter)));
+            if (e != hipSuccess) return outcome_hip(e, "halation scatter stage", error);
+        }
+        if (hs.bounce_active) {
+            mark("halation_bounce");
+            e = run_stage(hs.bounce, reinterpret_cast<const HalStageDev*>(dstages + offsetof(HalationStages, bounce)));
+            if (e != hipSuccess) return outcome_hip(e, "halation back-reflection stage", error);
+        }
+        for (int c = 0; c < 3; ++c) pp.carrier[2 - c] = cur[c];
+    }
+    (void)dir_off;
     if (spatial_dir) {
         for (int c = 0; c < 6; ++c) pp.dir_planes[c] = ws + c * plane;
         mark("dir_source");

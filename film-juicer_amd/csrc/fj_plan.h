@@ -25,6 +25,7 @@ struct DeviceState {
     int device = -1;
     int compute_units = 0;
     int max_line = 4096;  // longest DCT line the LDS transform accepts on this device
+    int halation_max_line = 8192;  // longest row the halation row pass holds in LDS
     std::unordered_map<uint64_t, DeviceTable> tables;
     std::list<uint64_t> lru;  // most recent at front
     // Device transfer functions of spatial filters, by descriptor and frame size.

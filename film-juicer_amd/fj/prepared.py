@@ -224,3 +224,25 @@ def simple_view(p: Prepared, *, encode: bool = True, gamut: bool = True, unsharp
         r.output_color.flags &= ~abi.FJ_COLOR_GAMUT_COMPRESSION
         r.output_color.gamut_cmax = abi.FjFloatSpan(None, 0)
     return p
+
+
+def with_halation(p: Prepared, width: int, height: int, pixel_size_um: float, *, legacy: bool = True,
+                  **recipe) -> Prepared:
+    """Turns on scatter/halation for a full width x height frame with the
+    descriptor the frame builder composes at `pixel_size_um`
+    (optics/halation_frame.rs); `recipe` goes to descriptors.halation_recipe."""
+    from . import descriptors
+    r = descriptors.halation_recipe(**recipe)
+    if not legacy:
+        r["legacy_scatter"] = False
+        r["legacy_reflection"] = False
+    d = descriptors.halation_frame(r, pixel_size_um)
+    o = p.record.optics
+    if d is None:
+        o.flags &= ~abi.FJ_OPTICS_SCATTER_HALATION
+        o.scatter_halation = abi.FjScatterHalation()
+        return p
+    o.flags |= abi.FJ_OPTICS_SCATTER_HALATION
+    o.full_frame = abi.FjExtent(0, 0, width, height)
+    o.scatter_halation = d
+    return p
