@@ -19,9 +19,15 @@
 // with lane shuffles and across the waves of a block through LDS, so a scan
 // costs two barriers.
 //
-// The recursions carry FP64 state and arithmetic (planes stay FP32): the
-// direct form's DC gain is 1/B for injected rounding noise, with B ~ 1e-4 at
-// a 30 px sigma, so an FP32 state would drift by ~1e-3 of the signal.
+// Every recursion runs in the factored form of JuicerCudaScatterHalation.cu
+// :183-198 (real pole p, second-order section y with increment d) in FP32.
+// Direct-form descriptors (B, B1, B2, B3) are converted on the host, in FP64,
+// to the factored form with the same poles, an output gain kappa (the direct
+// form's DC gain, which f32 coefficient rounding moves off 1) and the state
+// that reproduces the direct form's "all past outputs = first sample" seed.
+// The direct form itself has a rounding-noise gain of 1/B (B ~ 1e-4 at a
+// 30 px sigma), so evaluating it in FP32 would drift by ~1e-3 of the signal;
+// the factored form stays within a few 1e-7.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,17 +40,20 @@ constexpr int kHalColWidth = 16;     // columns per column-kernel block
 constexpr int kHalColChunks = 64;    // chunks per column
 constexpr int kHalMaxLine = 8192;    // longest row the row kernel holds in LDS
 
-enum HalKind : int { kHalIdentity = 0, kHalFir = 1, kHalYvv = 2, kHalYvvFactored = 3 };
+enum HalKind : int { kHalIdentity = 0, kHalFir = 1, kHalRecursive = 2 };
 
 struct HalGaussDev {
     int kind;
     int radius;
     float w[19];
-    double c[4];           // legacy {B, B1, B2, B3}; factored {h, g, t, 0} (the f32 words, widened)
-    // Chunk transition powers, row-major 3x3: row_pow[n] = (A^Lr)^n for
-    // n <= 64 (chunks per wave in the row pass); col_pow[n] = (A^Lc)^n, n <= 4.
-    double row_pow[65][9];
-    double col_pow[5][9];
+    float h, g, t;         // factored recursion
+    float kappa;           // output gain
+    float seed[3];         // state seeded from a pass's first sample: seed * first
+    // Chunk transition powers of the state (p, y, d), row-major 3x3:
+    // row_pow[n] = (A^Lr)^n for n <= 64 (chunks per wave in the row pass);
+    // col_pow[n] = (A^Lc)^n, n <= 4.
+    float row_pow[65][9];
+    float col_pow[5][9];
 };
 
 struct HalStageDev {

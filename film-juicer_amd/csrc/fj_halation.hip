@@ -30,35 +30,36 @@ __device__ __forceinline__ int hal_reflect(int i, int n) {
     return i >= n ? period - 1 - i : i;
 }
 
-// One step of the recursion. Legacy state (s1, s2, s3) = last three outputs;
-// factored state (p, y, d) = real-pole output, section output, last difference.
-__device__ __forceinline__ double hal_step(bool factored, const double c[4], double x, double s[3]) {
-    if (factored) {
-        s[0] = fma(c[0], x - s[0], s[0]);
-        s[2] = fma(c[2], s[2], c[1] * (s[0] - s[1]));
-        s[1] += s[2];
-        return s[1];
-    }
-    const double v = fma(c[0], x, fma(c[3], s[2], fma(c[2], s[1], c[1] * s[0])));
-    s[2] = s[1];
-    s[1] = s[0];
-    s[0] = v;
-    return v;
+// Factored recursion (JuicerCudaScatterHalation.cu:186-191): real pole p,
+// second-order section y advanced by the increment d; output kappa * y.
+struct Rec {
+    float h, g, t, kappa, seed[3];
+};
+
+__device__ __forceinline__ Rec load_rec(const HalGaussDev* g) {
+    return Rec{g->h, g->g, g->t, g->kappa, {g->seed[0], g->seed[1], g->seed[2]}};
 }
 
-__device__ __forceinline__ void hal_seed(bool factored, double first, double s[3]) {
-    s[0] = first;
-    s[1] = first;
-    s[2] = factored ? 0.0 : first;
+__device__ __forceinline__ float hal_step(const Rec& r, float x, float s[3]) {
+    s[0] = fmaf(r.h, x - s[0], s[0]);
+    s[2] = fmaf(r.t, s[2], r.g * (s[0] - s[1]));
+    s[1] += s[2];
+    return r.kappa * s[1];
 }
 
-__device__ __forceinline__ void hal_zero(double s[3]) { s[0] = s[1] = s[2] = 0.0; }
+__device__ __forceinline__ void hal_seed(const Rec& r, float first, float s[3]) {
+    s[0] = r.seed[0] * first;
+    s[1] = r.seed[1] * first;
+    s[2] = r.seed[2] * first;
+}
+
+__device__ __forceinline__ void hal_zero(float s[3]) { s[0] = s[1] = s[2] = 0.0f; }
 
 // Runs the recursion over samples i < len at load(i); with kWrite the outputs
 // go to store(i, v). Each group of kG samples is loaded before its serial
 // steps, so the load latency is paid once per group.
 template <int kG, bool kWrite, typename Load, typename Store>
-__device__ __forceinline__ void run_chunk(bool fac, const double c[4], double s[3], int len, Load load, Store store) {
+__device__ __forceinline__ void run_chunk(const Rec& r, float s[3], int len, Load load, Store store) {
     for (int b = 0; b < len; b += kG) {
         float v[kG];
 #pragma unroll
@@ -66,8 +67,8 @@ __device__ __forceinline__ void run_chunk(bool fac, const double c[4], double s[
 #pragma unroll
         for (int i = 0; i < kG; ++i) {
             if (b + i < len) {
-                const double o = hal_step(fac, c, v[i], s);
-                if constexpr (kWrite) store(b + i, static_cast<float>(o));
+                const float o = hal_step(r, v[i], s);
+                if constexpr (kWrite) store(b + i, o);
             }
         }
     }
@@ -79,52 +80,52 @@ __device__ __forceinline__ void run_chunk(bool fac, const double c[4], double s[
 // uses lane shuffles; the waves' totals meet in LDS. Returns E_{r-1} (zero
 // for r = 0). Every thread of the block must call it.
 template <int kLines>
-__device__ __forceinline__ void chunk_scan(double* buf, int line, const double e[3], const double* tab,
-                                           double prev[3]) {
+__device__ __forceinline__ void chunk_scan(float* buf, int line, const float e[3], const float* tab,
+                                           float prev[3]) {
     constexpr int kPer = 64 / kLines;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int rw = lane / kLines;
-    double E0 = e[0], E1 = e[1], E2 = e[2];
+    float E0 = e[0], E1 = e[1], E2 = e[2];
 #pragma unroll
     for (int d = 1; d < kPer; d <<= 1) {
-        const double o0 = __shfl_up(E0, d * kLines, 64);
-        const double o1 = __shfl_up(E1, d * kLines, 64);
-        const double o2 = __shfl_up(E2, d * kLines, 64);
+        const float o0 = __shfl_up(E0, d * kLines, 64);
+        const float o1 = __shfl_up(E1, d * kLines, 64);
+        const float o2 = __shfl_up(E2, d * kLines, 64);
         if (rw >= d) {
-            const double* P = tab + d * 9;
-            E0 += fma(P[0], o0, fma(P[1], o1, P[2] * o2));
-            E1 += fma(P[3], o0, fma(P[4], o1, P[5] * o2));
-            E2 += fma(P[6], o0, fma(P[7], o1, P[8] * o2));
+            const float* P = tab + d * 9;
+            E0 += fmaf(P[0], o0, fmaf(P[1], o1, P[2] * o2));
+            E1 += fmaf(P[3], o0, fmaf(P[4], o1, P[5] * o2));
+            E2 += fmaf(P[6], o0, fmaf(P[7], o1, P[8] * o2));
         }
     }
-    double x0 = __shfl_up(E0, kLines, 64);
-    double x1 = __shfl_up(E1, kLines, 64);
-    double x2 = __shfl_up(E2, kLines, 64);
-    if (rw == 0) x0 = x1 = x2 = 0.0;
+    float x0 = __shfl_up(E0, kLines, 64);
+    float x1 = __shfl_up(E1, kLines, 64);
+    float x2 = __shfl_up(E2, kLines, 64);
+    if (rw == 0) x0 = x1 = x2 = 0.0f;
     if (rw == kPer - 1) {
-        double* me = buf + (wave * kLines + line) * 3;
+        float* me = buf + (wave * kLines + line) * 3;
         me[0] = E0;
         me[1] = E1;
         me[2] = E2;
     }
     __syncthreads();
     // End state of the previous waves' last chunk.
-    double T0 = 0.0, T1 = 0.0, T2 = 0.0;
-    const double* Mw = tab + kPer * 9;
+    float T0 = 0.0, T1 = 0.0, T2 = 0.0f;
+    const float* Mw = tab + kPer * 9;
     for (int w = 0; w < wave; ++w) {
-        const double* o = buf + (w * kLines + line) * 3;
-        const double n0 = fma(Mw[0], T0, fma(Mw[1], T1, fma(Mw[2], T2, o[0])));
-        const double n1 = fma(Mw[3], T0, fma(Mw[4], T1, fma(Mw[5], T2, o[1])));
-        const double n2 = fma(Mw[6], T0, fma(Mw[7], T1, fma(Mw[8], T2, o[2])));
+        const float* o = buf + (w * kLines + line) * 3;
+        const float n0 = fmaf(Mw[0], T0, fmaf(Mw[1], T1, fmaf(Mw[2], T2, o[0])));
+        const float n1 = fmaf(Mw[3], T0, fmaf(Mw[4], T1, fmaf(Mw[5], T2, o[1])));
+        const float n2 = fmaf(Mw[6], T0, fmaf(Mw[7], T1, fmaf(Mw[8], T2, o[2])));
         T0 = n0;
         T1 = n1;
         T2 = n2;
     }
-    const double* P = tab + rw * 9;
-    prev[0] = fma(P[0], T0, fma(P[1], T1, fma(P[2], T2, x0)));
-    prev[1] = fma(P[3], T0, fma(P[4], T1, fma(P[5], T2, x1)));
-    prev[2] = fma(P[6], T0, fma(P[7], T1, fma(P[8], T2, x2)));
+    const float* P = tab + rw * 9;
+    prev[0] = fmaf(P[0], T0, fmaf(P[1], T1, fmaf(P[2], T2, x0)));
+    prev[1] = fmaf(P[3], T0, fmaf(P[4], T1, fmaf(P[5], T2, x1)));
+    prev[2] = fmaf(P[6], T0, fmaf(P[7], T1, fmaf(P[8], T2, x2)));
     __syncthreads();
 }
 
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(kHalRowThreads) void hal_rows_kernel(HalArgs a) {
     const int count = st->count;
     if (count == 0) return;
     extern __shared__ float hal_lds[];
-    __shared__ double scan[(kHalRowThreads / 64) * 3];
+    __shared__ float scan[(kHalRowThreads / 64) * 3];
     const int W = a.width;
     float* xs = hal_lds;
     float* ys = hal_lds + W;
@@ -184,29 +185,28 @@ __global__ __launch_bounds__(kHalRowThreads) void hal_rows_kernel(HalArgs a) {
                 ys[i] = sum;
             }
         } else {
-            const bool fac = kind == kHalYvvFactored;
-            const double c[4] = {g->c[0], g->c[1], g->c[2], g->c[3]};
-            const double* tab = &g->row_pow[0][0];
-            double s[3], prev[3];
+            const Rec rec = load_rec(g);
+            const float* tab = &g->row_pow[0][0];
+            float s[3], prev[3];
             // Left to right; chunk 0 is seeded with x[0].
-            if (t == 0) hal_seed(fac, xs[0], s);
+            if (t == 0) hal_seed(rec, xs[0], s);
             else hal_zero(s);
-            run_chunk<kRowGroup, false>(fac, c, s, len, fwd_x, none);
+            run_chunk<kRowGroup, false>(rec, s, len, fwd_x, none);
             chunk_scan<1>(scan, 0, s, tab, prev);
-            if (t == 0) hal_seed(fac, xs[0], s);
+            if (t == 0) hal_seed(rec, xs[0], s);
             else s[0] = prev[0], s[1] = prev[1], s[2] = prev[2];
-            run_chunk<kRowGroup, true>(fac, c, s, len, fwd_x, fwd_y);
+            run_chunk<kRowGroup, true>(rec, s, len, fwd_x, fwd_y);
             __syncthreads();
             // Right to left; the last (shortest) chunk is seeded with the
             // forward pass's terminal value.
             const float terminal = ys[W - 1];
-            if (qb == last) hal_seed(fac, terminal, s);
+            if (qb == last) hal_seed(rec, terminal, s);
             else hal_zero(s);
-            run_chunk<kRowGroup, false>(fac, c, s, lenb, bwd_y, none);
+            run_chunk<kRowGroup, false>(rec, s, lenb, bwd_y, none);
             chunk_scan<1>(scan, 0, s, tab, prev);
-            if (qb == last) hal_seed(fac, terminal, s);
+            if (qb == last) hal_seed(rec, terminal, s);
             else s[0] = prev[0], s[1] = prev[1], s[2] = prev[2];
-            run_chunk<kRowGroup, true>(fac, c, s, lenb, bwd_y, bwd_store);
+            run_chunk<kRowGroup, true>(rec, s, lenb, bwd_y, bwd_store);
         }
         __syncthreads();
         float* Q = a.q[ch][k] + static_cast<size_t>(y) * W;
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kHalColWidth * kHalColChunks) void hal_cols_kernel(
     const HalStageDev* st = a.stage + ch;
     const int count = st->count;
     if (count == 0) return;
-    __shared__ double scan[(kHalColChunks * kHalColWidth / 64) * kHalColWidth * 3];
+    __shared__ float scan[(kHalColChunks * kHalColWidth / 64) * kHalColWidth * 3];
     const int cx = threadIdx.x % kHalColWidth;
     const int j = threadIdx.x / kHalColWidth;
     const int W = a.width, H = a.height, L = a.col_len;
@@ -242,16 +242,15 @@ __global__ __launch_bounds__(kHalColWidth * kHalColChunks) void hal_cols_kernel(
     const int top = (H + L - 1) / L - 1;  // chunk holding row 0
     const float* Xin = a.in[ch];
     float* Xout = a.out[ch];
-    double S[kHalMaxGauss][3];
+    float S[kHalMaxGauss][3];
 #pragma unroll
     for (int k = 0; k < kHalMaxGauss; ++k) {
         hal_zero(S[k]);
         if (k >= count) continue;
         const HalGaussDev* g = st->g + k;
-        if (g->kind < kHalYvv) continue;
-        const bool fac = g->kind == kHalYvvFactored;
-        const double c[4] = {g->c[0], g->c[1], g->c[2], g->c[3]};
-        const double* tab = &g->col_pow[0][0];
+        if (g->kind != kHalRecursive) continue;
+        const Rec rec = load_rec(g);
+        const float* tab = &g->col_pow[0][0];
         float* Q = a.q[ch][k];
         auto up = [&](int i) { return Q[static_cast<size_t>(hi - i) * W + xc]; };
         auto up_store = [&](int i, float v) {
@@ -259,24 +258,24 @@ __global__ __launch_bounds__(kHalColWidth * kHalColChunks) void hal_cols_kernel(
         };
         auto down = [&](int i) { return Q[static_cast<size_t>(lod + i) * W + xc]; };
         auto none = [](int, float) {};
-        double s[3], prev[3];
+        float s[3], prev[3];
         // Bottom to top, seeded with the bottom row.
         const float bottom = Q[static_cast<size_t>(H - 1) * W + xc];
-        if (j == 0) hal_seed(fac, bottom, s);
+        if (j == 0) hal_seed(rec, bottom, s);
         else hal_zero(s);
-        run_chunk<kColGroup, false>(fac, c, s, len, up, none);
+        run_chunk<kColGroup, false>(rec, s, len, up, none);
         chunk_scan<kHalColWidth>(scan, cx, s, tab, prev);
-        if (j == 0) hal_seed(fac, bottom, s);
+        if (j == 0) hal_seed(rec, bottom, s);
         else s[0] = prev[0], s[1] = prev[1], s[2] = prev[2];
-        run_chunk<kColGroup, true>(fac, c, s, len, up, up_store);
+        run_chunk<kColGroup, true>(rec, s, len, up, up_store);
         __syncthreads();  // the downward chunks read other threads' upward output
         // Top to bottom, seeded with row 0 of the upward result.
         const float first = Q[xc];
-        if (jd == top) hal_seed(fac, first, s);
+        if (jd == top) hal_seed(rec, first, s);
         else hal_zero(s);
-        run_chunk<kColGroup, false>(fac, c, s, lend, down, none);
+        run_chunk<kColGroup, false>(rec, s, lend, down, none);
         chunk_scan<kHalColWidth>(scan, cx, s, tab, prev);
-        if (jd == top) hal_seed(fac, first, S[k]);
+        if (jd == top) hal_seed(rec, first, S[k]);
         else S[k][0] = prev[0], S[k][1] = prev[1], S[k][2] = prev[2];
     }
     // Final downward sweep: every Gaussian's column output, weighted sum.
@@ -313,8 +312,7 @@ __global__ __launch_bounds__(kHalColWidth * kHalColChunks) void hal_cols_kernel(
                     acc[i] = fmaf(om, sum, acc[i]);
                 }
             } else {
-                const bool fac = kind == kHalYvvFactored;
-                const double c[4] = {g->c[0], g->c[1], g->c[2], g->c[3]};
+                const Rec rec = load_rec(g);
                 const float* Q = a.q[ch][k];
                 float qv[kFinGroup];
 #pragma unroll
@@ -322,7 +320,7 @@ __global__ __launch_bounds__(kHalColWidth * kHalColChunks) void hal_cols_kernel(
                     qv[i] = b + i < lend ? Q[static_cast<size_t>(lod + b + i) * W + xc] : 0.0f;
 #pragma unroll
                 for (int i = 0; i < kFinGroup; ++i)
-                    if (b + i < lend) acc[i] = fmaf(om, static_cast<float>(hal_step(fac, c, qv[i], S[k])), acc[i]);
+                    if (b + i < lend) acc[i] = fmaf(om, hal_step(rec, qv[i], S[k]), acc[i]);
             }
         }
         if (live) {

@@ -18,8 +18,9 @@ struct HalationStages {
 
 // Returns false and names the malformed field.
 bool halation_validate(const FjScatterHalation& h, std::string& field);
-// Stage tables for a width x height carrier.
-void halation_stages(const FjScatterHalation& h, int width, int height, HalationStages& out);
+// Stage tables for a width x height carrier; false when a recursion has no
+// stable factored form.
+bool halation_stages(const FjScatterHalation& h, int width, int height, HalationStages& out);
 hipError_t halation_kernels_init();
 
 }  // namespace fjx

@@ -806,7 +806,9 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     HalationStages hs;
     int hal_k = 0;
     if (halation) {
-        halation_stages(prepared.optics.scatter_halation, pp.io.width, pp.io.height, hs);
+        if (!halation_stages(prepared.optics.scatter_halation, pp.io.width, pp.io.height, hs))
+            return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT,
+                                "UnsupportedInput halation recursion without a stable factored form", error);
         for (int c = 0; c < 3; ++c) hal_k = std::max({hal_k, hs.scatter[c].count, hs.bounce[c].count});
     }
     const int dir_off = 0;
