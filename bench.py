@@ -29,6 +29,9 @@ WIDTH, HEIGHT = 3840, 2160
 CASE = "PrintPreflash"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (packed); 78.6 unpacked
 PEAK_HBM_GBS = 8000.0
+# Measured HBM bytes per launch of the DIR-final pixel chain at this workload
+# (FETCH_SIZE 100064 KiB x 2.0 + WRITE_SIZE 97200 KiB x 1.0).
+TRAFFIC_PIXEL_CHAIN = (100064 * 2.0 + 97200 * 1.0) * 1024
 
 
 def flop_census(k_active: int, print_route: bool) -> int:
@@ -193,7 +196,9 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+            # (copy-calibrated x2.0 / x1.0), profiles/staged_uhd_print_halation_dir_unsharp.md.
+            "traffic": TRAFFIC_PIXEL_CHAIN,
             "flop_per_px": fpx,
             "kernel_ms": chain_ms,
         },
