@@ -319,3 +319,166 @@ def halation_frame(inp: dict, pixel_size_um: float):
                 sigma = _ordered_max(first * HALATION_BOUNCE_MULTIPLIERS[i] * scale / pixel, 1.0e-6)
                 put(c.bounce[i], halation_gaussian(sigma, inp["legacy_reflection"]))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Camera / enlarger diffusion (rust film-juicer-core optics.rs:342-372,
+# :376-404, :428-540 and optics/diffusion_frame.rs:182-220, :438-575).
+# ---------------------------------------------------------------------------
+DIFFUSION_FAMILIES = (  # lambda_um (core, halo, bloom), weights, warmth, gain
+    ((10.0, 50.0, 260.0), (0.60, 0.30, 0.10), 0.0, 0.65),
+    ((16.0, 95.0, 380.0), (0.40, 0.47, 0.13), 0.65, 0.75),
+    ((14.0, 150.0, 650.0), (0.28, 0.42, 0.30), 0.40, 1.05),
+    ((20.0, 200.0, 1000.0), (0.22, 0.30, 0.48), 0.85, 1.0),
+)
+BLOOM_ALPHA = (3.2, 3.5, 2.9, 2.5)
+MIN_SCALE = 1.0e-6
+
+
+def _f64_bytes(v: float) -> bytes:
+    import struct
+    return struct.pack("<d", 0.0 if v == 0.0 else v)
+
+
+def scatter_fraction(strength: float, gain: float) -> float:
+    """Piecewise log2-linear strength -> scatter fraction (optics.rs:379-400)."""
+    br = (0.125, 0.25, 0.5, 1.0, 2.0)
+    fr = (0.10, 0.20, 0.35, 0.55, 0.75)
+    ls = math.log2(_ordered_max(strength, MIN_SCALE))
+    f = fr[0]
+    if ls >= math.log2(br[4]):
+        f = fr[4]
+    elif ls > math.log2(br[0]):
+        for i in range(4):
+            lo, hi = math.log2(br[i]), math.log2(br[i + 1])
+            if ls <= hi:
+                t = (ls - lo) / (hi - lo)
+                f = fr[i] + t * (fr[i + 1] - fr[i])
+                break
+    return min(max(f * gain, 0.0), 0.99)
+
+
+def diffusion_recipe(family: int, strength: float, *, spatial_scale: float = 1.0, halo_warmth: float = 0.0,
+                     core_intensity: float = 1.0, core_size: float = 1.0, halo_intensity: float = 1.0,
+                     halo_size: float = 1.0, bloom_intensity: float = 1.0, bloom_size: float = 1.0) -> dict:
+    """Resolved diffusion controls (Diffusion::new, optics.rs:428-540)."""
+    lam, wts, warm, gain = DIFFUSION_FAMILIES[family]
+    sf = scatter_fraction(strength, gain)
+    gw = [wts[g] * _ordered_max(x, 0.0) for g, x in enumerate((core_intensity, halo_intensity, bloom_intensity))]
+    total = gw[0] + gw[1] + gw[2]
+    gl = list(lam)
+    if total <= 0.0:
+        gw = list(wts)
+    else:
+        sizes = (core_size, halo_size, bloom_size)
+        for g in range(3):
+            gw[g] /= total
+            gl[g] = lam[g] * _ordered_max(sizes[g], MIN_SCALE)
+    warmth = min(max(warm + halo_warmth, -1.5), 1.5)
+    return dict(family=family, scatter_fraction=sf, group_weights=tuple(gw), group_lambda_um=tuple(gl),
+                warmth=warmth, spatial_scale=spatial_scale)
+
+
+def diffusion_sample(r: dict, pixel_size_um: float, width: int, height: int):
+    """(radius_px, sample hash) of one stage sample (diffusion_frame.rs:182-220)."""
+    bloom_max = r["group_lambda_um"][2] * 2.5
+    requested = math.ceil(_ordered_max(8.0 * bloom_max * r["spatial_scale"] / pixel_size_um, 5.0))
+    cap = max(min(width, height) // 2 - 1, 1)
+    radius = cap if requested >= cap else int(requested)
+    h = _fnv(FNV_OFFSET, b"diffusion-psf-sample\0")
+    h = _fnv(h, (1).to_bytes(4, "little"))
+    h = _fnv(h, bytes([r["family"]]))
+    for v in tuple(r["group_weights"]) + tuple(r["group_lambda_um"]):
+        h = _fnv(h, _f64_bytes(v))
+    for v in (r["warmth"], r["spatial_scale"], pixel_size_um):
+        h = _fnv(h, _f64_bytes(v))
+    h = _fnv(h, int(radius).to_bytes(4, "little", signed=True))
+    return radius, h
+
+
+def diffusion_stage(r: dict, stage: int, pixel_size_um: float, width: int, height: int) -> abi.FjDiffusion:
+    """FjDiffusion of the camera (stage 0) or enlarger (stage 1) diffusion
+    (Stage::new, diffusion_frame.rs:465-500)."""
+    radius, sh = diffusion_sample(r, pixel_size_um, width, height)
+    d = abi.FjDiffusion()
+    d.family = r["family"]
+    d.scatter_fraction = r["scatter_fraction"]
+    for g in range(3):
+        d.group_weights[g] = r["group_weights"][g]
+        d.group_lambda_um[g] = r["group_lambda_um"][g]
+    d.warmth = r["warmth"]
+    d.spatial_scale = r["spatial_scale"]
+    d.pixel_size_um = pixel_size_um
+    d.radius_px = radius
+    d.sample_hash = sh
+    h = _fnv(FNV_OFFSET, b"diffusion-stage-frame\0")
+    h = _fnv(h, (2).to_bytes(4, "little"))
+    h = _fnv(h, bytes([stage]))
+    h = _fnv(h, _f64_bytes(r["scatter_fraction"]))
+    h = _fnv(h, sh.to_bytes(8, "little"))
+    d.hash = h
+    return d
+
+
+def diffusion_frame_set_hash(route: int, full_frame, camera, enlarger) -> int:
+    """Frame-set identity (FrameSet::new, diffusion_frame.rs:513-566)."""
+    h = _fnv(FNV_OFFSET, b"diffusion-frame-set\0")
+    h = _fnv(h, (2).to_bytes(4, "little"))
+    h = _fnv(h, bytes([route]))
+    for v in full_frame:
+        h = _fnv(h, int(v).to_bytes(4, "little", signed=True))
+    for st in (camera, enlarger):
+        h = _fnv(h, bytes([1 if st is not None else 0]))
+        if st is not None:
+            h = _fnv(h, st.hash.to_bytes(8, "little"))
+    return h
+
+
+def diffusion_components(d) -> tuple:
+    """(lambda_px[9], weights[3][9] for R, G, B) of a stage sample
+    (Components::new, diffusion_frame.rs:286-420)."""
+    def expand(center, spread, count, alpha=None):
+        lo, hi = math.log(center / spread), math.log(center * spread)
+        lam, wts = [], []
+        for i in range(count):
+            if i + 1 == count:
+                ll = hi
+            elif i != 0:
+                ll = lo + (hi - lo) * (i / (count - 1))
+            else:
+                ll = lo
+            lam.append(math.exp(ll))
+            wts.append(lam[-1] ** (2.0 - alpha) if alpha is not None else 1.0)
+        t = 0.0
+        for w in wts:
+            t += w
+        return lam, [w / t for w in wts]
+
+    core = expand(d.group_lambda_um[0], 1.5, 2)
+    halo = expand(d.group_lambda_um[1], 2.0, 3)
+    bloom = expand(d.group_lambda_um[2], 2.5, 4, BLOOM_ALPHA[d.family])
+    # halo warmth weights (halo_weights, diffusion_frame.rs:340-373)
+    hw = halo[1]
+    grad = [-1.0 + 2.0 * i / 2 for i in range(3)]
+    tt = 0.0
+    wg = 0.0
+    for i in range(3):
+        tt += hw[i]
+        wg += hw[i] * grad[i]
+    wg /= tt
+    grad = [g - wg for g in grad]
+    halo_rgb = []
+    for axis in (1.30, 0.15, -1.45):
+        vals = [_ordered_max(hw[i] * (1.0 + d.warmth * axis * grad[i]), 0.0) for i in range(3)]
+        tot = 0.0
+        for v in vals:
+            tot += v
+        halo_rgb.append([v * (tt / tot) for v in vals] if tot > 0.0 else list(hw))
+    scale = _ordered_max(d.spatial_scale, 1e-6)
+    lam_px, W = [], [[], [], []]
+    for gi, (lam, wts) in enumerate((core, halo, bloom)):
+        for i, l in enumerate(lam):
+            lam_px.append(_ordered_max(l * scale / d.pixel_size_um, 1e-6))
+            for ch in range(3):
+                W[ch].append(d.group_weights[1] * halo_rgb[ch][i] if gi == 1 else d.group_weights[gi] * wts[i])
+    return lam_px, W

@@ -382,6 +382,7 @@ typedef struct Frame {
     ptrdiff_t dst_stride;
     double* plane[9]; /* scratch planes */
     double* carrier[3]; /* camera film linear exposure B/G/R, or NULL */
+    double* plin[3];    /* enlarger print-linear exposure C/M/Y, or NULL */
     double* linear;   /* [h][w][3] */
     uint32_t errors;
     pthread_mutex_t lock;
@@ -561,13 +562,15 @@ static uint32_t px_develop(const FjoPlan* pl, const double E[3], double Dcmy[3])
 /* Print exposure (JuicerCudaDeviceHelpers.cuh:866-964) and print development
  * (:1034-1071): 81-λ transmittance x filtered illuminant x paper sensitivity,
  * balance normalizer, preflash, print exposure, log10, print curves. */
-static void px_print(const FjoPlan* pl, double D[3]) {
+/* 81-λ print exposure up to the mid-gray normalizer and preflash
+ * (print_inner_raw_device, JuicerCudaDeviceHelpers.cuh:866-995). */
+static void print_inner(const FjoPlan* pl, const double D[3], double acc[3]) {
     const FjPrint* pr = &pl->p.print;
     const float* eps = pr->film_density_cmy.data;
     const float* base = pr->film_base_density.data;
     const float* sens = pr->sensitivity_cmy.data;
     const float* ill = pr->main_illuminant.data;
-    double acc[3] = {0, 0, 0};
+    acc[0] = acc[1] = acc[2] = 0.0;
     for (int i = 0; i < NSPEC; ++i) {
         const double s[3] = {sens[i * 3], sens[i * 3 + 1], sens[i * 3 + 2]};
         if (isnan(s[0]) && isnan(s[1]) && isnan(s[2])) continue;
@@ -582,14 +585,39 @@ static void px_print(const FjoPlan* pl, double D[3]) {
     for (int c = 0; c < 3; ++c) {
         acc[c] *= (double)pr->normalizer;
         if (preflash && isfinite(pf) && pf > 0.0) acc[c] += (double)pr->preflash_raw_cmy[c] * pf;
-        acc[c] *= (double)pr->exposure;
-        acc[c] *= (double)pl->p.scanner_correction.exposure_scale;
     }
+}
+
+static void print_curves(const FjoPlan* pl, const double logp[3], double D[3]) {
+    const FjPrint* pr = &pl->p.print;
     const int n = (int)pr->log_exposure.count;
     for (int c = 0; c < 3; ++c) {
         const Curve cv = {pr->log_exposure.data, pr->density_cmy.data + c, n, 3};
-        D[c] = curve_at(&cv, log10(acc[c] + 1e-10), 1.0);
+        D[c] = curve_at(&cv, logp[c], 1.0);
     }
+}
+
+/* Print exposure and development (apply_print_pipeline_device, :1047-1071). */
+static void px_print(const FjoPlan* pl, double D[3]) {
+    const FjPrint* pr = &pl->p.print;
+    double acc[3], logp[3];
+    print_inner(pl, D, acc);
+    for (int c = 0; c < 3; ++c) {
+        acc[c] *= (double)pr->exposure;
+        acc[c] *= (double)pl->p.scanner_correction.exposure_scale;
+        logp[c] = log10(acc[c] + 1e-10);
+    }
+    print_curves(pl, logp, D);
+}
+
+/* Enlarger-linear print exposure the enlarger diffusion blurs
+ * (print_linear_for_diffusion_device, :997-1010), C/M/Y-forming planes. */
+static void px_print_linear(const FjoPlan* pl, const double D[3], double lin[3]) {
+    const FjPrint* pr = &pl->p.print;
+    double acc[3];
+    print_inner(pl, D, acc);
+    const double outer = (double)(float)(pr->exposure * pl->p.scanner_correction.exposure_scale);
+    for (int c = 0; c < 3; ++c) lin[c] = (fmax(acc[c], 0.0) + 1e-10) * outer;
 }
 
 /* Tricubic PCHIP sample of the log2 XYZ LUT (JuicerCudaDeviceHelpers.cuh:407-506). */
@@ -1033,6 +1061,129 @@ static void hal_gauss(const FjHalationGaussian* g, const double* src, double* ds
     }
 }
 
+/* ---- camera / enlarger diffusion (JuicerCudaDiffusion.cu) ---------------- */
+
+static double ordered_maxd(double v, double floor) { return v < floor ? floor : v; }
+
+/* Nine exponential PSF components of a stage sample
+ * (Components::new, rust optics/diffusion_frame.rs:286-420). */
+int fjo_diffusion_components(const FjDiffusion* d, double lambda_px[9], double weights[3][9]) {
+    static const double kBloomAlpha[4] = {3.2, 3.5, 2.9, 2.5};
+    if (d->family > 3 || !(d->pixel_size_um > 0.0) || !(d->spatial_scale > 0.0)) return 2;
+    const double centers[3] = {d->group_lambda_um[0], d->group_lambda_um[1], d->group_lambda_um[2]};
+    const double spreads[3] = {1.5, 2.0, 2.5};
+    const int counts[3] = {2, 3, 4};
+    double lam[3][4], w[3][4];
+    for (int g = 0; g < 3; ++g) {
+        const double lo = log(centers[g] / spreads[g]), hi = log(centers[g] * spreads[g]);
+        double total = 0.0;
+        for (int i = 0; i < counts[g]; ++i) {
+            const double ll = i + 1 == counts[g] ? hi : (i ? lo + (hi - lo) * ((double)i / (counts[g] - 1)) : lo);
+            lam[g][i] = exp(ll);
+            w[g][i] = g == 2 ? pow(lam[g][i], 2.0 - kBloomAlpha[d->family]) : 1.0;
+            total += w[g][i];
+        }
+        for (int i = 0; i < counts[g]; ++i) w[g][i] /= total;
+    }
+    /* halo warmth: per-channel reweighting along a centred gradient (:340-373) */
+    double grad[3], tt = 0.0, wg = 0.0, halo[3][3];
+    for (int i = 0; i < 3; ++i) {
+        grad[i] = -1.0 + 2.0 * i / 2.0;
+        tt += w[1][i];
+        wg += w[1][i] * grad[i];
+    }
+    wg /= tt;
+    for (int i = 0; i < 3; ++i) grad[i] -= wg;
+    const double axis[3] = {1.30, 0.15, -1.45};
+    for (int ch = 0; ch < 3; ++ch) {
+        double tot = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            halo[ch][i] = ordered_maxd(w[1][i] * (1.0 + d->warmth * axis[ch] * grad[i]), 0.0);
+            tot += halo[ch][i];
+        }
+        for (int i = 0; i < 3; ++i) halo[ch][i] = tot > 0.0 ? halo[ch][i] * (tt / tot) : w[1][i];
+    }
+    const double scale = ordered_maxd(d->spatial_scale, 1e-6);
+    int k = 0;
+    for (int g = 0; g < 3; ++g)
+        for (int i = 0; i < counts[g]; ++i, ++k) {
+            lambda_px[k] = ordered_maxd(lam[g][i] * scale / d->pixel_size_um, 1e-6);
+            for (int ch = 0; ch < 3; ++ch)
+                weights[ch][k] = g == 1 ? d->group_weights[1] * halo[ch][i] : d->group_weights[g] * w[g][i];
+        }
+    return 0;
+}
+
+/* Normalized PSF samples of one channel on the (2r+1)^2 grid, row-major
+ * [dy + r][dx + r]: FP64 sum of nine exponentials stored as f32, divided by
+ * the FP64 sum of the stored samples (JuicerCudaDiffusion.cu:294-400). */
+int fjo_diffusion_psf(const FjDiffusion* d, int channel, float* psf) {
+    double lam[9], wts[3][9];
+    if (fjo_diffusion_components(d, lam, wts)) return 2;
+    const int r = d->radius_px, n = 2 * r + 1;
+    double sum = 0.0;
+    for (int dy = -r; dy <= r; ++dy)
+        for (int dx = -r; dx <= r; ++dx) {
+            const double rad = sqrt((double)dx * dx + (double)dy * dy);
+            double v = 0.0;
+            for (int k = 0; k < 9; ++k) v += wts[channel][k] * exp(-rad / lam[k]) / (2.0 * 3.141592653589793238462643383279502884 * lam[k] * lam[k]);
+            const float stored = (float)v;
+            psf[(dy + r) * n + dx + r] = stored;
+            sum += stored;
+        }
+    for (int i = 0; i < n * n; ++i) psf[i] = isfinite(sum) && sum > 0.0 ? (float)((double)psf[i] / sum) : NAN;
+    return 0;
+}
+
+/* Whole-sample mirror (JuicerCudaDiffusion.cu:403-411). */
+static int reflect101(long i, int n) {
+    const long period = 2L * (n - 1);
+    long v = i % period;
+    if (v < 0) v += period;
+    return (int)(v < n ? v : period - v);
+}
+
+int fjo_diffusion(const FjDiffusion* d, double* const planes[3], int W, int H) {
+    if (!d || W < 2 || H < 2 || d->radius_px <= 0) return 2;
+    const int r = d->radius_px, n = 2 * r + 1;
+    const size_t N = (size_t)W * H;
+    float* psf = (float*)malloc(sizeof(float) * (size_t)n * n);
+    double* conv = (double*)malloc(sizeof(double) * N);
+    int* rx = (int*)malloc(sizeof(int) * (size_t)(W + 2 * r));
+    int* ry = (int*)malloc(sizeof(int) * (size_t)(H + 2 * r));
+    if (!psf || !conv || !rx || !ry) {
+        free(psf);
+        free(conv);
+        free(rx);
+        free(ry);
+        return 2;
+    }
+    for (int i = 0; i < W + 2 * r; ++i) rx[i] = reflect101(i - r, W);
+    for (int i = 0; i < H + 2 * r; ++i) ry[i] = reflect101(i - r, H);
+    const double f = d->scatter_fraction;
+    for (int ch = 0; ch < 3; ++ch) {
+        if (fjo_diffusion_psf(d, ch, psf)) continue;
+        const double* x = planes[ch];
+        for (int y = 0; y < H; ++y)
+            for (int xx = 0; xx < W; ++xx) {
+                double acc = 0.0;
+                for (int dy = 0; dy < n; ++dy) {
+                    const double* row = x + (size_t)ry[y + dy] * W;
+                    const float* pk = psf + (size_t)dy * n;
+                    for (int dx = 0; dx < n; ++dx) acc += (double)pk[dx] * row[rx[xx + dx]];
+                }
+                conv[(size_t)y * W + xx] = acc;
+            }
+        double* out = planes[ch];
+        for (size_t i = 0; i < N; ++i) out[i] = (1.0 - f) * out[i] + f * conv[i];
+    }
+    free(psf);
+    free(conv);
+    free(rx);
+    free(ry);
+    return 0;
+}
+
 int fjo_halation(const FjScatterHalation* d, double* const planes[3], int W, int H) {
     static const double kTailAmplitude[3] = {0.1633f, 0.6496f, 0.1870f};
     if (!d || W <= 0 || H <= 0) return 2;
@@ -1154,11 +1305,38 @@ static void add_errors(Frame* f, uint32_t bits) {
 /* Print + scan + colour of one developed pixel into the linear planes. */
 static uint32_t px_back(const FjoPlan* pl, Frame* f, size_t idx, double D[3]) {
     double rgb[3];
+    if (pl->print && f->plin[0]) {
+        /* enlarger diffusion follows: keep the print-linear exposure */
+        double lin[3];
+        px_print_linear(pl, D, lin);
+        for (int c = 0; c < 3; ++c) f->plin[c][idx] = lin[c];
+        return 0;
+    }
     if (pl->print) px_print(pl, D);
     const int ok = px_scan(pl, D, rgb);
     if (!ok) rgb[0] = rgb[1] = rgb[2] = 0.0;
     memcpy(f->linear + idx * 3, rgb, sizeof(rgb));
     return ok ? 0u : 1u;
+}
+
+/* Print development from the diffused enlarger exposure, then the scanner
+ * (develop_print_density_from_enlarger_linear_kernel,
+ * JuicerCudaScanPipeline.cu:1449-1494; print_log_encode_diffused_device). */
+static void rows_enlarged(Frame* f, int y0, int y1) {
+    const FjoPlan* pl = f->plan;
+    uint32_t errs = 0;
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < f->w; ++x) {
+            const size_t idx = (size_t)y * f->w + x;
+            double logp[3], D[3], rgb[3];
+            for (int c = 0; c < 3; ++c) logp[c] = log10(fmax(f->plin[c][idx], 0.0) + 1e-10);
+            print_curves(pl, logp, D);
+            const int ok = px_scan(pl, D, rgb);
+            if (!ok) rgb[0] = rgb[1] = rgb[2] = 0.0;
+            memcpy(f->linear + idx * 3, rgb, sizeof(rgb));
+            if (!ok) errs |= 1u;
+        }
+    add_errors(f, errs);
 }
 
 /* Non-spatial chain: exposure -> development -> back end. */
@@ -1297,10 +1475,10 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
                uint32_t* error_flags) {
     if (!plan || !src || width <= 0 || height <= 0 || (comps != 3 && comps != 4)) return 2;
     const FjPreparedHostData* p = &plan->p;
-    if ((p->optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION)) ||
-        (p->grain.flags & FJ_GRAIN_ACTIVE))
-        return 3;
+    if (p->grain.flags & FJ_GRAIN_ACTIVE) return 3;
     const int halation = (p->optics.flags & FJ_OPTICS_SCATTER_HALATION) != 0;
+    const int camera_diffusion = (p->optics.flags & FJ_OPTICS_CAMERA_DIFFUSION) != 0;
+    const int enlarger_diffusion = plan->print && (p->optics.flags & FJ_OPTICS_ENLARGER_DIFFUSION) != 0;
     const size_t N = (size_t)width * height;
     Frame f;
     memset(&f, 0, sizeof(f));
@@ -1315,14 +1493,17 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     f.linear = (double*)malloc(sizeof(double) * N * 3);
     pthread_mutex_init(&f.lock, NULL);
     int rc = 0;
-    if (halation) {
+    if (halation || camera_diffusion) {
         /* expose_print_camera (JuicerCudaExecutor.cpp:1874-1989): carrier,
-         * then scatter/halation in place on it. */
+         * camera diffusion, then scatter/halation in place on it. */
         for (int c = 0; c < 3; ++c) f.carrier[c] = (double*)malloc(sizeof(double) * N);
         run_rows(&f, rows_carrier, threads);
         double* const rgb_planes[3] = {f.carrier[2], f.carrier[1], f.carrier[0]};
-        rc = fjo_halation(&p->optics.scatter_halation, rgb_planes, width, height);
+        if (camera_diffusion) rc = fjo_diffusion(&p->optics.camera, rgb_planes, width, height);
+        if (rc == 0 && halation) rc = fjo_halation(&p->optics.scatter_halation, rgb_planes, width, height);
     }
+    if (enlarger_diffusion)
+        for (int c = 0; c < 3; ++c) f.plin[c] = (double*)malloc(sizeof(double) * N);
     if (rc != 0) {
     } else if (p->spatial_dir.hash != 0) {
         for (int i = 0; i < 6; ++i) f.plane[i] = (double*)malloc(sizeof(double) * N);
@@ -1332,6 +1513,11 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
         for (int i = 0; i < 6; ++i) free(f.plane[i]);
     } else {
         run_rows(&f, rows_chain, threads);
+    }
+    if (rc == 0 && enlarger_diffusion) {
+        /* develop_print_medium (JuicerCudaExecutor.cpp:2282-2382). */
+        rc = fjo_diffusion(&p->optics.enlarger, f.plin, width, height);
+        if (rc == 0) run_rows(&f, rows_enlarged, threads);
     }
     const FjScannerPostEffects* sp = &p->scanner_post;
     if (rc == 0 && sp->hash != 0 && sp->unsharp_radius > 0 && isfinite(sp->unsharp_amount) && sp->unsharp_amount > 0.0f) {
@@ -1364,7 +1550,10 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
                     if (comps == 4) o[3] = px[3];
                 }
     }
-    for (int c = 0; c < 3; ++c) free(f.carrier[c]);
+    for (int c = 0; c < 3; ++c) {
+        free(f.carrier[c]);
+        free(f.plin[c]);
+    }
     free(f.linear);
     pthread_mutex_destroy(&f.lock);
     if (error_flags) *error_flags = f.errors;

@@ -56,6 +56,15 @@ int fjo_dir_filter(const FjSpatialDir* sd, double* const* planes, int nplanes, i
  * (JuicerCudaScatterHalation.cu:690-852). */
 int fjo_halation(const FjScatterHalation* d, double* const planes[3], int W, int H);
 
+/* Diffusion PSF components (rust optics/diffusion_frame.rs:286-420):
+ * lambda_px[9] and per-channel (R, G, B) weights. Returns 0 on success. */
+int fjo_diffusion_components(const FjDiffusion* d, double lambda_px[9], double weights[3][9]);
+/* Normalized f32 PSF samples of one channel, (2r+1)^2 row-major. */
+int fjo_diffusion_psf(const FjDiffusion* d, int channel, float* psf);
+/* Diffusion stage (whole-sample mirrored borders, (1-f) x + f psf*x) on
+ * three W x H planes (R, G, B) in place (JuicerCudaDiffusion.cu:403-487). */
+int fjo_diffusion(const FjDiffusion* d, double* const planes[3], int W, int H);
+
 typedef struct FjoPlan FjoPlan;
 
 /* Builds host state for one prepared record (scanner LUT, curves). Returns

@@ -41,6 +41,9 @@ def lib():
         L.fjo_yvv_boundary.argtypes = [dp, C.c_int, dp, dp]
         L.fjo_dir_filter.argtypes = [C.c_void_p, C.POINTER(dp), C.c_int, C.c_int, C.c_int]
         L.fjo_halation.argtypes = [C.c_void_p, C.POINTER(dp), C.c_int, C.c_int]
+        L.fjo_diffusion.argtypes = [C.c_void_p, C.POINTER(dp), C.c_int, C.c_int]
+        L.fjo_diffusion_psf.argtypes = [C.c_void_p, C.c_int, fp]
+        L.fjo_diffusion_components.argtypes = [C.c_void_p, dp, dp]
         L.fjo_render.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t, fp, C.c_ssize_t,
                                  C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_uint32)]
         _lib = L
@@ -122,6 +125,37 @@ def halation(descriptor, planes) -> list:
     if rc != 0:
         raise RuntimeError(f"oracle halation rc={rc}")
     return arrs
+
+
+def diffusion(descriptor, planes) -> list:
+    """Diffusion stage (FP64) on [R, G, B] HxW planes; returns new planes."""
+    arrs = [np.ascontiguousarray(p, dtype=np.float64).copy() for p in planes]
+    h, w = arrs[0].shape
+    dp = C.POINTER(C.c_double)
+    ptrs = (dp * 3)(*[a.ctypes.data_as(dp) for a in arrs])
+    rc = lib().fjo_diffusion(C.addressof(descriptor), ptrs, w, h)
+    if rc != 0:
+        raise RuntimeError(f"oracle diffusion rc={rc}")
+    return arrs
+
+
+def diffusion_components(descriptor):
+    lam = np.zeros(9)
+    wts = np.zeros((3, 9))
+    dp = C.POINTER(C.c_double)
+    rc = lib().fjo_diffusion_components(C.addressof(descriptor), lam.ctypes.data_as(dp), wts.ctypes.data_as(dp))
+    if rc != 0:
+        raise RuntimeError(f"oracle diffusion components rc={rc}")
+    return lam, wts
+
+
+def diffusion_psf(descriptor, channel: int) -> np.ndarray:
+    n = 2 * descriptor.radius_px + 1
+    out = np.zeros((n, n), dtype=np.float32)
+    rc = lib().fjo_diffusion_psf(C.addressof(descriptor), channel, out.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc != 0:
+        raise RuntimeError(f"oracle diffusion psf rc={rc}")
+    return out
 
 
 class Plan:

@@ -21,6 +21,11 @@ Sources (all data files held by the reference's tests):
   * tests/scatter_halation/fixtures/{manifest.json,scatter_halation_reference.bin}
       -> halation_cohort.json + halation_cohort.npz: the external operator
       cohort (source carrier planes and expected planes).
+  * tests/diffusion/fixtures/diffusion_reference_v1.{json,npz}
+      -> diffusion_cohort.json + diffusion_cohort.npz: the external diffusion
+      operator cohort (17x19 and radius-cap images, camera and enlarger
+      stages, four filter families, HDR inputs) and its family, strength and
+      warmth expansion tables.
 """
 from __future__ import annotations
 
@@ -244,8 +249,25 @@ def halation_fixtures() -> None:
         json.dump(meta, fh, indent=0)
 
 
+def diffusion_fixtures() -> None:
+    """Diffusion operator cohort (inputs/outputs only; PSF arrays dropped)."""
+    base = f"{REF}/tests/diffusion/fixtures/diffusion_reference_v1"
+    man = json.load(open(base + ".json"))
+    z = np.load(base + ".npz", allow_pickle=False)
+    keep = {}
+    meta = {k: man[k] for k in ("family_order", "family_cases", "strength_cases", "warmth_cases")}
+    for group in ("image_cases", "hdr_cases", "radius_cases"):
+        meta[group] = man[group]
+        for c in man[group]:
+            for key in ("input_array", "output_array"):
+                keep[c[key]] = z[c[key]]
+    np.savez_compressed(os.path.join(HERE, "diffusion_cohort.npz"), **keep)
+    with open(os.path.join(HERE, "diffusion_cohort.json"), "w") as fh:
+        json.dump(meta, fh, indent=0)
+
+
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["abi", "prepared", "lut", "dir", "halation"]
+    what = sys.argv[1:] or ["abi", "prepared", "lut", "dir", "halation", "diffusion"]
     if "abi" in what:
         abi_facts()
     if "prepared" in what:
@@ -256,3 +278,5 @@ if __name__ == "__main__":
         dir_fixtures()
     if "halation" in what:
         halation_fixtures()
+    if "diffusion" in what:
+        diffusion_fixtures()
