@@ -112,10 +112,14 @@ struct PixelParams {
     // set, the exposure of the source/fused stages is read from it instead
     // of computed from the source image; the carrier stage writes it.
     float* carrier[3];
+    // Enlarger print-linear exposure (C/M/Y-forming, width*height each). When
+    // set, the fused and DIR-final stages stop at the print exposure and
+    // write it here; the enlarger-final stage develops the print from it.
+    float* plin[3];
 };
 
 // Pixel stage of one launch.
-enum PixelMode : int { kFused = 0, kDirSource = 1, kDirFinal = 2, kCarrier = 3 };
+enum PixelMode : int { kFused = 0, kDirSource = 1, kDirFinal = 2, kCarrier = 3, kEnlargerFinal = 4 };
 
 // Post stage: unsharp mask on the three linear planes, output encoding,
 // alpha copied from the source (JuicerCudaScanPipeline.cu:2733-2878).

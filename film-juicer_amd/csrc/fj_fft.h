@@ -25,17 +25,29 @@ struct DctRowsArgs {
     const float2* half_tw = nullptr;  // exp(-i pi k / (2n)), k < n
     const float* in[3] = {};
     float* out[3] = {};
-    int rows = 0;
+    int rows = 0;           // rows [row_begin, rows) are transformed
+    int row_begin = 0;
     long long in_stride = 0;
     long long out_stride = 0;
+    // Padded-plane mode (pad >= 0) for a src_w x src_h plane:
+    //  forward rows read the reflect-101 extension: padded row y, element i
+    //    <- in[reflect101(y - pad, src_h) * in_stride + reflect101(i - pad, src_w)];
+    //  inverse rows keep only the plane's own samples and blend them with it:
+    //    out[(y - pad) * out_stride + i - pad] = (1 - fraction) * mix[same] + fraction * value.
+    int pad = -1;
+    int src_w = 0, src_h = 0;
+    const float* mix[3] = {};
+    float fraction = 0.0f;
 };
 
-// Separable-sum transfer function T(row, k) = sum_c weight[c] * t_rows[c][row] * t_cols[c][k].
+// Separable-sum transfer function T(row, k) = sum_c weight[c] * t_rows[c][row] * t_cols[c][k],
+// or, for a plane whose full[plane] is set, the table T(row, k) = full[plane][row * n + k].
 struct DctTransfer {
     int count = 0;
     float weight[kMaxDctComponents] = {};
     const float* t_rows[kMaxDctComponents] = {};
     const float* t_cols[kMaxDctComponents] = {};
+    const float* full[3] = {};
 };
 
 struct TransposeArgs {

@@ -133,6 +133,15 @@ __device__ float2* fft_lds(float2* a, float2* b, const FftPlanDev& plan, const f
     return src;
 }
 
+// Whole-sample mirror with period 2 (n - 1), n >= 2 (reflect-101).
+__device__ __forceinline__ int reflect101(int i, int n) {
+    if (i >= 0 && i < n) return i;
+    const int period = 2 * (n - 1);
+    int v = i % period;
+    if (v < 0) v += period;
+    return v < n ? v : period - v;
+}
+
 // Makhoul permutation index: position of x[i] in v.
 __device__ __forceinline__ int makhoul_pos(int i, int n) { return (i & 1) ? n - 1 - (i >> 1) : (i >> 1); }
 
@@ -173,6 +182,18 @@ __global__ __launch_bounds__(256) void dct2_rows_kernel(DctRowsArgs args) {
     const bool has1 = row0 + 1 < args.rows;
     const float* in = args.in[plane];
     float* out = args.out[plane];
+    if (args.pad >= 0) {
+        const float* r0 = in + static_cast<long long>(reflect101(row0 - args.pad, args.src_h)) * args.in_stride;
+        const float* r1 = in + static_cast<long long>(reflect101(row0 + 1 - args.pad, args.src_h)) * args.in_stride;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const int si = reflect101(i - args.pad, args.src_w);
+            a[makhoul_pos(i, n)] = make_float2(r0[si], has1 ? r1[si] : 0.0f);
+        }
+        __syncthreads();
+        dct2_pair(a, b, plan, args.table, args.half_tw, out + static_cast<long long>(row0) * args.out_stride,
+                  has1 ? out + static_cast<long long>(row0 + 1) * args.out_stride : nullptr, 1.0f, 1.0f);
+        return;
+    }
     const float* r0 = in + static_cast<long long>(row0) * args.in_stride;
     const float* r1 = has1 ? in + static_cast<long long>(row0 + 1) * args.in_stride : nullptr;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -194,7 +215,7 @@ __global__ __launch_bounds__(256) void dct3_rows_kernel(DctRowsArgs args) {
     const int n = plan.n;
     float2* a = lds2;
     float2* b = lds2 + n;
-    const int row0 = 2 * blockIdx.x;
+    const int row0 = args.row_begin + 2 * blockIdx.x;
     const int plane = blockIdx.y;
     if (row0 >= args.rows) return;
     const bool has1 = row0 + 1 < args.rows;
@@ -218,6 +239,20 @@ __global__ __launch_bounds__(256) void dct3_rows_kernel(DctRowsArgs args) {
     __syncthreads();
     float2* res = fft_lds(a, b, plan, args.table);
     const float inv = 1.0f / static_cast<float>(n);
+    if (args.pad >= 0) {
+        // Rows [pad, pad + src_h) of the padded plane hold the plane's rows.
+        const int pad = args.pad;
+        const float f = args.fraction, keep = 1.0f - args.fraction;
+        const long long y0 = row0 - pad;
+        const float* m = args.mix[plane];
+        for (int i = threadIdx.x; i < args.src_w; i += blockDim.x) {
+            const float2 z = res[makhoul_pos(i + pad, n)];
+            const long long o = y0 * args.out_stride + i;
+            out[o] = keep * m[o] + f * (z.x * inv);
+            if (has1) out[o + args.out_stride] = keep * m[o + args.out_stride] + f * (-z.y * inv);
+        }
+        return;
+    }
     float* o0 = out + static_cast<long long>(row0) * args.out_stride;
     float* o1 = has1 ? out + static_cast<long long>(row0 + 1) * args.out_stride : nullptr;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -249,7 +284,9 @@ __global__ __launch_bounds__(256) void dct_filter_rows_kernel(DctRowsArgs args, 
         a[makhoul_pos(i, n)] = make_float2(r0[i], has1 ? r1[i] : 0.0f);
     }
     __syncthreads();
-    // Per-row component factors for rows row0 and row0 + 1.
+    // Per-row component factors for rows row0 and row0 + 1, or the plane's
+    // full transfer table.
+    const float* full = tf.full[plane];
     float fa[kMaxDctComponents], fb[kMaxDctComponents];
     for (int c = 0; c < tf.count; ++c) {
         fa[c] = tf.weight[c] * tf.t_rows[c][row0];
@@ -265,10 +302,15 @@ __global__ __launch_bounds__(256) void dct_filter_rows_kernel(DctRowsArgs args, 
         const float2 vb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
         const float2 w = args.half_tw[k];
         float ta = 0.0f, tb = 0.0f;
-        for (int c = 0; c < tf.count; ++c) {
-            const float tk = tf.t_cols[c][k];
-            ta = fmaf(fa[c], tk, ta);
-            tb = fmaf(fb[c], tk, tb);
+        if (full) {
+            ta = full[static_cast<long long>(row0) * n + k];
+            tb = has1 ? full[static_cast<long long>(row0 + 1) * n + k] : 0.0f;
+        } else {
+            for (int c = 0; c < tf.count; ++c) {
+                const float tk = tf.t_cols[c][k];
+                ta = fmaf(fa[c], tk, ta);
+                tb = fmaf(fb[c], tk, tb);
+            }
         }
         spec[k] = (w.x * va.x - w.y * va.y) * ta;
         spec[n + k] = (w.x * vb.x - w.y * vb.y) * tb;
@@ -328,7 +370,8 @@ hipError_t launch_dct2_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
 }
 
 hipError_t launch_dct3_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
-    const dim3 grid((a.rows + 1) / 2, planes);
+    if (a.rows <= a.row_begin) return hipSuccess;
+    const dim3 grid((a.rows - a.row_begin + 1) / 2, planes);
     hipLaunchKernelGGL(dct3_rows_kernel, grid, dim3(256), dct_lds_bytes(a.plan.n, false), s, a);
     return hipGetLastError();
 }

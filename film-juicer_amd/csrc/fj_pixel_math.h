@@ -345,7 +345,7 @@ __device__ __forceinline__ unsigned int develop_dir_final(const DevelopParams& d
 // 81-λ print exposure over the compacted active wavelengths, then the print
 // curves (JuicerCudaDeviceHelpers.cuh:866-1071). A NaN transmitted sample
 // contributes zero, as in the reference.
-__device__ __forceinline__ void print_raw(const PrintParams& pr, const float4* spec, const float D[3], float raw[3]) {
+__device__ __forceinline__ void print_inner(const PrintParams& pr, const float4* spec, const float D[3], float inner[3]) {
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
     for (int i = 0; i < pr.k; ++i) {
         const float4 e = spec[2 * i];
@@ -357,19 +357,41 @@ __device__ __forceinline__ void print_raw(const PrintParams& pr, const float4* s
         a1 = fmaf(t, s.y, a1);
         a2 = fmaf(t, s.z, a2);
     }
-    raw[0] = fmaf(a0, pr.normalizer, pr.preflash[0]) * pr.outer_scale;
-    raw[1] = fmaf(a1, pr.normalizer, pr.preflash[1]) * pr.outer_scale;
-    raw[2] = fmaf(a2, pr.normalizer, pr.preflash[2]) * pr.outer_scale;
+    inner[0] = fmaf(a0, pr.normalizer, pr.preflash[0]);
+    inner[1] = fmaf(a1, pr.normalizer, pr.preflash[1]);
+    inner[2] = fmaf(a2, pr.normalizer, pr.preflash[2]);
 }
 
 __device__ __forceinline__ void print_develop(const PrintParams& pr, const float4* spec, const float* curves,
                                               float D[3]) {
-    float raw[3];
-    print_raw(pr, spec, D, raw);
+    float inner[3];
+    print_inner(pr, spec, D, inner);
     const int n = pr.n;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const float l = log10_fast(raw[c] + 1e-10f);
+        const float l = log10_fast(inner[c] * pr.outer_scale + 1e-10f);
+        D[c] = sample_logE(curves, curves + (1 + c) * n, n, l, 1.0f);
+    }
+}
+
+// Enlarger-linear print exposure, the quantity the enlarger diffusion blurs
+// (print_linear_for_diffusion_device, JuicerCudaDeviceHelpers.cuh:997-1010).
+__device__ __forceinline__ void print_linear(const PrintParams& pr, const float4* spec, const float D[3], float lin[3]) {
+    float inner[3];
+    print_inner(pr, spec, D, inner);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) lin[c] = (fmaxf(inner[c], 0.0f) + 1e-10f) * pr.outer_scale;
+}
+
+// Print development from the diffused print-linear exposure
+// (develop_print_density_from_enlarger_linear_kernel,
+// JuicerCudaScanPipeline.cu:1457-1499).
+__device__ __forceinline__ void print_develop_linear(const PrintParams& pr, const float* curves, const float lin[3],
+                                                     float D[3]) {
+    const int n = pr.n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float l = log10_fast(fmaxf(lin[c], 0.0f) + 1e-10f);
         D[c] = sample_logE(curves, curves + (1 + c) * n, n, l, 1.0f);
     }
 }

@@ -1,6 +1,7 @@
 // fj_plan.cpp — validation of one render call, packing of prepared tables into
 // device layouts, and the frame schedule on the caller's stream.
 #include "fj_plan.h"
+#include "fj_diffusion_host.h"
 #include "fj_halation_host.h"
 
 #include <algorithm>
@@ -100,15 +101,18 @@ hipError_t DeviceState::table(uint64_t key, const void* host, size_t bytes, cons
     if (it != tables.end()) {
         lru.remove(key);
         lru.push_front(key);
+        it->second.serial = frame_serial;
         *out = it->second.ptr;
         return hipSuccess;
     }
-    constexpr size_t kBudget = size_t(1) << 30;  // device bytes kept for cached tables
+    constexpr size_t kBudget = size_t(2) << 30;  // device bytes kept for cached tables
     constexpr size_t kMaxEntries = 256;
     while (!lru.empty() && (table_bytes + bytes > kBudget || tables.size() >= kMaxEntries)) {
         const uint64_t victim = lru.back();
-        lru.pop_back();
         auto v = tables.find(victim);
+        // Tables the current frame already uses stay: the budget is exceeded instead.
+        if (v != tables.end() && v->second.serial == frame_serial) break;
+        lru.pop_back();
         if (v != tables.end()) {
             // hipFree synchronizes the device, so no in-flight frame still reads it.
             const hipError_t e = hipFree(v->second.ptr);
@@ -120,10 +124,12 @@ hipError_t DeviceState::table(uint64_t key, const void* host, size_t bytes, cons
     }
     DeviceTable t;
     t.bytes = bytes;
+    t.serial = frame_serial;
     hipError_t e = hipMalloc(&t.ptr, std::max<size_t>(bytes, 16));
     if (e != hipSuccess) return e;
-    // Synchronous copy: the prepared spans expire when the call returns.
-    e = hipMemcpy(t.ptr, host, bytes, hipMemcpyHostToDevice);
+    // Synchronous copy: the prepared spans expire when the call returns. A
+    // table without host bytes is filled by device work on the frame's stream.
+    if (host) e = hipMemcpy(t.ptr, host, bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(t.ptr);
         return e;
@@ -318,11 +324,14 @@ bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
     if (p.scanner_post.flags & FJ_POST_GLARE) return d.unsupported("scanner glare");
     if (p.scanner_post.hash != 0 && p.scanner_post.lens_blur_sigma_px > 0.0f && p.scanner_post.lens_blur_radius > 0)
         return d.unsupported("scanner lens blur");
-    if (p.optics.flags & (FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_HIGHLIGHT_BOOST))
-        return d.unsupported("camera/enlarger diffusion and highlight boost");
     if (p.optics.flags & ~(FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_SCATTER_HALATION |
                            FJ_OPTICS_HIGHLIGHT_BOOST))
         return d.malformed("optics.flags");
+    {
+        std::string field;
+        if (!diffusion_decode(p.optics, is_print_route(p.route), field)) return d.malformed(field.c_str());
+    }
+    if (p.optics.flags & FJ_OPTICS_HIGHLIGHT_BOOST) return d.unsupported("highlight boost");
     if (p.optics.flags & FJ_OPTICS_SCATTER_HALATION) {
         std::string field;
         if (!halation_validate(p.optics.scatter_halation, field)) return d.malformed(field.c_str());
@@ -358,6 +367,16 @@ bool check_spatial_dir(const FjPreparedHostData& p, const FjFrame& f, int max_li
         return d.malformed("spatial_dir.frame");
     const int w = f.render_window.x2 - f.render_window.x1, h = f.render_window.y2 - f.render_window.y1;
     if (w > max_line || h > max_line) return d.unsupported("spatial DIR frame side above the LDS line limit");
+    return true;
+}
+
+// Diffusion binding: the optics frame is the whole frame
+// (native/juicer_cuda_prepared.cpp:216-221).
+bool check_optics_frame(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
+    if (p.optics.diffusion_hash == 0 && p.optics.highlight_boost.recipe_hash == 0) return true;
+    if (!same_extent(p.optics.full_frame, f.source.bounds) || !same_extent(p.optics.full_frame, f.render_window) ||
+        !same_extent(p.optics.full_frame, f.full_frame_extent))
+        return d.malformed("optics.frame");
     return true;
 }
 
@@ -739,8 +758,10 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     Diag d;
     if (!validate_prepared(prepared, d) || !validate_binding(prepared, frame, submission, d))
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
-    if (!validate_frame(frame, d) || !check_supported(prepared, frame, d) || !check_spatial_dir(prepared, frame, dev.max_line, d))
+    if (!validate_frame(frame, d) || !check_supported(prepared, frame, d) ||
+        !check_spatial_dir(prepared, frame, dev.max_line, d) || !check_optics_frame(prepared, frame, d))
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
+    ++dev.frame_serial;
     bool invalid = false;
     if (abort_requested(abort_callback, invalid))
         return outcome_diag(invalid ? FJ_STATUS_INTERNAL_FAILURE : FJ_STATUS_CANCELLED,
@@ -797,12 +818,33 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     const bool spatial_dir = prepared.spatial_dir.hash != 0;
     const bool post = post_active(prepared.scanner_post);
     const bool halation = (prepared.optics.flags & FJ_OPTICS_SCATTER_HALATION) != 0;
+    const bool camera_diffusion = (prepared.optics.flags & FJ_OPTICS_CAMERA_DIFFUSION) != 0;
+    const bool enlarger_diffusion = pp.print && (prepared.optics.flags & FJ_OPTICS_ENLARGER_DIFFUSION) != 0;
+    const bool carrier = halation || camera_diffusion;
     const long long plane = static_cast<long long>(pp.io.width) * pp.io.height;
     if (halation && pp.io.width > dev.halation_max_line)
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput halation row above the LDS line limit", error);
 
-    // Workspace planes: [DIR 12][carrier 3][halation rows 3K]; the unsharp
-    // input planes reuse DIR scratch or the halation rows once those are done.
+    // Diffusion stages: components (host behaviour) and padded sizes.
+    const FjDiffusion* diff_desc[2] = {camera_diffusion ? &prepared.optics.camera : nullptr,
+                                       enlarger_diffusion ? &prepared.optics.enlarger : nullptr};
+    long long padded_floats = 0;
+    for (const FjDiffusion* dd : diff_desc) {
+        if (!dd) continue;
+        double lam[kDiffusionComponents], wts[3][kDiffusionComponents];
+        std::string field;
+        if (!diffusion_components(*dd, lam, wts, field))
+            return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "InvalidDiffusionHostBehavior field=" + field, error);
+        if (pp.io.width < 2 || pp.io.height < 2)
+            return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput diffusion frame side below 2", error);
+        int pw = 0, ph = 0;
+        diffusion_padded(*dd, pp.io.width, pp.io.height, pw, ph);
+        if (pw > dev.max_line || ph > dev.max_line)
+            return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT,
+                                "UnsupportedInput diffusion padded frame side above the LDS line limit", error);
+        padded_floats = std::max({padded_floats, 6LL * pw * ph, static_cast<long long>(diffusion_build_floats(*dd, pw, ph))});
+    }
+
     HalationStages hs;
     int hal_k = 0;
     if (halation) {
@@ -811,85 +853,113 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
                                 "UnsupportedInput halation recursion without a stable factored form", error);
         for (int c = 0; c < 3; ++c) hal_k = std::max({hal_k, hs.scatter[c].count, hs.bounce[c].count});
     }
-    const int dir_off = 0;
-    const int carrier_off = spatial_dir ? 12 : 0;
-    const int q_off = carrier_off + (halation ? 6 : 0);  // carrier + its ping-pong copy
-    int planes_needed = q_off + (halation ? 3 * hal_k : 0);
-    const int post_off = spatial_dir ? 9 : (halation && hal_k > 0 ? q_off : planes_needed);
-    if (post) planes_needed = std::max(planes_needed, post_off + 3);
+    // Workspace (floats): [DIR 12 planes][carrier 3 + ping-pong 3][halation
+    // rows 3K][print-linear 3][post 3, unless it reuses DIR or halation
+    // scratch][diffusion padded planes / transfer build scratch].
+    long long ws_floats = 0;
+    auto take = [&](long long floats) {
+        const long long o = ws_floats;
+        ws_floats += (floats + 63) / 64 * 64;
+        return o;
+    };
+    const long long dir_off = spatial_dir ? take(12 * plane) : -1;
+    const long long carrier_off = carrier ? take(6 * plane) : -1;
+    const long long q_off = (halation && hal_k > 0) ? take(3LL * hal_k * plane) : -1;
+    const long long plin_off = enlarger_diffusion ? take(3 * plane) : -1;
+    long long post_off = -1;
+    if (post) post_off = spatial_dir ? dir_off + 9 * plane : (q_off >= 0 ? q_off : take(3 * plane));
+    const long long pad_off = padded_floats ? take(padded_floats) : -1;
     float* ws = nullptr;
-    if (planes_needed) {
-        e = dev.ensure_workspace(static_cast<size_t>(plane) * planes_needed * sizeof(float));
+    if (ws_floats) {
+        e = dev.ensure_workspace(static_cast<size_t>(ws_floats) * sizeof(float));
         if (e != hipSuccess) return outcome_hip(e, "workspace", error);
         ws = static_cast<float*>(dev.workspace);
     }
     if (post) {
         pp.linear_out = 1;
-        pp.linear_planes = ws + post_off * plane;
+        pp.linear_planes = ws + post_off;
     }
-    if (halation) {
-        for (int c = 0; c < 3; ++c) pp.carrier[c] = ws + (carrier_off + c) * plane;
+    // Transfer tables first (built once per PSF and padded size).
+    DiffusionStage diff_stage[2];
+    for (int k = 0; k < 2; ++k) {
+        if (!diff_desc[k]) continue;
+        e = diffusion_prepare(dev, *diff_desc[k], pp.io.width, pp.io.height, ws + pad_off, stream, diff_stage[k]);
+        if (e == hipErrorInvalidValue)
+            return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "InvalidDiffusionHostBehavior field=component_output", error);
+        if (e != hipSuccess) return outcome_hip(e, "diffusion transfer", error);
+    }
+    if (carrier) {
+        for (int c = 0; c < 3; ++c) pp.carrier[c] = ws + carrier_off + c * plane;
         mark("camera_carrier");
         e = launch_pixel_stage(pp, kCarrier, grid_limit, stream);
         if (e != hipSuccess) return outcome_hip(e, "camera carrier stage", error);
-        HalArgs ha{};
         // Descriptor channels are R, G, B; carrier planes are B/G/R-sensitive.
         float* cur[3];
         float* alt[3];
         for (int c = 0; c < 3; ++c) {
             cur[c] = pp.carrier[2 - c];
-            alt[c] = ws + (carrier_off + 3 + 2 - c) * plane;
-            for (int k = 0; k < hal_k; ++k) ha.q[c][k] = ws + (q_off + c * hal_k + k) * plane;
+            alt[c] = ws + carrier_off + (3 + 2 - c) * plane;
         }
-        auto fir_only = [](const HalStageDev* st) {
+        if (camera_diffusion) {
+            mark("camera_diffusion");
+            e = run_diffusion(dev, diff_stage[0], cur, pp.io.width, pp.io.height, ws + pad_off, stream);
+            if (e != hipSuccess) return outcome_hip(e, "camera diffusion stage", error);
+        }
+        if (halation) {
+            HalArgs ha{};
             for (int c = 0; c < 3; ++c)
-                for (int k = 0; k < st[c].count; ++k)
-                    if (st[c].g[k].kind > kHalFir) return false;
-            return true;
-        };
-        // FIR-only stages run tiled from one buffer into the other; the
-        // recursive stages run in place.
-        auto run_stage = [&](const HalStageDev* host, const HalStageDev* device) -> hipError_t {
-            ha.stage = device;
-            if (fir_only(host)) {
+                for (int k = 0; k < hal_k; ++k) ha.q[c][k] = ws + q_off + (c * hal_k + k) * plane;
+            auto fir_only = [](const HalStageDev* st) {
+                for (int c = 0; c < 3; ++c)
+                    for (int k = 0; k < st[c].count; ++k)
+                        if (st[c].g[k].kind > kHalFir) return false;
+                return true;
+            };
+            // FIR-only stages run tiled from one buffer into the other; the
+            // recursive stages run in place.
+            auto run_stage = [&](const HalStageDev* host, const HalStageDev* device) -> hipError_t {
+                ha.stage = device;
+                if (fir_only(host)) {
+                    for (int c = 0; c < 3; ++c) {
+                        ha.in[c] = cur[c];
+                        ha.out[c] = alt[c];
+                    }
+                    hipError_t r = launch_halation_fir_stage(ha, stream);
+                    for (int c = 0; c < 3; ++c) std::swap(cur[c], alt[c]);
+                    return r;
+                }
                 for (int c = 0; c < 3; ++c) {
                     ha.in[c] = cur[c];
-                    ha.out[c] = alt[c];
+                    ha.out[c] = cur[c];
                 }
-                hipError_t r = launch_halation_fir_stage(ha, stream);
-                for (int c = 0; c < 3; ++c) std::swap(cur[c], alt[c]);
-                return r;
+                return launch_halation_stage(ha, stream);
+            };
+            ha.width = pp.io.width;
+            ha.height = pp.io.height;
+            ha.row_len = hs.row_len;
+            ha.col_len = hs.col_len;
+            const void* stage_ptr = nullptr;
+            const uint64_t hkey = hash_bytes(&hs, sizeof(hs), 0x4841ull);
+            e = dev.table(hkey, &hs, sizeof(hs), &stage_ptr);
+            if (e != hipSuccess) return outcome_hip(e, "halation tables", error);
+            const char* dstages = static_cast<const char*>(stage_ptr);
+            if (hs.scatter_active) {
+                mark("halation_scatter");
+                e = run_stage(hs.scatter, reinterpret_cast<const HalStageDev*>(dstages + offsetof(HalationStages, scatter)));
+                if (e != hipSuccess) return outcome_hip(e, "halation scatter stage", error);
             }
-            for (int c = 0; c < 3; ++c) {
-                ha.in[c] = cur[c];
-                ha.out[c] = cur[c];
+            if (hs.bounce_active) {
+                mark("halation_bounce");
+                e = run_stage(hs.bounce, reinterpret_cast<const HalStageDev*>(dstages + offsetof(HalationStages, bounce)));
+                if (e != hipSuccess) return outcome_hip(e, "halation back-reflection stage", error);
             }
-            return launch_halation_stage(ha, stream);
-        };
-        ha.width = pp.io.width;
-        ha.height = pp.io.height;
-        ha.row_len = hs.row_len;
-        ha.col_len = hs.col_len;
-        const void* stage_ptr = nullptr;
-        const uint64_t hkey = hash_bytes(&hs, sizeof(hs), 0x4841ull);
-        e = dev.table(hkey, &hs, sizeof(hs), &stage_ptr);
-        if (e != hipSuccess) return outcome_hip(e, "halation tables", error);
-        const char* dstages = static_cast<const char*>(stage_ptr);
-        if (hs.scatter_active) {
-            mark("halation_scatter");
-            e = run_stage(hs.scatter, reinterpret_cast<const HalStageDev*>(dstages + offsetof(HalationStages, scatter)));
-            if (e != hipSuccess) return outcome_hip(e, "halation scatter stage", error);
-        }
-        if (hs.bounce_active) {
-            mark("halation_bounce");
-            e = run_stage(hs.bounce, reinterpret_cast<const HalStageDev*>(dstages + offsetof(HalationStages, bounce)));
-            if (e != hipSuccess) return outcome_hip(e, "halation back-reflection stage", error);
         }
         for (int c = 0; c < 3; ++c) pp.carrier[2 - c] = cur[c];
     }
-    (void)dir_off;
+    if (enlarger_diffusion)
+        for (int c = 0; c < 3; ++c) pp.plin[c] = ws + plin_off + c * plane;
     if (spatial_dir) {
-        for (int c = 0; c < 6; ++c) pp.dir_planes[c] = ws + c * plane;
+        for (int c = 0; c < 6; ++c) pp.dir_planes[c] = ws + dir_off + c * plane;
         mark("dir_source");
         e = launch_pixel_stage(pp, kDirSource, grid_limit, stream);
         if (e != hipSuccess) return outcome_hip(e, "DIR source stage", error);
@@ -916,8 +986,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         for (int c = 0; c < 3; ++c) {
             job.in[c] = pp.dir_planes[c];
             job.out[c] = pp.dir_planes[c];
-            job.tmp_a[c] = ws + (6 + c) * plane;
-            job.tmp_b[c] = ws + (9 + c) * plane;
+            job.tmp_a[c] = ws + dir_off + (6 + c) * plane;
+            job.tmp_b[c] = ws + dir_off + (9 + c) * plane;
         }
         mark("dir_dct_filter");
         e = run_dct_filter(dev, job, stream);
@@ -929,6 +999,16 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         mark("pixel_chain");
         e = launch_pixel_stage(pp, kFused, grid_limit, stream);
         if (e != hipSuccess) return outcome_hip(e, "fused pixel kernel", error);
+    }
+    if (enlarger_diffusion) {
+        // develop_print_medium (JuicerCudaExecutor.cpp:2282-2382): enlarger
+        // diffusion of the print-linear exposure, then the print and scan.
+        mark("enlarger_diffusion");
+        e = run_diffusion(dev, diff_stage[1], pp.plin, pp.io.width, pp.io.height, ws + pad_off, stream);
+        if (e != hipSuccess) return outcome_hip(e, "enlarger diffusion stage", error);
+        mark("print_scan");
+        e = launch_pixel_stage(pp, kEnlargerFinal, grid_limit, stream);
+        if (e != hipSuccess) return outcome_hip(e, "enlarger print stage", error);
     }
     if (post) {
         PostParams po{};

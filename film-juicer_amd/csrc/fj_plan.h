@@ -19,6 +19,7 @@ namespace fjx {
 struct DeviceTable {
     void* ptr = nullptr;
     size_t bytes = 0;
+    uint64_t serial = 0;  // last frame that used it
 };
 
 struct DeviceState {
@@ -31,6 +32,7 @@ struct DeviceState {
     // Device transfer functions of spatial filters, by descriptor and frame size.
     std::unordered_map<uint64_t, DctTransfer> transfers;
     size_t table_bytes = 0;
+    uint64_t frame_serial = 1;  // advanced per rendered frame
 
     // Error word ring: each frame clears one device word, its kernels OR scan
     // and DIR failure bits into it, and the word is copied to pinned memory.
@@ -53,7 +55,9 @@ struct DeviceState {
 
     hipError_t init(int dev);
     hipError_t release();
-    // Returns the device copy of `bytes` host bytes under `key`, uploading once.
+    // Returns the device copy of `bytes` host bytes under `key`, uploading once
+    // (host == nullptr: uninitialized device bytes the caller fills). Tables
+    // used by the current frame are never evicted during it.
     hipError_t table(uint64_t key, const void* host, size_t bytes, const void** out);
     hipError_t ensure_workspace(size_t bytes);
     // Collects completed error words of earlier frames; returns OR of their bits.

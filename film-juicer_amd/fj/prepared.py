@@ -246,3 +246,32 @@ def with_halation(p: Prepared, width: int, height: int, pixel_size_um: float, *,
     o.full_frame = abi.FjExtent(0, 0, width, height)
     o.scatter_halation = d
     return p
+
+
+def with_diffusion(p: Prepared, width: int, height: int, pixel_size_um: float, *, camera: dict | None = None,
+                   enlarger: dict | None = None) -> Prepared:
+    """Turns on camera and/or enlarger diffusion for a full width x height
+    frame. `camera` / `enlarger` are resolved controls from
+    descriptors.diffusion_recipe (None: stage off); the stage descriptors and
+    the frame-set identity follow optics/diffusion_frame.rs:465-566."""
+    from . import descriptors
+    o = p.record.optics
+    o.flags &= ~(abi.FJ_OPTICS_CAMERA_DIFFUSION | abi.FJ_OPTICS_ENLARGER_DIFFUSION)
+    o.camera = abi.FjDiffusion()
+    o.enlarger = abi.FjDiffusion()
+    o.diffusion_hash = 0
+    stages = [None, None]
+    for k, (rec, flag) in enumerate(((camera, abi.FJ_OPTICS_CAMERA_DIFFUSION), (enlarger, abi.FJ_OPTICS_ENLARGER_DIFFUSION))):
+        if rec is None:
+            continue
+        stages[k] = descriptors.diffusion_stage(rec, k, pixel_size_um, width, height)
+        o.flags |= flag
+    if stages[0] is not None:
+        o.camera = stages[0]
+    if stages[1] is not None:
+        o.enlarger = stages[1]
+    if stages[0] is not None or stages[1] is not None:
+        o.full_frame = abi.FjExtent(0, 0, width, height)
+        o.diffusion_hash = descriptors.diffusion_frame_set_hash(int(p.record.route), (0, 0, width, height),
+                                                                stages[0], stages[1])
+    return p
