@@ -4,8 +4,9 @@ Workload: UHD 3840x2160 RGBA float32 frame from the seeded synthetic
 generator, negative -> enlarger (filtered, preflash 0.01) -> paper print
 route, Portra 400 capture on Portra Endura paper, scatter/halation on the
 camera carrier (still-film defaults: 65 um first bounce, 3 bounces,
-renormalized), spatial DIR couplers (20 um core, 200 um tail, weight 0.03),
-scanner LUT res 17, OkLch gamut compression,
+renormalized), camera and enlarger diffusion (Black Pro-Mist 1/2: PSF radius
+834 px at this pitch), spatial DIR couplers (20 um core, 200 um tail, weight
+0.03), scanner LUT res 17, OkLch gamut compression,
 scanner unsharp (0.7 px, 0.7), sRGB output encoding. One step = one
 fj_cuda_render call over the whole frame on the caller's HIP stream, inputs
 resident in HBM. Per-stage times come from HIP events the library records on
@@ -112,6 +113,8 @@ def main():
     pixel_um = descriptors.pixel_size_for(WIDTH, HEIGHT)
     descriptors.spatial_dir(p.record, 20.0, 200.0, 0.03, pixel_um, WIDTH, HEIGHT)
     prepared.with_halation(p, WIDTH, HEIGHT, pixel_um)
+    mist = descriptors.diffusion_recipe(1, 0.5)
+    prepared.with_diffusion(p, WIDTH, HEIGHT, pixel_um, camera=mist, enlarger=mist)
     src_np = synth.frame(WIDTH, HEIGHT, seed=synth.DEFAULT_SEED)
     src = torch.from_numpy(src_np).to(dev)
     dst = torch.empty_like(src)
@@ -155,6 +158,7 @@ def main():
     chain_ms = stage_ms["pixel_chain"]
     dct_ms = stage_ms.get("dir_dct_filter")
     hal_ms = stage_ms.get("halation_scatter", 0.0) + stage_ms.get("halation_bounce", 0.0)
+    diff_ms = stage_ms.get("camera_diffusion", 0.0) + stage_ms.get("enlarger_diffusion", 0.0)
 
     k = active_wavelengths(p.record)
     fpx = flop_census(k, True)
@@ -165,6 +169,8 @@ def main():
     # Halation: each of the two linear stages reads and writes the three
     # carrier planes once (8 B/px per channel and stage).
     hal_bytes = pixels * 3 * 8 * 2
+    # Diffusion: each stage reads and writes its three planes once (8 B/px per channel and stage).
+    diff_bytes = pixels * 3 * 8 * 2
     result = {
         "metric": "UHD frames/s (print route, <=2e-4 err) plain vs batched; fwd+bwd ms/frame",
         "value": fps,
@@ -182,8 +188,9 @@ def main():
             "workload": "plain UHD print route, 3840x2160 RGBA f32: Portra 400 (Mallett) -> filtered "
                         "enlarger + preflash 0.01 -> Portra Endura, spatial DIR 20 um core + 200 um tail "
                         "weight 0.03 (DCT domain), scatter/halation (65 um, 3 bounces, renormalized) on the "
-                        "camera carrier, LUT scanner res 17, OkLch gamut, unsharp 0.7/0.7, sRGB encode; grain "
-                        "and camera/enlarger diffusion are not yet in this step",
+                        "camera carrier, camera + enlarger diffusion (Black Pro-Mist 1/2, PSF radius 834 px, "
+                        "DCT domain), LUT scanner res 17, OkLch gamut, unsharp 0.7/0.7, sRGB encode; grain is "
+                        "not yet in this step",
             "stage_ms": stage_ms,
             "frame": [WIDTH, HEIGHT],
             "profile_pair": ["kodak_portra_400", "kodak_portra_endura"],
@@ -219,6 +226,15 @@ def main():
             "unit": "GB/s",
             "frac": hal_bytes / (hal_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
             "kernel_ms": hal_ms,
+        },
+        "roofline_diffusion": None if not diff_ms else {
+            "bound": "hbm",
+            "kernel": "camera + enlarger diffusion (padded-plane DCT pipeline, 5 launches per stage, 3 planes)",
+            "achieved": diff_bytes / (diff_ms * 1e-3) / 1e9,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": diff_bytes / (diff_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "kernel_ms": diff_ms,
         },
         "frame_event_ms": ms_event,
     }

@@ -11,6 +11,7 @@ namespace fjx {
 constexpr int kMaxFftPasses = 24;
 constexpr int kMaxDctComponents = 8;
 constexpr int kMaxLdsFft = 8192;  // 2 x 8192 float2 = 128 KiB of the 160 KiB LDS
+constexpr int kMaxRowThreads = 1024;
 
 struct FftPlanDev {
     int n = 0;

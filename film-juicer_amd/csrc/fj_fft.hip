@@ -28,10 +28,9 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 // multiply by -i (forward) : (x, y) -> (y, -x)
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
-// Twiddle w_N^e = exp(-2*pi*i*e/N), e taken modulo N.
-__device__ __forceinline__ float2 tw(const float2* table, int n, long long e) {
-    return table[static_cast<int>(e % n)];
-}
+// Twiddle w_N^e = exp(-2*pi*i*e/N) for 0 <= e < N. Every caller's exponent
+// is below N: r * k * (N / (ns R)) < R ns N / (ns R) = N for r < R, k < ns.
+__device__ __forceinline__ float2 tw(const float2* table, int e) { return table[e]; }
 
 // One Stockham pass of radix R over the whole line: reads src, writes dst.
 // Ns = product of radices already applied. Forward transform (e^{-i}).
@@ -46,7 +45,7 @@ __device__ __forceinline__ void pass_fixed(const float2* src, float2* dst, int n
         for (int r = 0; r < R; ++r) a[r] = src[j + r * count];
         if (ns > 1) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) a[r] = cmul(a[r], tw(table, n, static_cast<long long>(r) * k * stride_tw));
+            for (int r = 1; r < R; ++r) a[r] = cmul(a[r], tw(table, r * k * stride_tw));
         }
         float2 b[R];
         if constexpr (R == 2) {
@@ -102,9 +101,8 @@ __device__ void pass_generic(const float2* src, float2* dst, int n, int ns, int 
         float2 acc = make_float2(0.0f, 0.0f);
         for (int r = 0; r < R; ++r) {
             float2 a = src[j + r * count];
-            if (ns > 1 && r > 0) a = cmul(a, tw(table, n, static_cast<long long>(r) * k * stride_tw));
-            const long long e = static_cast<long long>((r * q) % R) * root;
-            acc = cadd(acc, cmul(a, tw(table, n, e)));
+            if (ns > 1 && r > 0) a = cmul(a, tw(table, r * k * stride_tw));
+            acc = cadd(acc, cmul(a, tw(table, ((r * q) % R) * root)));
         }
         dst[(j / ns) * ns * R + k + q * ns] = acc;
     }
@@ -170,7 +168,7 @@ __device__ void dct2_pair(float2* a, float2* b, const FftPlanDev& plan, const fl
 // ---------------------------------------------------------------------------
 
 // Rows [2*blockIdx.x, 2*blockIdx.x+1] of each plane (blockIdx.y) -> DCT-II rows.
-__global__ __launch_bounds__(256) void dct2_rows_kernel(DctRowsArgs args) {
+__global__ __launch_bounds__(kMaxRowThreads) void dct2_rows_kernel(DctRowsArgs args) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
     const int n = plan.n;
@@ -209,7 +207,7 @@ __global__ __launch_bounds__(256) void dct2_rows_kernel(DctRowsArgs args) {
 // Inverse (DCT-III scaled so that it undoes dct2) of row pairs, optionally
 // multiplied by a separable-sum transfer function first:
 //   T(k, row) = sum_c w_c * tk[c][k] * trow[c][row]
-__global__ __launch_bounds__(256) void dct3_rows_kernel(DctRowsArgs args) {
+__global__ __launch_bounds__(kMaxRowThreads) void dct3_rows_kernel(DctRowsArgs args) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
     const int n = plan.n;
@@ -266,7 +264,7 @@ __global__ __launch_bounds__(256) void dct3_rows_kernel(DctRowsArgs args) {
 // Forward DCT-II along rows, multiply by T, inverse along rows, in one pass.
 // Used on the transposed plane (rows = original columns): the row index is the
 // first-axis frequency kx, the in-row index the second-axis frequency.
-__global__ __launch_bounds__(256) void dct_filter_rows_kernel(DctRowsArgs args, DctTransfer tf) {
+__global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRowsArgs args, DctTransfer tf) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
     const int n = plan.n;
@@ -363,22 +361,28 @@ __global__ __launch_bounds__(256) void transpose_kernel(TransposeArgs t) {
 // ---------------------------------------------------------------------------
 size_t dct_lds_bytes(int n, bool) { return static_cast<size_t>(n) * sizeof(float2) * 2; }
 
+// Threads per line-pair block: about one radix-4 butterfly per thread, whole waves.
+static int row_threads(int n) {
+    const int t = (n / 4 + 63) / 64 * 64;
+    return t < 64 ? 64 : (t > kMaxRowThreads ? kMaxRowThreads : t);
+}
+
 hipError_t launch_dct2_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
     const dim3 grid((a.rows + 1) / 2, planes);
-    hipLaunchKernelGGL(dct2_rows_kernel, grid, dim3(256), dct_lds_bytes(a.plan.n, false), s, a);
+    hipLaunchKernelGGL(dct2_rows_kernel, grid, dim3(row_threads(a.plan.n)), dct_lds_bytes(a.plan.n, false), s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_dct3_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
     if (a.rows <= a.row_begin) return hipSuccess;
     const dim3 grid((a.rows - a.row_begin + 1) / 2, planes);
-    hipLaunchKernelGGL(dct3_rows_kernel, grid, dim3(256), dct_lds_bytes(a.plan.n, false), s, a);
+    hipLaunchKernelGGL(dct3_rows_kernel, grid, dim3(row_threads(a.plan.n)), dct_lds_bytes(a.plan.n, false), s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_dct_filter_rows(const DctRowsArgs& a, const DctTransfer& tf, int planes, hipStream_t s) {
     const dim3 grid((a.rows + 1) / 2, planes);
-    hipLaunchKernelGGL(dct_filter_rows_kernel, grid, dim3(256), dct_lds_bytes(a.plan.n, true), s, a, tf);
+    hipLaunchKernelGGL(dct_filter_rows_kernel, grid, dim3(row_threads(a.plan.n)), dct_lds_bytes(a.plan.n, true), s, a, tf);
     return hipGetLastError();
 }
 
