@@ -68,6 +68,17 @@ FJ_DIR_OPERATOR_FIR_REFLECT = 2
 FJ_DIR_OPERATOR_YVV_REFLECT = 3
 FJ_GRAIN_ACTIVE = 1
 FJ_GRAIN_SUBLAYERS = 2
+FJ_GRAIN_FULL_FRAME = 4
+FJ_GRAIN_AXIS_FINITE_C = 8
+FJ_GRAIN_AXIS_FINITE_M = 16
+FJ_GRAIN_AXIS_FINITE_Y = 32
+FJ_GRAIN_DEBUG_OFF = 0
+FJ_GRAIN_DEBUG_DELTA_MIX = 1
+FJ_GRAIN_DEBUG_DELTA_FINE = 2
+FJ_GRAIN_DEBUG_DELTA_COARSE = 3
+FJ_GRAIN_DEBUG_DELTA_FINE_RAW = 4
+FJ_GRAIN_DEBUG_DELTA_COARSE_RAW = 5
+FJ_GRAIN_DEBUG_MEAN_DENSITY = 6
 
 
 def _s(name, fields):

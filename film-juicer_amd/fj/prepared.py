@@ -275,3 +275,71 @@ def with_diffusion(p: Prepared, width: int, height: int, pixel_size_um: float, *
         o.diffusion_hash = descriptors.diffusion_frame_set_hash(int(p.record.route), (0, 0, width, height),
                                                                 stages[0], stages[1])
     return p
+
+
+def grain_params(case: str = "GammaAndGrainLayers", directory: str = DEFAULT_DIR) -> dict:
+    """Grain controls and the development's sub-layer curves of a prepared
+    case that carries a grain recipe (the plug-in defaults: Portra 400)."""
+    s, a = _load_case(case, directory)
+    keys = ("particle_area_um2", "particle_scale_cmy", "particle_scale_layers", "particle_density_min_cmy",
+            "uniformity_cmy", "correlation_sigma_px", "dye_cloud_blur_um", "micro_structure", "sublayer_count",
+            "amplitude", "chroma_mix", "chroma_shared_weight", "chroma_independent_weight", "coarse_weight",
+            "mid_weight", "size_mix_scale", "clump_temporal_mix", "clump_morph_period_sec", "debug_view")
+    params = {k: (np.asarray(a[f"grain.{k}"]) if f"grain.{k}" in a else s[f"grain.{k}"]) for k in keys}
+    params["sublayers_active"] = bool(s["grain.sublayers_active"])
+    layers = [np.asarray(a[f"film_development.density_layers[{i}]"], dtype=np.float32) for i in range(9)]
+    return dict(params=params, layers=layers, layers_hash=int(s["film_development.density_layers_hash"]),
+                authored_max_cmy=np.asarray(a["enlarger_film_bounds.authored_maximum_cmy"], dtype=np.float32))
+
+
+def with_grain(p: Prepared, width: int, height: int, pixel_size_um: float, *, session_seed: int = 7,
+               clip_token: int = 0, frame_time: float = 0.0, frame_rate: float = 24.0, window=None,
+               noise_shape=(128, 128, 64), sublayers: bool | None = None, **overrides) -> Prepared:
+    """Turns on visual grain: resolves the recipe (fj.grain.recipe) from the
+    plug-in default controls with `overrides`, composes the frame plan for a
+    width x height frame (render window `window`, default the whole frame),
+    attaches the development's sub-layer curves and synthetic static noise.
+    The frame passed to the render must carry the same pixel size, session
+    seed, clip token and time."""
+    from . import grain
+    g = grain_params()
+    params = dict(g["params"])
+    params.update(overrides)
+    if sublayers is not None:
+        params["sublayers_active"] = bool(sublayers)
+    r = p.record
+    fd = r.film_development
+    n = fd.log_exposure.count
+    curves = np.ctypeslib.as_array(fd.density_rgb.data, shape=(n * 3,)).reshape(n, 3).copy()
+    positive = r.capture_polarity == abi.FJ_POLARITY_POSITIVE
+    rec = grain.recipe(params, curves, positive, g["layers_hash"])
+    full = (0, 0, width, height)
+    render = tuple(window) if window else full
+    layer_curves = [[g["layers"][l * 3 + c] for c in range(3)] for l in range(3)]
+    plan = grain.frame_plan(rec, capture_polarity=int(r.capture_polarity), film_polarity=int(r.capture_polarity),
+                            layer_curves_required=True, pixel_size_um=pixel_size_um,
+                            authored_max_cmy=g["authored_max_cmy"], render_extent=render, full_frame_extent=full,
+                            frame_time=frame_time, frame_rate=frame_rate, session_seed=session_seed,
+                            clip_token=clip_token, film_layers_hash=g["layers_hash"], layer_curves=layer_curves)
+    r.grain = grain.visual_grain(plan, rec)
+    if rec["sublayers_active"]:
+        for l in range(3):
+            for c in range(3):
+                fd.density_layers[l][c] = p.fspan(g["layers"][l * 3 + c])
+        fd.density_layers_hash = g["layers_hash"]
+    else:
+        for l in range(3):
+            for c in range(3):
+                fd.density_layers[l][c] = abi.FjFloatSpan(None, 0)
+        fd.density_layers_hash = 0
+    stbn, tiles, lut = grain.synthetic_noise(noise_shape)
+    nz = abi.FjStaticNoise()
+    nz.stbn = p.bspan(stbn.ravel())
+    nz.stbn_width, nz.stbn_height, nz.stbn_frames = noise_shape
+    nz.wang_tiles = p.bspan(tiles.ravel())
+    nz.wang_lut = p.bspan(lut)
+    nz.wang_width = nz.wang_height = tiles.shape[1]
+    nz.wang_tile_count = tiles.shape[0]
+    nz.wang_colors = 2
+    r.noise = nz
+    return p

@@ -383,6 +383,8 @@ typedef struct Frame {
     double* plane[9]; /* scratch planes */
     double* carrier[3]; /* camera film linear exposure B/G/R, or NULL */
     double* plin[3];    /* enlarger print-linear exposure C/M/Y, or NULL */
+    float* dens[3];     /* film density C/M/Y planes for the grain stage, or NULL */
+    int dens_out;       /* the back end stores density instead of printing */
     double* linear;   /* [h][w][3] */
     uint32_t errors;
     pthread_mutex_t lock;
@@ -1305,6 +1307,12 @@ static void add_errors(Frame* f, uint32_t bits) {
 /* Print + scan + colour of one developed pixel into the linear planes. */
 static uint32_t px_back(const FjoPlan* pl, Frame* f, size_t idx, double D[3]) {
     double rgb[3];
+    if (f->dens_out) {
+        /* visual grain follows on the density planes (develop_print_capture,
+         * JuicerCudaExecutor.cpp:2198-2280) */
+        for (int c = 0; c < 3; ++c) f->dens[c][idx] = (float)D[c];
+        return 0;
+    }
     if (pl->print && f->plin[0]) {
         /* enlarger diffusion follows: keep the print-linear exposure */
         double lin[3];
@@ -1335,6 +1343,19 @@ static void rows_enlarged(Frame* f, int y0, int y1) {
             if (!ok) rgb[0] = rgb[1] = rgb[2] = 0.0;
             memcpy(f->linear + idx * 3, rgb, sizeof(rgb));
             if (!ok) errs |= 1u;
+        }
+    add_errors(f, errs);
+}
+
+/* Back end from the grained density planes. */
+static void rows_density(Frame* f, int y0, int y1) {
+    uint32_t errs = 0;
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < f->w; ++x) {
+            const size_t idx = (size_t)y * f->w + x;
+            double D[3];
+            for (int c = 0; c < 3; ++c) D[c] = (double)f->dens[c][idx];
+            errs |= px_back(f->plan, f, idx, D);
         }
     add_errors(f, errs);
 }
@@ -1475,7 +1496,7 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
                uint32_t* error_flags) {
     if (!plan || !src || width <= 0 || height <= 0 || (comps != 3 && comps != 4)) return 2;
     const FjPreparedHostData* p = &plan->p;
-    if (p->grain.flags & FJ_GRAIN_ACTIVE) return 3;
+    const int grain = (p->grain.flags & FJ_GRAIN_ACTIVE) != 0;
     const int halation = (p->optics.flags & FJ_OPTICS_SCATTER_HALATION) != 0;
     const int camera_diffusion = (p->optics.flags & FJ_OPTICS_CAMERA_DIFFUSION) != 0;
     const int enlarger_diffusion = plan->print && (p->optics.flags & FJ_OPTICS_ENLARGER_DIFFUSION) != 0;
@@ -1504,6 +1525,10 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     }
     if (enlarger_diffusion)
         for (int c = 0; c < 3; ++c) f.plin[c] = (double*)malloc(sizeof(double) * N);
+    if (grain) {
+        for (int c = 0; c < 3; ++c) f.dens[c] = (float*)malloc(sizeof(float) * N);
+        f.dens_out = 1;
+    }
     if (rc != 0) {
     } else if (p->spatial_dir.hash != 0) {
         for (int i = 0; i < 6; ++i) f.plane[i] = (double*)malloc(sizeof(double) * N);
@@ -1513,6 +1538,11 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
         for (int i = 0; i < 6; ++i) free(f.plane[i]);
     } else {
         run_rows(&f, rows_chain, threads);
+    }
+    if (rc == 0 && grain) {
+        rc = fjo_grain_apply(p, f.dens, width, height);
+        f.dens_out = 0;
+        if (rc == 0) run_rows(&f, rows_density, threads);
     }
     if (rc == 0 && enlarger_diffusion) {
         /* develop_print_medium (JuicerCudaExecutor.cpp:2282-2382). */
@@ -1553,6 +1583,7 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     for (int c = 0; c < 3; ++c) {
         free(f.carrier[c]);
         free(f.plin[c]);
+        free(f.dens[c]);
     }
     free(f.linear);
     pthread_mutex_destroy(&f.lock);

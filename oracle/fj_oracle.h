@@ -40,6 +40,19 @@ int fjo_scanner_lut_build(const float* eps_c, const float* eps_m, const float* e
 /* Gaussian FIR weights of 2*radius+1 taps (rust optics/gaussian.rs:20-56). */
 int fjo_gaussian_weights(int radius, float sigma, float* out);
 
+/* Visual grain (fjo_grain.c; FP32 restatement of
+ * juicer_cuda_apply_visual_grain, JuicerCudaScanPipeline.cu:3141-3756): applies
+ * the record's grain to three C/M/Y density planes in place. 0 on success. */
+int fjo_grain_apply(const FjPreparedHostData* p, float* const dens[3], int width, int height);
+/* One uniform of the STBN stream of pixel (x, y) of the render window. */
+float fjo_grain_uniform(const FjPreparedHostData* p, uint64_t seed, int x, int y, int frame_offset, uint32_t draw);
+uint64_t fjo_splitmix64(uint64_t v);
+/* Deterministic FP32 elementary functions of the grain restatement. */
+float fjo_det_expf(float x);
+float fjo_det_logf(float x);
+float fjo_det_sinf(float x);
+float fjo_det_cosf(float x);
+
 /* Reflected-YVV boundary solve for a line of n samples: weights[3*n]
  * (start-state rows) and terminal[size*size]; returns size = min(n, 3). */
 int fjo_yvv_boundary(const double c[9], int n, double* weights, double* terminal);

@@ -44,6 +44,14 @@ def lib():
         L.fjo_diffusion.argtypes = [C.c_void_p, C.POINTER(dp), C.c_int, C.c_int]
         L.fjo_diffusion_psf.argtypes = [C.c_void_p, C.c_int, fp]
         L.fjo_diffusion_components.argtypes = [C.c_void_p, dp, dp]
+        L.fjo_grain_apply.argtypes = [C.c_void_p, C.POINTER(fp), C.c_int, C.c_int]
+        L.fjo_grain_uniform.restype = C.c_float
+        L.fjo_grain_uniform.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_uint32]
+        L.fjo_splitmix64.restype = C.c_uint64
+        L.fjo_splitmix64.argtypes = [C.c_uint64]
+        for name in ("fjo_det_expf", "fjo_det_logf", "fjo_det_sinf", "fjo_det_cosf"):
+            getattr(L, name).restype = C.c_float
+            getattr(L, name).argtypes = [C.c_float]
         L.fjo_render.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t, fp, C.c_ssize_t,
                                  C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_uint32)]
         _lib = L
@@ -193,3 +201,28 @@ class Plan:
         if rc != 0:
             raise RuntimeError(f"oracle render failed rc={rc}")
         return dst, lin, flags.value
+
+
+def grain_apply(record, density) -> np.ndarray:
+    """Visual grain of `record` (an FjPreparedHostData) applied to density
+    planes [3][H][W] (C, M, Y; float32); returns the grained planes."""
+    d = np.ascontiguousarray(density, dtype=np.float32).copy()
+    _, h, w = d.shape
+    ptrs = (C.POINTER(C.c_float) * 3)(*[d[c].ctypes.data_as(C.POINTER(C.c_float)) for c in range(3)])
+    rc = lib().fjo_grain_apply(C.byref(record), ptrs, w, h)
+    if rc != 0:
+        raise RuntimeError(f"oracle grain rc={rc}")
+    return d
+
+
+def grain_uniform(record, seed: int, x: int, y: int, frame_offset: int, draw: int) -> float:
+    return float(lib().fjo_grain_uniform(C.byref(record), seed, x, y, frame_offset, draw))
+
+
+def splitmix64(v: int) -> int:
+    return int(lib().fjo_splitmix64(v & 0xFFFFFFFFFFFFFFFF))
+
+
+def det(name: str, values) -> np.ndarray:
+    f = getattr(lib(), f"fjo_det_{name}f")
+    return np.array([f(float(v)) for v in np.asarray(values, dtype=np.float32).ravel()], dtype=np.float32)
