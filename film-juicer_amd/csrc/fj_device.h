@@ -116,10 +116,14 @@ struct PixelParams {
     // set, the fused and DIR-final stages stop at the print exposure and
     // write it here; the enlarger-final stage develops the print from it.
     float* plin[3];
+    // Film density C/M/Y planes (width*height each) for the visual-grain
+    // stage: the fused and DIR-final stages stop after development and write
+    // them; the density-final stage continues from them.
+    float* dens[3];
 };
 
 // Pixel stage of one launch.
-enum PixelMode : int { kFused = 0, kDirSource = 1, kDirFinal = 2, kCarrier = 3, kEnlargerFinal = 4 };
+enum PixelMode : int { kFused = 0, kDirSource = 1, kDirFinal = 2, kCarrier = 3, kEnlargerFinal = 4, kDensityFinal = 5 };
 
 // Post stage: unsharp mask on the three linear planes, output encoding,
 // alpha copied from the source (JuicerCudaScanPipeline.cu:2733-2878).

@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int til
         const int y = static_cast<int>(idx / width);
         const int x = static_cast<int>(idx - static_cast<long long>(y) * width);
         float in[4] = {0.0f, 0.0f, 0.0f, 1.0f};
-        if constexpr (kMode != kDirFinal && kMode != kEnlargerFinal) {
+        if constexpr (kMode != kDirFinal && kMode != kEnlargerFinal && kMode != kDensityFinal) {
             const float* sp = reinterpret_cast<const float*>(p.io.src + y * p.io.src_row_bytes) + x * p.io.comps;
             if (p.io.comps == 4) {
                 const float4 v = *reinterpret_cast<const float4*>(sp);
@@ -87,7 +87,10 @@ __global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int til
             }
         }
         float D[3];
-        if constexpr (kMode == kEnlargerFinal) {
+        if constexpr (kMode == kDensityFinal) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) D[c] = p.dens[c][idx];
+        } else if constexpr (kMode == kEnlargerFinal) {
             float lin[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) lin[c] = p.plin[c][idx];
@@ -123,6 +126,13 @@ __global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int til
             film_exposure(p, L.mallett, in, idx, E);
             errors |= develop_film(p.dev, L.film, E, D);
         }
+        if constexpr (kMode == kFused || kMode == kDirFinal) {
+            if (p.dens[0]) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) p.dens[c][idx] = D[c];
+                continue;
+            }
+        }
         if constexpr (kPrint && kMode != kEnlargerFinal) {
             if (p.plin[0]) {
                 float lin[3];
@@ -146,7 +156,7 @@ __global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int til
             p.linear_planes[2 * plane + idx] = rgb[2];
             continue;
         }
-        if constexpr (kMode == kDirFinal || kMode == kEnlargerFinal) {
+        if constexpr (kMode == kDirFinal || kMode == kEnlargerFinal || kMode == kDensityFinal) {
             const float* sp = reinterpret_cast<const float*>(p.io.src + y * p.io.src_row_bytes) + x * p.io.comps;
             if (p.io.comps == 4) in[3] = sp[3];
         }
@@ -236,11 +246,32 @@ hipError_t launch_pixel_stage(const PixelParams& p, int mode, int grid_limit, hi
             return hipGetLastError();
         }
         case kDirFinal: return launch_mode<kDirFinal>(p, grid, tiles, lds, use_lds, stream);
+        case kDensityFinal: {
+            // Print exposure (or its linear planes), scanner and colour: no
+            // exposure or film tables.
+            PixelParams q = p;
+            q.ex.mallett = nullptr;
+            q.dev.n = 0;
+            q.dev.dir_mode = 0;
+            const size_t lq = fused_lds_bytes(q);
+            const bool lds_ok = lq <= 64 * 1024 && (!q.print || q.pr.n <= kMaxLdsCurve);
+            if (q.print) {
+                if (lds_ok)
+                    hipLaunchKernelGGL((pixel_stage_kernel<true, true, kDensityFinal>), dim3(grid), dim3(block), lq, stream, q, tiles);
+                else
+                    hipLaunchKernelGGL((pixel_stage_kernel<true, false, kDensityFinal>), dim3(grid), dim3(block), 0, stream, q, tiles);
+            } else {
+                if (lds_ok)
+                    hipLaunchKernelGGL((pixel_stage_kernel<false, true, kDensityFinal>), dim3(grid), dim3(block), lq, stream, q, tiles);
+                else
+                    hipLaunchKernelGGL((pixel_stage_kernel<false, false, kDensityFinal>), dim3(grid), dim3(block), 0, stream, q, tiles);
+            }
+            return hipGetLastError();
+        }
         case kEnlargerFinal: {
             // Print curves, scanner and colour only: no film tables.
             if (!p.print) return hipErrorInvalidValue;
             PixelParams q = p;
-            q.plin[0] = p.plin[0];
             q.ex.mallett = nullptr;
             q.dev.n = 0;
             q.pr.k = 0;

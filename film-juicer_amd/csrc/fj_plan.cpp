@@ -2,6 +2,7 @@
 // device layouts, and the frame schedule on the caller's stream.
 #include "fj_plan.h"
 #include "fj_diffusion_host.h"
+#include "fj_grain_host.h"
 #include "fj_halation_host.h"
 
 #include <algorithm>
@@ -60,6 +61,8 @@ hipError_t DeviceState::init(int dev) {
         e = hipEventCreate(&stage_events[i]);
         if (e != hipSuccess) return e;
     }
+    e = hipEventCreateWithFlags(&workspace_done, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
     e = fft_kernels_init(&max_line);
     if (e != hipSuccess) return e;
     if (halation_kernels_init() != hipSuccess) {
@@ -85,6 +88,9 @@ hipError_t DeviceState::release() {
     if (workspace) keep(hipFree(workspace));
     workspace = nullptr;
     workspace_bytes = 0;
+    if (workspace_done) keep(hipEventDestroy(workspace_done));
+    workspace_done = nullptr;
+    workspace_used = false;
     for (int i = 0; i < kRing; ++i)
         if (error_events[i]) keep(hipEventDestroy(error_events[i]));
     for (int i = 0; i <= kMaxStages; ++i)
@@ -139,6 +145,17 @@ hipError_t DeviceState::table(uint64_t key, const void* host, size_t bytes, cons
     table_bytes += bytes;
     *out = t.ptr;
     return hipSuccess;
+}
+
+hipError_t DeviceState::acquire_workspace(hipStream_t stream) {
+    if (!workspace_used || workspace_stream == stream) return hipSuccess;
+    return hipStreamWaitEvent(stream, workspace_done, 0);
+}
+
+hipError_t DeviceState::release_workspace(hipStream_t stream) {
+    workspace_stream = stream;
+    workspace_used = true;
+    return hipEventRecord(workspace_done, stream);
 }
 
 hipError_t DeviceState::ensure_workspace(size_t bytes) {
@@ -347,7 +364,10 @@ bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
             !same(f.render_window, f.full_frame_extent))
             return d.unsupported("optics require the complete frame domain");
     }
-    if (p.grain.flags & FJ_GRAIN_ACTIVE) return d.unsupported("visual grain");
+    {
+        std::string field;
+        if (!grain_decode(p.grain, field)) return d.malformed(field.c_str());
+    }
     if (p.effects.flags != 0 || p.effects.hash != 0) return d.unsupported("film/gate effects");
     return true;
 }
@@ -377,6 +397,19 @@ bool check_optics_frame(const FjPreparedHostData& p, const FjFrame& f, Diag& d) 
     if (!same_extent(p.optics.full_frame, f.source.bounds) || !same_extent(p.optics.full_frame, f.render_window) ||
         !same_extent(p.optics.full_frame, f.full_frame_extent))
         return d.malformed("optics.frame");
+    return true;
+}
+
+// Grain binding (native/juicer_cuda_prepared.cpp:200-207); a descriptor whose
+// blurs need the whole frame renders the whole frame.
+bool check_grain_frame(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
+    if (!grain_binding(p.grain, f)) return d.malformed("grain.frame");
+    if ((p.grain.flags & FJ_GRAIN_FULL_FRAME) != 0) {
+        const FjRect& w = f.render_window;
+        const FjRect& full = f.full_frame_extent;
+        if (w.x1 != full.x1 || w.y1 != full.y1 || w.x2 != full.x2 || w.y2 != full.y2)
+            return d.unsupported("grain blurs require the complete frame domain");
+    }
     return true;
 }
 
@@ -759,7 +792,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     if (!validate_prepared(prepared, d) || !validate_binding(prepared, frame, submission, d))
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
     if (!validate_frame(frame, d) || !check_supported(prepared, frame, d) ||
-        !check_spatial_dir(prepared, frame, dev.max_line, d) || !check_optics_frame(prepared, frame, d))
+        !check_spatial_dir(prepared, frame, dev.max_line, d) || !check_optics_frame(prepared, frame, d) ||
+        !check_grain_frame(prepared, frame, d))
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
     ++dev.frame_serial;
     bool invalid = false;
@@ -845,6 +879,20 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         padded_floats = std::max({padded_floats, 6LL * pw * ph, static_cast<long long>(diffusion_build_floats(*dd, pw, ph))});
     }
 
+    const bool grain = (prepared.grain.flags & FJ_GRAIN_ACTIVE) != 0;
+    GrainPlan gp;
+    if (grain) {
+        bool gok = false, rejected = false;
+        std::string gdiag;
+        e = grain_prepare(dev, prepared, gp, gok, gdiag, rejected);
+        if (e != hipSuccess) return outcome_hip(e, "grain tables", error);
+        if (rejected) return outcome_hip(hipErrorInvalidValue, "print visual grain launch", error);
+        if (!gok)
+            return outcome_diag(gdiag.rfind("UnsupportedInput", 0) == 0 ? FJ_STATUS_UNSUPPORTED_INPUT
+                                                                        : FJ_STATUS_PREPARATION_FAILURE,
+                                gdiag, error);
+    }
+
     HalationStages hs;
     int hal_k = 0;
     if (halation) {
@@ -866,6 +914,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     const long long carrier_off = carrier ? take(6 * plane) : -1;
     const long long q_off = (halation && hal_k > 0) ? take(3LL * hal_k * plane) : -1;
     const long long plin_off = enlarger_diffusion ? take(3 * plane) : -1;
+    const long long dens_off = grain ? take(3 * plane) : -1;
+    const long long grain_off = grain ? take(5 * plane) : -1;
     long long post_off = -1;
     if (post) post_off = spatial_dir ? dir_off + 9 * plane : (q_off >= 0 ? q_off : take(3 * plane));
     const long long pad_off = padded_floats ? take(padded_floats) : -1;
@@ -874,6 +924,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         e = dev.ensure_workspace(static_cast<size_t>(ws_floats) * sizeof(float));
         if (e != hipSuccess) return outcome_hip(e, "workspace", error);
         ws = static_cast<float*>(dev.workspace);
+        e = dev.acquire_workspace(stream);
+        if (e != hipSuccess) return outcome_hip(e, "workspace order", error);
     }
     if (post) {
         pp.linear_out = 1;
@@ -958,6 +1010,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     }
     if (enlarger_diffusion)
         for (int c = 0; c < 3; ++c) pp.plin[c] = ws + plin_off + c * plane;
+    if (grain)
+        for (int c = 0; c < 3; ++c) pp.dens[c] = ws + dens_off + c * plane;
     if (spatial_dir) {
         for (int c = 0; c < 6; ++c) pp.dir_planes[c] = ws + dir_off + c * plane;
         mark("dir_source");
@@ -1000,6 +1054,16 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         e = launch_pixel_stage(pp, kFused, grid_limit, stream);
         if (e != hipSuccess) return outcome_hip(e, "fused pixel kernel", error);
     }
+    if (grain) {
+        // develop_print_capture (JuicerCudaExecutor.cpp:2198-2280): visual
+        // grain on the developed density, then the print exposure.
+        mark("visual_grain");
+        e = run_grain(gp, pp.dens, pp.io.width, pp.io.height, ws + grain_off, stream);
+        if (e != hipSuccess) return outcome_hip(e, "print visual grain launch", error);
+        mark(enlarger_diffusion ? "print_linear" : "print_scan");
+        e = launch_pixel_stage(pp, kDensityFinal, grid_limit, stream);
+        if (e != hipSuccess) return outcome_hip(e, "density final stage", error);
+    }
     if (enlarger_diffusion) {
         // develop_print_medium (JuicerCudaExecutor.cpp:2282-2382): enlarger
         // diffusion of the print-linear exposure, then the print and scan.
@@ -1031,6 +1095,10 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     }
     mark(nullptr);
     dev.stage_count = trace ? std::min(stage - 1, static_cast<int>(DeviceState::kMaxStages)) : dev.stage_count;
+    if (ws_floats) {
+        e = dev.release_workspace(stream);
+        if (e != hipSuccess) return outcome_hip(e, "workspace order", error);
+    }
     e = hipMemcpyAsync(dev.error_host + slot, pp.error_word, sizeof(unsigned int), hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) return outcome_hip(e, "error readback", error);
     e = hipEventRecord(dev.error_events[slot], stream);
@@ -1046,6 +1114,45 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     }
     if (earlier & (1u << 8)) out.flags |= FJ_RENDER_DEFERRED_DIR_ERROR;
     return out;
+}
+
+FjStatus grain_apply_planes(DeviceState& dev, const FjPreparedHostData& prepared, float* const density[3], int width,
+                            int height, hipStream_t stream, FjErrorBuffer* error) {
+    Diag d;
+    if (!validate_prepared(prepared, d)) return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error).status;
+    std::string field;
+    if (!grain_decode(prepared.grain, field)) {
+        d.malformed(field.c_str());
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error).status;
+    }
+    if ((prepared.grain.flags & FJ_GRAIN_ACTIVE) == 0 || width <= 0 || height <= 0 || !density[0] || !density[1] ||
+        !density[2] || prepared.grain.render_extent.width != width || prepared.grain.render_extent.height != height)
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput grain planes do not match an active descriptor",
+                            error)
+            .status;
+    ++dev.frame_serial;
+    GrainPlan gp;
+    bool ok = false, rejected = false;
+    std::string diag;
+    hipError_t e = grain_prepare(dev, prepared, gp, ok, diag, rejected);
+    if (e != hipSuccess) return outcome_hip(e, "grain tables", error).status;
+    if (rejected) return outcome_hip(hipErrorInvalidValue, "visual grain launch", error).status;
+    if (!ok)
+        return outcome_diag(diag.rfind("UnsupportedInput", 0) == 0 ? FJ_STATUS_UNSUPPORTED_INPUT
+                                                                    : FJ_STATUS_PREPARATION_FAILURE,
+                            diag, error)
+            .status;
+    const long long plane = static_cast<long long>(width) * height;
+    e = dev.ensure_workspace(static_cast<size_t>(plane) * 5 * sizeof(float));
+    if (e != hipSuccess) return outcome_hip(e, "workspace", error).status;
+    e = dev.acquire_workspace(stream);
+    if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
+    e = run_grain(gp, density, width, height, static_cast<float*>(dev.workspace), stream);
+    if (e != hipSuccess) return outcome_hip(e, "visual grain launch", error).status;
+    e = dev.release_workspace(stream);
+    if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
+    put_error(error, "");
+    return status_of(FJ_STATUS_SUCCESS);
 }
 
 }  // namespace fjx

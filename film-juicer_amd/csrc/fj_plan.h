@@ -43,9 +43,14 @@ struct DeviceState {
     bool error_pending[kRing] = {};
     int error_next = 0;
 
-    // Frame workspace for staged (spatial) schedules.
+    // Frame workspace for staged (spatial) schedules. Work that uses it on
+    // one stream is ordered after the previous user's stream through
+    // `workspace_done` (recorded after each user's last launch).
     void* workspace = nullptr;
     size_t workspace_bytes = 0;
+    hipEvent_t workspace_done = nullptr;
+    hipStream_t workspace_stream = nullptr;
+    bool workspace_used = false;
 
     // Stage timing of the last traced frame (FJ_FRAME_TRACE_VERBOSE).
     static constexpr int kMaxStages = 16;
@@ -60,6 +65,10 @@ struct DeviceState {
     // used by the current frame are never evicted during it.
     hipError_t table(uint64_t key, const void* host, size_t bytes, const void** out);
     hipError_t ensure_workspace(size_t bytes);
+    // Orders `stream` after the workspace's previous user; call before the
+    // first launch that touches the workspace, then release after the last.
+    hipError_t acquire_workspace(hipStream_t stream);
+    hipError_t release_workspace(hipStream_t stream);
     // Collects completed error words of earlier frames; returns OR of their bits.
     unsigned int drain_errors(bool wait_all);
 };
@@ -67,6 +76,11 @@ struct DeviceState {
 FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSubmission& submission,
                              const FjPreparedHostData& prepared, FjAbortCallback abort_callback,
                              FjErrorBuffer* error);
+
+// The visual-grain stage of `prepared` applied to three caller-owned C/M/Y
+// density planes (device memory, width x height each) on `stream`.
+FjStatus grain_apply_planes(DeviceState& dev, const FjPreparedHostData& prepared, float* const density[3], int width,
+                            int height, hipStream_t stream, FjErrorBuffer* error);
 
 uint64_t hash_bytes(const void* data, size_t bytes, uint64_t seed);
 

@@ -297,7 +297,8 @@ def with_grain(p: Prepared, width: int, height: int, pixel_size_um: float, *, se
                noise_shape=(128, 128, 64), sublayers: bool | None = None, **overrides) -> Prepared:
     """Turns on visual grain: resolves the recipe (fj.grain.recipe) from the
     plug-in default controls with `overrides`, composes the frame plan for a
-    width x height frame (render window `window`, default the whole frame),
+    width x height frame (render extent `window` = (x, y, width, height),
+    default the whole frame),
     attaches the development's sub-layer curves and synthetic static noise.
     The frame passed to the render must carry the same pixel size, session
     seed, clip token and time."""

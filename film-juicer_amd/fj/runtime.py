@@ -46,6 +46,9 @@ def lib():
         L.fj_cuda_retire_instance.argtypes = [C.c_void_p, C.c_uint64, eb]
         L.fj_cuda_shutdown.restype = abi.FjStatus
         L.fj_cuda_shutdown.argtypes = [C.c_void_p, eb]
+        L.fjx_grain_apply.restype = abi.FjStatus
+        L.fjx_grain_apply.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjPreparedHostData), C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, eb]
         L.fjx_stage_timings.restype = C.c_int32
         L.fjx_stage_timings.argtypes = [C.c_void_p, C.c_int32, C.POINTER(FjxStageTiming), C.c_int32]
         L.fj_cuda_destroy.restype = abi.FjStatus
@@ -109,7 +112,7 @@ def image(address: int, width: int, height: int, components: int, row_bytes: int
 
 def frame_for(src_ptr: int, dst_ptr: int, width: int, height: int, components: int = 4,
               stream: int | None = None, pixel_size_um: float | None = None, time_frames: float = 0.0,
-              window: tuple | None = None) -> abi.FjFrame:
+              window: tuple | None = None, session_seed: int = 0, clip_token: int = 0) -> abi.FjFrame:
     f = abi.FjFrame()
     f.source = image(src_ptr, width, height, components)
     f.destination = image(dst_ptr, width, height, components)
@@ -123,6 +126,8 @@ def frame_for(src_ptr: int, dst_ptr: int, width: int, height: int, components: i
     f.pixel_size_um = pixel_size_um if pixel_size_um is not None else 35000.0 / max(width, height)
     f.time_frames = time_frames
     f.frame_rate = 24.0
+    f.session_seed = session_seed
+    f.clip_token = clip_token
     return f
 
 
@@ -166,6 +171,15 @@ class Renderer:
         if out.status.category != abi.FJ_STATUS_SUCCESS:
             raise FjError(out.status, text)
         return out
+
+    def grain_apply(self, prepared_record, planes, width: int, height: int, stream: int = 0,
+                    device_id: int = 0):
+        """Runs the visual-grain stage on three device density planes
+        (addresses C, M, Y); returns (FjStatus, diagnostic text)."""
+        err = ErrorBuffer()
+        st = lib().fjx_grain_apply(self.handle, device_id, C.byref(prepared_record), planes[0], planes[1], planes[2],
+                                   width, height, stream, err.ptr())
+        return st, err.text
 
     def stage_timings(self, device_id: int = 0) -> list:
         """(name, ms) of each stage of the last frame rendered with FJ_FRAME_TRACE_VERBOSE."""

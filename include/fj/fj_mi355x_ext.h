@@ -24,6 +24,19 @@ typedef struct FjxStageTiming {
  * stage timings into `out`. Returns the number of stages, or -1 on error. */
 int32_t fjx_stage_timings(FjCuda* cuda, int32_t device_id, FjxStageTiming* out, int32_t capacity);
 
+/* Applies the visual-grain stage of `prepared` (its FjVisualGrain, static
+ * noise and development sub-layer curves) to three C/M/Y film-density planes
+ * in device memory, width x height each (the grain render extent), on the HIP
+ * stream `stream` (0: the null stream). This is the stage fj_cuda_render runs
+ * between film development and the print exposure
+ * (juicer_cuda_apply_visual_grain, src/Cuda/Scan/JuicerCudaScanPipeline.cu:
+ * 3141-3756); it is exported for callers that hold densities of their own and
+ * for stage-level tests. Returns the status; the planes are modified when the
+ * stream completes. */
+FjStatus fjx_grain_apply(FjCuda* cuda, int32_t device_id, const FjPreparedHostData* prepared, float* density_c,
+                         float* density_m, float* density_y, int32_t width, int32_t height, uint64_t stream,
+                         FjErrorBuffer* error);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,202 @@
+"""GPU visual grain against the oracle's FP32 restatement.
+
+Stage level (fjx_grain_apply on caller density planes): the GPU output must be
+bit-identical to the oracle's, across the schedule's branches — sub-layer
+triplets, per-layer dye-cloud blurs, the no-sub-layer sampler, fine/mid/coarse
+scales, shared and independent chroma, frame retiming, clump and breathing
+fields, partial render windows and every debug view. Identical bits mean the
+STBN streams (every splitmix64-addressed draw), the Poisson counts and the
+FIR accumulations agree exactly.
+
+Frame level (fj_cuda_render): the film density entering the grain differs
+from the FP64 oracle's by FP32 rounding (~1e-7 relative), so a Poisson count
+can differ by one where a sample sits on a decision boundary. The gate: all
+but at most 1 in 2000 pixels within 2e-4 of the oracle in linear output, with
+the mean error below 2e-5; with zero grain amplitude the render matches the
+oracle everywhere within 2e-4.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from fj import abi, descriptors, prepared, runtime, synth  # noqa: E402
+from oracle import oracle  # noqa: E402
+
+UHD_PX = descriptors.pixel_size_for(3840, 2160)
+
+
+def record(case, w, h, **kw):
+    p = prepared.simple_view(prepared.load(case), encode=False)
+    return prepared.with_grain(p, w, h, kw.pop("px", UHD_PX), **kw)
+
+
+def densities(w, h, seed):
+    rng = np.random.default_rng(seed)
+    base = rng.uniform(0.05, 2.4, size=(3, 1, 1)).astype(np.float32)
+    ramp = np.linspace(0.0, 1.0, w, dtype=np.float32)[None, None, :]
+    d = base * (0.3 + 0.7 * ramp) + rng.normal(0, 0.05, size=(3, h, w)).astype(np.float32)
+    d[:, :2, :3] = [[[0.0]], [[1e-7]], [[3.0]]]   # extremes: zero, tiny, beyond the layer curves
+    return np.ascontiguousarray(d.astype(np.float32))
+
+
+def gpu_grain(p, dens):
+    _, h, w = dens.shape
+    t = torch.from_numpy(dens.copy()).cuda()
+    st, text = runtime.shared_renderer().grain_apply(p.record, [t[c].data_ptr() for c in range(3)], w, h,
+                                                     torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return st, text, t.cpu().numpy()
+
+
+CASES = {
+    "defaults": dict(),
+    "simple-independent": dict(sublayers=False, chroma_shared_weight=0.0, chroma_independent_weight=1.0),
+    "mixed-scales": dict(mid_weight=0.3, coarse_weight=0.3, size_mix_scale=4.0, chroma_shared_weight=0.0,
+                         chroma_independent_weight=1.0),
+    "dye-cloud-layers": dict(dye_cloud_blur_um=60.0),
+    "shared-plus-independent": dict(chroma_mix=0.4, chroma_shared_weight=0.6, chroma_independent_weight=0.5),
+    "retimed-clumps": dict(frame_time=3.4, clump_temporal_mix=0.2, micro_structure=[8.0, 60.0]),
+    "sublayer-count-3": dict(sublayers=False, sublayer_count=3, chroma_shared_weight=0.0,
+                             chroma_independent_weight=1.0),
+    "coarse-pitch": dict(px=40.0),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_grain_stage_bit_exact(name):
+    w, h = 160, 96
+    kw = dict(CASES[name])
+    p = record("PrintPreflash", w, h, **kw)
+    dens = densities(w, h, seed=len(name))
+    st, text, got = gpu_grain(p, dens)
+    assert st.category == abi.FJ_STATUS_SUCCESS, text
+    want = oracle.grain_apply(p.record, dens)
+    moved = np.abs(want - dens).max()
+    assert moved > 1e-4, "grain must change the densities"
+    diff = got != want
+    assert not diff.any(), f"{diff.sum()} of {diff.size} differ; max {np.abs(got - want).max():.3e}"
+
+
+@pytest.mark.parametrize("view", [1, 2, 3, 4, 5, 6])
+def test_grain_debug_views_bit_exact(view):
+    w, h = 96, 64
+    p = record("PrintPreflash", w, h, debug_view=view, mid_weight=0.2, coarse_weight=0.3, size_mix_scale=3.0)
+    dens = densities(w, h, seed=view)
+    st, text, got = gpu_grain(p, dens)
+    assert st.category == abi.FJ_STATUS_SUCCESS, text
+    np.testing.assert_array_equal(got, oracle.grain_apply(p.record, dens))
+
+
+def test_grain_partial_window_bit_exact():
+    """Absolute pixel coordinates drive the streams: a window's grain equals
+    the same pixels of the whole frame's grain (no blurs, so no frame-edge
+    dependence)."""
+    w, h = 120, 80
+    kw = dict(correlation_sigma_px=0.0, dye_cloud_blur_um=0.0)
+    full = record("PrintPreflash", w, h, **kw)
+    part = record("PrintPreflash", w, h, window=(24, 16, 64, 40), **kw)  # x, y, width, height
+    assert part.record.grain.flags & abi.FJ_GRAIN_FULL_FRAME == 0
+    dens = densities(w, h, seed=5)
+    _, _, got_full = gpu_grain(full, dens)
+    st, text, got_part = gpu_grain(part, np.ascontiguousarray(dens[:, 16:56, 24:88]))
+    assert st.category == abi.FJ_STATUS_SUCCESS, text
+    np.testing.assert_array_equal(got_part, got_full[:, 16:56, 24:88])
+    np.testing.assert_array_equal(got_part, oracle.grain_apply(part.record, np.ascontiguousarray(dens[:, 16:56, 24:88])))
+
+
+def render_gpu(p, src, px, **frame_kw):
+    h, w, c = src.shape
+    s = torch.from_numpy(np.ascontiguousarray(src)).cuda()
+    d = torch.full_like(s, -7.0)
+    f = runtime.frame_for(s.data_ptr(), d.data_ptr(), w, h, c, stream=torch.cuda.current_stream().cuda_stream,
+                          pixel_size_um=px, **frame_kw)
+    out, text = runtime.shared_renderer().render_raw(f, p.record)
+    torch.cuda.synchronize()
+    return d.cpu().numpy(), out, text
+
+
+def frame_gate(got, lin):
+    err = np.abs(got[..., :3].astype(np.float64) - np.clip(lin, 0, 1))
+    bad = (err > 2e-4).any(axis=-1)
+    assert bad.mean() <= 1.0 / 2000, f"{bad.sum()} pixels beyond 2e-4 (max {err.max():.3e})"
+    assert err.mean() < 2e-5, f"mean {err.mean():.3e}"
+    return err
+
+
+@pytest.mark.parametrize("case", ["PrintPreflash", "NegativeDirectHanatos"])
+def test_render_with_grain(case):
+    w, h = 192, 108
+    src = synth.frame(w, h, seed=44)
+    p = record(case, w, h, session_seed=7)
+    got, out, text = render_gpu(p, src, UHD_PX, session_seed=7)
+    assert out.status.category == abi.FJ_STATUS_SUCCESS, text
+    _, lin, _ = oracle.Plan(p).render(src, threads=4)
+    err = frame_gate(got, lin)
+    np.testing.assert_array_equal(got[..., 3], src[..., 3])
+    print(f"{case} grain frame: max {err.max():.3e} mean {err.mean():.3e}")
+
+
+def test_render_with_zero_amplitude_matches_everywhere():
+    w, h = 160, 90
+    src = synth.frame(w, h, seed=45)
+    p = record("PrintPreflash", w, h, amplitude=0.0)
+    got, out, text = render_gpu(p, src, UHD_PX, session_seed=7)
+    assert out.status.category == abi.FJ_STATUS_SUCCESS, text
+    _, lin, _ = oracle.Plan(p).render(src, threads=4)
+    assert np.abs(got[..., :3].astype(np.float64) - np.clip(lin, 0, 1)).max() <= 2e-4
+
+
+def test_full_stage_set_with_grain():
+    """Grain between DIR development and the enlarger stage, with halation
+    and camera and enlarger diffusion."""
+    w, h = 192, 108
+    src = synth.frame(w, h, seed=46)
+    p = record("PrintPreflash", w, h, session_seed=7)
+    descriptors.spatial_dir(p.record, 20.0, 200.0, 0.03, descriptors.pixel_size_for(w, h), w, h)
+    prepared.with_halation(p, w, h, 400.0, primary_amount=(0.3, 0.1, 0.015))
+    mist = descriptors.diffusion_recipe(1, 0.5)
+    prepared.with_diffusion(p, w, h, 400.0, camera=mist, enlarger=mist)
+    got, out, text = render_gpu(p, src, UHD_PX, session_seed=7)
+    assert out.status.category == abi.FJ_STATUS_SUCCESS, text
+    _, lin, _ = oracle.Plan(p).render(src, threads=4)
+    frame_gate(got, lin)
+
+
+def test_grain_binding_and_descriptor_rules():
+    w, h = 64, 48
+    src = synth.frame(w, h, seed=2)
+    p = record("PrintPreflash", w, h, session_seed=7)
+    got, out, text = render_gpu(p, src, UHD_PX, session_seed=8)
+    assert out.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT and "grain.frame" in text
+    got, out, text = render_gpu(p, src, UHD_PX * 1.5, session_seed=7)
+    assert out.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT and "grain.frame" in text
+    p.record.grain.recipe_hash = 0
+    got, out, text = render_gpu(p, src, UHD_PX, session_seed=7)
+    assert out.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT and "grain.identity" in text
+    p = record("PrintPreflash", w, h, session_seed=7)
+    p.record.noise.stbn_frames = 0
+    got, out, text = render_gpu(p, src, UHD_PX, session_seed=7)
+    assert out.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT and "noise" in text
+    assert np.all(got == -7.0)
+
+
+def test_uhd_grain_frame_properties():
+    """UHD print route with grain (seed 7): finite, in range, repeatable, alpha
+    exact, and a different session seed changes the grain."""
+    w, h = 3840, 2160
+    src = synth.frame(w, h, seed=20261203)
+    p = prepared.simple_view(prepared.load("PrintPreflash"), encode=True, unsharp=True)
+    prepared.with_grain(p, w, h, UHD_PX, session_seed=7, noise_shape=(512, 512, 16))
+    a, out, text = render_gpu(p, src, UHD_PX, session_seed=7)
+    assert out.status.category == abi.FJ_STATUS_SUCCESS, text
+    b, _, _ = render_gpu(p, src, UHD_PX, session_seed=7)
+    np.testing.assert_array_equal(a, b)
+    assert np.isfinite(a).all() and a[..., :3].min() >= 0.0 and a[..., :3].max() <= 1.0
+    np.testing.assert_array_equal(a[..., 3], src[..., 3])
+    q = prepared.simple_view(prepared.load("PrintPreflash"), encode=True, unsharp=True)
+    prepared.with_grain(q, w, h, UHD_PX, session_seed=8, noise_shape=(512, 512, 16))
+    c, out, text = render_gpu(q, src, UHD_PX, session_seed=8)
+    assert out.status.category == abi.FJ_STATUS_SUCCESS, text
+    assert np.abs(c[..., :3] - a[..., :3]).mean() > 1e-4
