@@ -6,7 +6,8 @@ route, Portra 400 capture on Portra Endura paper, scatter/halation on the
 camera carrier (still-film defaults: 65 um first bounce, 3 bounces,
 renormalized), camera and enlarger diffusion (Black Pro-Mist 1/2: PSF radius
 834 px at this pitch), spatial DIR couplers (20 um core, 200 um tail, weight
-0.03), scanner LUT res 17, OkLch gamut compression,
+0.03), density-space grain (plug-in default recipe, session seed 7),
+scanner LUT res 17, OkLch gamut compression,
 scanner unsharp (0.7 px, 0.7), sRGB output encoding. One step = one
 fj_cuda_render call over the whole frame on the caller's HIP stream, inputs
 resident in HBM. Per-stage times come from HIP events the library records on
@@ -27,6 +28,7 @@ sys.path.insert(0, os.path.join(ROOT, "film-juicer_amd"))
 sys.path.insert(0, ROOT)
 
 WIDTH, HEIGHT = 3840, 2160
+GRAIN_SEED = 7
 CASE = "PrintPreflash"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (packed); 78.6 unpacked
 PEAK_HBM_GBS = 8000.0
@@ -63,31 +65,74 @@ def active_wavelengths(rec) -> int:
 
 
 def cpu_baseline(p, src, budget_s: float = 10.0):
-    """FP64 oracle on a bounded sample of the same frame, all host cores.
+    """The oracle on bounded samples of the same workload, all host cores.
 
-    Renders growing bands of the UHD frame until one band takes about a
-    second, then repeats that band until about `budget_s` of CPU wall time.
+    Two samples, scaled to whole frames and summed: (1) every stage except the
+    two diffusion convolutions, on bands of the UHD frame grown until one band
+    takes about a second and repeated for about `budget_s`; (2) the diffusion
+    operator (FP64 direct convolution with the stage's true 834 px PSF
+    radius, whose cost per output pixel does not depend on the frame size) on
+    one small frame per host thread, run concurrently.
     """
+    import ctypes as C
+    import threading
+
+    import numpy as np
+
+    from fj import abi
     from oracle import oracle
     cores = min(len(os.sched_getaffinity(0)), 16)
-    plan = oracle.Plan(p)
-    rows = 64
-    while True:
+    rec = p.record
+    flags = rec.optics.flags
+    diff_flags = abi.FJ_OPTICS_CAMERA_DIFFUSION | abi.FJ_OPTICS_ENLARGER_DIFFUSION
+    camera = abi.FjDiffusion.from_buffer_copy(rec.optics.camera)
+    enlarger = abi.FjDiffusion.from_buffer_copy(rec.optics.enlarger)
+    dhash = rec.optics.diffusion_hash
+    rec.optics.flags = flags & ~diff_flags
+    rec.optics.camera, rec.optics.enlarger, rec.optics.diffusion_hash = abi.FjDiffusion(), abi.FjDiffusion(), 0
+    try:
+        plan = oracle.Plan(p)
+        rows = 16
+        while True:
+            t0 = time.perf_counter()
+            plan.render(src[:rows], threads=cores, want_dst=True)
+            t = time.perf_counter() - t0
+            if t >= 1.0 or rows >= HEIGHT:
+                break
+            rows = min(HEIGHT, rows * 4)
+        reps = max(1, int(round(budget_s / max(t, 1e-6))))
         t0 = time.perf_counter()
-        plan.render(src[:rows], threads=cores, want_dst=True)
-        t = time.perf_counter() - t0
-        if t >= 1.0 or rows >= HEIGHT:
-            break
-        rows = min(HEIGHT, rows * 4)
-    reps = max(1, int(round(budget_s / max(t, 1e-6))))
+        for _ in range(reps):
+            plan.render(src[:rows], threads=cores, want_dst=True)
+        t_band = time.perf_counter() - t0
+    finally:
+        rec.optics.flags = flags
+        rec.optics.camera, rec.optics.enlarger, rec.optics.diffusion_hash = camera, enlarger, dhash
+    per_px_chain = t_band / (reps * rows * WIDTH)
+    # Diffusion: each thread convolves a 4 x 2 frame (three planes) with the
+    # camera stage's PSF; both stages share the same radius.
+    sw, sh = 4, 2
+    planes = [[np.random.default_rng(i).random((sh, sw)) for _ in range(3)] for i in range(cores)]
+
+    def work(i):
+        arr = (C.POINTER(C.c_double) * 3)(*[pl.ctypes.data_as(C.POINTER(C.c_double)) for pl in planes[i]])
+        oracle.lib().fjo_diffusion(C.byref(camera), arr, sw, sh)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(cores)]
     t0 = time.perf_counter()
-    for _ in range(reps):
-        plan.render(src[:rows], threads=cores, want_dst=True)
-    t = time.perf_counter() - t0
-    fps = reps * rows / HEIGHT / t
-    return {"value": fps, "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle, {reps} x the top {rows} rows of the UHD frame ({reps * rows * WIDTH} px in "
-                      f"{t:.1f} s on {cores} threads), scaled to whole frames"}
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    t_diff = time.perf_counter() - t0
+    per_px_diff = t_diff / (cores * sw * sh)  # one stage, three channels, all threads busy
+    stages = sum(1 for f in (abi.FJ_OPTICS_CAMERA_DIFFUSION, abi.FJ_OPTICS_ENLARGER_DIFFUSION) if flags & f)
+    frame_s = (per_px_chain + stages * per_px_diff) * WIDTH * HEIGHT
+    return {"value": 1.0 / frame_s, "unit": "frames/s", "cores": cores, "kind": "port",
+            "sample": f"oracle on {cores} threads: all stages but diffusion on {reps} x the top {rows} rows of "
+                      f"the UHD frame ({t_band:.1f} s), plus the diffusion convolution (radius "
+                      f"{camera.radius_px} px) on {cores} concurrent {sw}x{sh} frames ({t_diff:.1f} s); "
+                      f"per-pixel costs scaled to whole frames ({stages} diffusion stages)"}
 
 
 def main():
@@ -115,12 +160,13 @@ def main():
     prepared.with_halation(p, WIDTH, HEIGHT, pixel_um)
     mist = descriptors.diffusion_recipe(1, 0.5)
     prepared.with_diffusion(p, WIDTH, HEIGHT, pixel_um, camera=mist, enlarger=mist)
+    prepared.with_grain(p, WIDTH, HEIGHT, pixel_um, session_seed=GRAIN_SEED, noise_shape=(512, 512, 256))
     src_np = synth.frame(WIDTH, HEIGHT, seed=synth.DEFAULT_SEED)
     src = torch.from_numpy(src_np).to(dev)
     dst = torch.empty_like(src)
     stream = torch.cuda.current_stream()
     frame = runtime.frame_for(src.data_ptr(), dst.data_ptr(), WIDTH, HEIGHT, 4, stream=stream.cuda_stream,
-                              pixel_size_um=pixel_um)
+                              pixel_size_um=pixel_um, session_seed=GRAIN_SEED)
     r = runtime.shared_renderer()
     ctx = r.inspect(frame)
     sub = runtime.submission_for(p.record)
@@ -146,7 +192,7 @@ def main():
 
     # Per-stage HIP-event timings on the same stream (traced frames).
     traced = runtime.frame_for(src.data_ptr(), dst.data_ptr(), WIDTH, HEIGHT, 4, stream=stream.cuda_stream,
-                               pixel_size_um=pixel_um)
+                               pixel_size_um=pixel_um, session_seed=GRAIN_SEED)
     traced.flags |= abi.FJ_FRAME_TRACE_VERBOSE
     stage_ms = {}
     reps = max(3, min(args.steps, 10))
@@ -189,8 +235,10 @@ def main():
                         "enlarger + preflash 0.01 -> Portra Endura, spatial DIR 20 um core + 200 um tail "
                         "weight 0.03 (DCT domain), scatter/halation (65 um, 3 bounces, renormalized) on the "
                         "camera carrier, camera + enlarger diffusion (Black Pro-Mist 1/2, PSF radius 834 px, "
-                        "DCT domain), LUT scanner res 17, OkLch gamut, unsharp 0.7/0.7, sRGB encode; grain is "
-                        "not yet in this step",
+                        "DCT domain), density-space grain (plug-in defaults: sub-layers, shared chroma, "
+                        "correlation 0.46 px; session seed 7, 35 mm pitch, STBN 512x512x256 + Wang 2-colour "
+                        "tiles of synthetic bytes), LUT scanner res 17, OkLch gamut, unsharp 0.7/0.7, sRGB "
+                        "encode",
             "stage_ms": stage_ms,
             "frame": [WIDTH, HEIGHT],
             "profile_pair": ["kodak_portra_400", "kodak_portra_endura"],
