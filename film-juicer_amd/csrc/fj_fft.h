@@ -17,6 +17,12 @@ struct FftPlanDev {
     int n = 0;
     int passes = 0;
     int radix[kMaxFftPasses] = {};
+    // Per pass p (ns = product of the radices before p): its twiddles
+    // w_n^(r k n/(ns R)) for k < ns, 1 <= r < R sit at table[n + tw_offset[p] +
+    // k (R-1) + r - 1], so neighbouring butterflies read neighbouring words;
+    // ns_magic[p] = ceil(2^32 / ns) divides butterfly indices by ns exactly.
+    int tw_offset[kMaxFftPasses] = {};
+    unsigned ns_magic[kMaxFftPasses] = {};
 };
 
 // Up to three planes transformed by one launch (blockIdx.y selects the plane).
@@ -62,7 +68,8 @@ struct TransposeArgs {
 
 // Host-side plan: radices of n (largest-first 4, then 2, 3, 5, then other primes).
 bool make_fft_plan(int n, FftPlanDev& plan);
-// exp(-2 pi i e / n) for e < n, and exp(-i pi k / 2n) for k < n, in float2 pairs.
+// exp(-2 pi i e / n) for e < n, then the per-pass twiddles of the plan, in
+// float2 pairs; and exp(-i pi k / 2n) for k < n.
 std::vector<float> fft_twiddles(int n);
 std::vector<float> dct_half_twiddles(int n);
 

@@ -34,18 +34,22 @@ __device__ __forceinline__ float2 tw(const float2* table, int e) { return table[
 
 // One Stockham pass of radix R over the whole line: reads src, writes dst.
 // Ns = product of radices already applied. Forward transform (e^{-i}).
+// `stw` holds this pass's twiddles, R-1 per butterfly index k < ns, and
+// `magic` = ceil(2^32 / ns) (exact division for j ns < 2^32).
 template <int R>
-__device__ __forceinline__ void pass_fixed(const float2* src, float2* dst, int n, int ns, const float2* table) {
+__device__ __forceinline__ void pass_fixed(const float2* src, float2* dst, int n, int ns, const float2* stw,
+                                           unsigned magic) {
     const int count = n / R;
-    const int stride_tw = n / (ns * R);
     for (int j = threadIdx.x; j < count; j += blockDim.x) {
-        const int k = j % ns;
+        const int q = ns > 1 ? static_cast<int>(__umulhi(static_cast<unsigned>(j), magic)) : j;
+        const int k = j - q * ns;
         float2 a[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = src[j + r * count];
         if (ns > 1) {
+            const float2* t = stw + k * (R - 1);
 #pragma unroll
-            for (int r = 1; r < R; ++r) a[r] = cmul(a[r], tw(table, r * k * stride_tw));
+            for (int r = 1; r < R; ++r) a[r] = cmul(a[r], t[r - 1]);
         }
         float2 b[R];
         if constexpr (R == 2) {
@@ -83,7 +87,7 @@ __device__ __forceinline__ void pass_fixed(const float2* src, float2* dst, int n
             b[2] = cadd(m2, v2);
             b[3] = csub(m2, v2);
         }
-        const int base = (j / ns) * ns * R + k;
+        const int base = q * ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) dst[base + r * ns] = b[r];
     }
@@ -115,11 +119,13 @@ __device__ float2* fft_lds(float2* a, float2* b, const FftPlanDev& plan, const f
     float2* dst = b;
     for (int p = 0; p < plan.passes; ++p) {
         const int r = plan.radix[p];
+        const float2* stw = table + plan.n + plan.tw_offset[p];
+        const unsigned magic = plan.ns_magic[p];
         switch (r) {
-            case 4: pass_fixed<4>(src, dst, plan.n, ns, table); break;
-            case 2: pass_fixed<2>(src, dst, plan.n, ns, table); break;
-            case 3: pass_fixed<3>(src, dst, plan.n, ns, table); break;
-            case 5: pass_fixed<5>(src, dst, plan.n, ns, table); break;
+            case 4: pass_fixed<4>(src, dst, plan.n, ns, stw, magic); break;
+            case 2: pass_fixed<2>(src, dst, plan.n, ns, stw, magic); break;
+            case 3: pass_fixed<3>(src, dst, plan.n, ns, stw, magic); break;
+            case 5: pass_fixed<5>(src, dst, plan.n, ns, stw, magic); break;
             default: pass_generic(src, dst, plan.n, ns, r, table); break;
         }
         __syncthreads();

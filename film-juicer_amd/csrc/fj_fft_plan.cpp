@@ -31,17 +31,37 @@ bool make_fft_plan(int n, FftPlanDev& plan) {
     if (m != 1) radices.push_back(m);
     if (static_cast<int>(radices.size()) > kMaxFftPasses) return false;
     plan.passes = static_cast<int>(radices.size());
-    for (int i = 0; i < plan.passes; ++i) plan.radix[i] = radices[i];
+    int ns = 1, offset = 0;
+    for (int i = 0; i < plan.passes; ++i) {
+        plan.radix[i] = radices[i];
+        plan.tw_offset[i] = offset;
+        plan.ns_magic[i] = ns > 1 ? static_cast<unsigned>((0x100000000ull + ns - 1) / ns) : 0u;
+        offset += ns * (radices[i] - 1);
+        ns *= radices[i];
+    }
     return true;
 }
 
 std::vector<float> fft_twiddles(int n) {
-    std::vector<float> t(2 * static_cast<size_t>(n));
+    FftPlanDev plan;
+    make_fft_plan(n, plan);
+    size_t stage = 0;
+    for (int p = 0, ns = 1; p < plan.passes; ns *= plan.radix[p], ++p) stage += static_cast<size_t>(ns) * (plan.radix[p] - 1);
+    std::vector<float> t(2 * (static_cast<size_t>(n) + stage));
     const double two_pi = 6.283185307179586476925286766559;
-    for (int e = 0; e < n; ++e) {
-        const double a = -two_pi * static_cast<double>(e) / n;
-        t[2 * e] = static_cast<float>(std::cos(a));
-        t[2 * e + 1] = static_cast<float>(std::sin(a));
+    auto put = [&](size_t i, long long e) {
+        const double a = -two_pi * static_cast<double>(e % n) / n;
+        t[2 * i] = static_cast<float>(std::cos(a));
+        t[2 * i + 1] = static_cast<float>(std::sin(a));
+    };
+    for (int e = 0; e < n; ++e) put(static_cast<size_t>(e), e);
+    for (int p = 0, ns = 1; p < plan.passes; ns *= plan.radix[p], ++p) {
+        const int R = plan.radix[p];
+        const int stride = n / (ns * R);
+        for (int k = 0; k < ns; ++k)
+            for (int r = 1; r < R; ++r)
+                put(static_cast<size_t>(n) + plan.tw_offset[p] + static_cast<size_t>(k) * (R - 1) + (r - 1),
+                    static_cast<long long>(r) * k * stride);
     }
     return t;
 }
