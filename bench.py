@@ -32,9 +32,24 @@ GRAIN_SEED = 7
 CASE = "PrintPreflash"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (packed); 78.6 unpacked
 PEAK_HBM_GBS = 8000.0
-# Measured HBM bytes per launch of the DIR-final pixel chain at this workload
-# (FETCH_SIZE 100064 KiB x 2.0 + WRITE_SIZE 97200 KiB x 1.0).
-TRAFFIC_PIXEL_CHAIN = (100064 * 2.0 + 97200 * 1.0) * 1024
+# HBM bytes per launch of the diffusion column filter (dct_filter_rows_kernel<true>)
+# at this workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
+# scaled by the factors a plain copy kernel of known size reads (x2.0 fetch, x1.0 write).
+FILTER_FETCH_KIB = 253442.9
+FILTER_WRITE_KIB = 253125.0
+
+
+def smooth_length(n: int) -> int:
+    """Smallest 2^a 3^b 5^c >= n: the padded FFT line length of a diffusion stage."""
+    m = max(n, 1)
+    while True:
+        v = m
+        for q in (2, 3, 5):
+            while v % q == 0:
+                v //= q
+        if v == 1:
+            return m
+        m += 1
 
 
 def flop_census(k_active: int, print_route: bool) -> int:
@@ -141,7 +156,20 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 256 MiB float32 buffer three times (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 64 * 1024 * 1024
+        a = torch.rand(n, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); frames shard independently "
               f"across GPUs, see the sharding notes. Run with --gpus 1.", file=sys.stderr)
@@ -201,22 +229,46 @@ def main():
         for name, ms in r.stage_timings(ctx.device_id):
             stage_ms.setdefault(name, []).append(ms)
     stage_ms = {k: float(np.mean(v)) for k, v in stage_ms.items()}
-    chain_ms = stage_ms["pixel_chain"]
-    dct_ms = stage_ms.get("dir_dct_filter")
-    hal_ms = stage_ms.get("halation_scatter", 0.0) + stage_ms.get("halation_bounce", 0.0)
-    diff_ms = stage_ms.get("camera_diffusion", 0.0) + stage_ms.get("enlarger_diffusion", 0.0)
+    pixels = WIDTH * HEIGHT
 
+    def stage(prefix):
+        return sum(v for k, v in stage_ms.items() if k == prefix or k.startswith(prefix + "."))
+
+    # Dominant kernel: the diffusion column filter, one launch per diffusion
+    # stage over three padded planes (forward DCT-II along padded columns,
+    # x T(u, v), inverse). Algorithmic bytes per launch: read the transposed
+    # padded plane (4 B), read the transfer table (4 B), write the plane (4 B)
+    # per padded sample and channel.
+    filt = [stage_ms[k] for k in ("camera_diffusion.filter", "enlarger_diffusion.filter") if k in stage_ms]
+    r_px = p.record.optics.camera.radius_px
+    pw, ph = smooth_length(WIDTH + 2 * r_px), smooth_length(HEIGHT + 2 * r_px)
+    filter_bytes = 3 * 12 * pw * ph
+    filter_ms = float(np.mean(filt))
+    filter_gbs = filter_bytes / (filter_ms * 1e-3) / 1e9
+    traffic = None
+    if FILTER_FETCH_KIB is not None and FILTER_WRITE_KIB is not None:
+        traffic = (FILTER_FETCH_KIB * 2.0 + FILTER_WRITE_KIB * 1.0) * 1024
+
+    # Spectral per-pixel chain: with grain and enlarger diffusion in the frame
+    # it runs as three kernels (film development, print exposure, print
+    # development + scan); the census counts the whole chain.
     k = active_wavelengths(p.record)
     fpx = flop_census(k, True)
-    pixels = WIDTH * HEIGHT
+    chain_ms = stage("pixel_chain") + stage("print_linear") + stage("print_scan")
     achieved_tflops = fpx * pixels / (chain_ms * 1e-3) / 1e12
+    dct_ms = stage_ms.get("dir_dct_filter")
+    hal_ms = stage("halation_scatter") + stage("halation_bounce")
+    diff_ms = stage("camera_diffusion") + stage("enlarger_diffusion")
     # DCT filter: algorithmic bytes = read three correction planes + write three filtered planes.
     dct_bytes = pixels * 3 * 8
     # Halation: each of the two linear stages reads and writes the three
     # carrier planes once (8 B/px per channel and stage).
     hal_bytes = pixels * 3 * 8 * 2
-    # Diffusion: each stage reads and writes its three planes once (8 B/px per channel and stage).
-    diff_bytes = pixels * 3 * 8 * 2
+    # Diffusion stages end to end: B_conv = 12 B/px (read, write, read the
+    # original for the blend) + 20 B per padded sample (row transform out,
+    # column pass in/table/out, inverse in) per channel and stage.
+    diff_stages = len(filt)
+    diff_bytes = diff_stages * 3 * (12 * pixels + 20 * pw * ph)
     result = {
         "metric": "UHD frames/s (print route, <=2e-4 err) plain vs batched; fwd+bwd ms/frame",
         "value": fps,
@@ -245,15 +297,25 @@ def main():
             "active_wavelengths": k,
         },
         "roofline": {
+            "bound": "hbm",
+            "kernel": "dct_filter_rows_kernel<true> (diffusion column filter: DCT-II, x T, DCT-III along "
+                      f"padded columns; 3 planes of {pw}x{ph})",
+            "achieved": filter_gbs,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": filter_gbs / PEAK_HBM_GBS,
+            "traffic": traffic,
+            "bytes_per_launch": filter_bytes,
+            "kernel_ms": filter_ms,
+            "launches_per_frame": len(filt),
+        },
+        "roofline_spectral": {
             "bound": "valu",
-            "kernel": "pixel_stage_kernel (spectral chain, DIR final development)",
+            "kernel": "pixel_stage_kernel (development, print exposure, print development + scan)",
             "achieved": achieved_tflops,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-            # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-            # (copy-calibrated x2.0 / x1.0), profiles/staged_uhd_print_halation_dir_unsharp.md.
-            "traffic": TRAFFIC_PIXEL_CHAIN,
             "flop_per_px": fpx,
             "kernel_ms": chain_ms,
         },
@@ -277,7 +339,8 @@ def main():
         },
         "roofline_diffusion": None if not diff_ms else {
             "bound": "hbm",
-            "kernel": "camera + enlarger diffusion (padded-plane DCT pipeline, 5 launches per stage, 3 planes)",
+            "kernel": "camera + enlarger diffusion end to end (padded-plane DCT pipeline, 5 launches per "
+                      "stage, 3 planes)",
             "achieved": diff_bytes / (diff_ms * 1e-3) / 1e9,
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",

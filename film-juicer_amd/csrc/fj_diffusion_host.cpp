@@ -319,7 +319,7 @@ hipError_t diffusion_prepare(DeviceState& dev, const FjDiffusion& d, int width, 
 }
 
 hipError_t run_diffusion(DeviceState& dev, const DiffusionStage& st, float* const planes[3], int width, int height,
-                         float* padded, hipStream_t stream) {
+                         float* padded, hipStream_t stream, const std::function<void(int)>& mark) {
     const int pw = st.pw, ph = st.ph, r = st.radius;
     const long long pplane = static_cast<long long>(pw) * ph;
     float* ext[3];
@@ -367,8 +367,10 @@ hipError_t run_diffusion(DeviceState& dev, const DiffusionStage& st, float* cons
         b.in[c] = tr[c];
         b.out[c] = tr[c];
     }
+    if (mark) mark(1);
     e = launch_dct_filter_rows(b, st.transfer, 3, stream);
     if (e != hipSuccess) return e;
+    if (mark) mark(2);
     // 4) tr -> ext
     t.width = ph;
     t.height = pw;

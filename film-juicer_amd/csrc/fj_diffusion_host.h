@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -41,8 +42,9 @@ size_t diffusion_build_floats(const FjDiffusion& d, int pw, int ph);
 hipError_t diffusion_prepare(DeviceState& dev, const FjDiffusion& d, int width, int height, float* scratch,
                              hipStream_t stream, DiffusionStage& out);
 // planes[c] (R, G, B order of the descriptor) are blended in place;
-// `padded` holds 9 pw x ph planes.
+// `padded` holds 6 pw x ph planes. `mark`, when set, is called with 1 just
+// before the column filter launch and with 2 just after it (trace events).
 hipError_t run_diffusion(DeviceState& dev, const DiffusionStage& st, float* const planes[3], int width, int height,
-                         float* padded, hipStream_t stream);
+                         float* padded, hipStream_t stream, const std::function<void(int)>& mark = {});
 
 }  // namespace fjx

@@ -48,7 +48,8 @@ struct DctRowsArgs {
 };
 
 // Separable-sum transfer function T(row, k) = sum_c weight[c] * t_rows[c][row] * t_cols[c][k],
-// or, for a plane whose full[plane] is set, the table T(row, k) = full[plane][row * n + k].
+// or, when full[] is set (for every plane of the launch), the table
+// T(row, k) = full[plane][row * n + k].
 struct DctTransfer {
     int count = 0;
     float weight[kMaxDctComponents] = {};

@@ -270,6 +270,9 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct3_rows_kernel(DctRowsArgs a
 // Forward DCT-II along rows, multiply by T, inverse along rows, in one pass.
 // Used on the transposed plane (rows = original columns): the row index is the
 // first-axis frequency kx, the in-row index the second-axis frequency.
+// kFullTable: T is a full per-plane table (diffusion); otherwise a weighted sum
+// of separable component factors (spatial DIR).
+template <bool kFullTable>
 __global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRowsArgs args, DctTransfer tf) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
@@ -292,9 +295,11 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRows
     // full transfer table.
     const float* full = tf.full[plane];
     float fa[kMaxDctComponents], fb[kMaxDctComponents];
-    for (int c = 0; c < tf.count; ++c) {
-        fa[c] = tf.weight[c] * tf.t_rows[c][row0];
-        fb[c] = has1 ? tf.weight[c] * tf.t_rows[c][row0 + 1] : 0.0f;
+    if (!kFullTable) {
+        for (int c = 0; c < tf.count; ++c) {
+            fa[c] = tf.weight[c] * tf.t_rows[c][row0];
+            fb[c] = has1 ? tf.weight[c] * tf.t_rows[c][row0 + 1] : 0.0f;
+        }
     }
     float2* V = fft_lds(a, b, plan, args.table);
     float2* other = V == a ? b : a;
@@ -306,7 +311,7 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRows
         const float2 vb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
         const float2 w = args.half_tw[k];
         float ta = 0.0f, tb = 0.0f;
-        if (full) {
+        if (kFullTable) {
             ta = full[static_cast<long long>(row0) * n + k];
             tb = has1 ? full[static_cast<long long>(row0 + 1) * n + k] : 0.0f;
         } else {
@@ -388,7 +393,15 @@ hipError_t launch_dct3_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
 
 hipError_t launch_dct_filter_rows(const DctRowsArgs& a, const DctTransfer& tf, int planes, hipStream_t s) {
     const dim3 grid((a.rows + 1) / 2, planes);
-    hipLaunchKernelGGL(dct_filter_rows_kernel, grid, dim3(row_threads(a.plan.n)), dct_lds_bytes(a.plan.n, true), s, a, tf);
+    if (tf.full[0]) {
+        for (int p = 1; p < planes; ++p)
+            if (!tf.full[p]) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(dct_filter_rows_kernel<true>, grid, dim3(row_threads(a.plan.n)), dct_lds_bytes(a.plan.n, true),
+                           s, a, tf);
+    } else {
+        hipLaunchKernelGGL(dct_filter_rows_kernel<false>, grid, dim3(row_threads(a.plan.n)),
+                           dct_lds_bytes(a.plan.n, true), s, a, tf);
+    }
     return hipGetLastError();
 }
 
@@ -406,9 +419,10 @@ hipError_t fft_kernels_init(int* max_lds_n) {
     // Line transforms beyond 4096 points need more than the default 64 KiB of
     // LDS; fall back to 4096-point lines if the runtime refuses the request.
     const int bytes = static_cast<int>(dct_lds_bytes(kMaxLdsFft, true));
-    const void* kernels[3] = {reinterpret_cast<const void*>(dct2_rows_kernel),
+    const void* kernels[4] = {reinterpret_cast<const void*>(dct2_rows_kernel),
                               reinterpret_cast<const void*>(dct3_rows_kernel),
-                              reinterpret_cast<const void*>(dct_filter_rows_kernel)};
+                              reinterpret_cast<const void*>(dct_filter_rows_kernel<false>),
+                              reinterpret_cast<const void*>(dct_filter_rows_kernel<true>)};
     *max_lds_n = kMaxLdsFft;
     for (const void* k : kernels) {
         if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {

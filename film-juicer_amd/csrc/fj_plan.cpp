@@ -954,7 +954,9 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         }
         if (camera_diffusion) {
             mark("camera_diffusion");
-            e = run_diffusion(dev, diff_stage[0], cur, pp.io.width, pp.io.height, ws + pad_off, stream);
+            e = run_diffusion(dev, diff_stage[0], cur, pp.io.width, pp.io.height, ws + pad_off, stream, [&](int step) {
+                mark(step == 1 ? "camera_diffusion.filter" : "camera_diffusion.inverse");
+            });
             if (e != hipSuccess) return outcome_hip(e, "camera diffusion stage", error);
         }
         if (halation) {
@@ -1068,7 +1070,9 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         // develop_print_medium (JuicerCudaExecutor.cpp:2282-2382): enlarger
         // diffusion of the print-linear exposure, then the print and scan.
         mark("enlarger_diffusion");
-        e = run_diffusion(dev, diff_stage[1], pp.plin, pp.io.width, pp.io.height, ws + pad_off, stream);
+        e = run_diffusion(dev, diff_stage[1], pp.plin, pp.io.width, pp.io.height, ws + pad_off, stream, [&](int step) {
+            mark(step == 1 ? "enlarger_diffusion.filter" : "enlarger_diffusion.inverse");
+        });
         if (e != hipSuccess) return outcome_hip(e, "enlarger diffusion stage", error);
         mark("print_scan");
         e = launch_pixel_stage(pp, kEnlargerFinal, grid_limit, stream);

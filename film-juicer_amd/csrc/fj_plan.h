@@ -53,7 +53,7 @@ struct DeviceState {
     bool workspace_used = false;
 
     // Stage timing of the last traced frame (FJ_FRAME_TRACE_VERBOSE).
-    static constexpr int kMaxStages = 16;
+    static constexpr int kMaxStages = 32;
     hipEvent_t stage_events[kMaxStages + 1] = {};
     const char* stage_names[kMaxStages] = {};
     int stage_count = 0;

@@ -183,11 +183,12 @@ class Renderer:
 
     def stage_timings(self, device_id: int = 0) -> list:
         """(name, ms) of each stage of the last frame rendered with FJ_FRAME_TRACE_VERBOSE."""
-        buf = (FjxStageTiming * 16)()
-        n = lib().fjx_stage_timings(self.handle, device_id, buf, 16)
+        cap = 64
+        buf = (FjxStageTiming * cap)()
+        n = lib().fjx_stage_timings(self.handle, device_id, buf, cap)
         if n < 0:
             raise RuntimeError("stage timings unavailable")
-        return [(buf[i].name.decode(), float(buf[i].milliseconds)) for i in range(n)]
+        return [(buf[i].name.decode(), float(buf[i].milliseconds)) for i in range(min(n, cap))]
 
     def shutdown(self):
         err = ErrorBuffer()
