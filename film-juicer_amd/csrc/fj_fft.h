@@ -23,6 +23,10 @@ struct FftPlanDev {
     // ns_magic[p] = ceil(2^32 / ns) divides butterfly indices by ns exactly.
     int tw_offset[kMaxFftPasses] = {};
     unsigned ns_magic[kMaxFftPasses] = {};
+    // In-place engine (2,3,5-smooth n): radices from {2,3,4,5,6,8,9,10,12,15,
+    // 16,20,25}, one butterfly per thread, `threads` >= n / radix of every pass.
+    int inplace = 0;
+    int threads = 0;
 };
 
 // Up to three planes transformed by one launch (blockIdx.y selects the plane).
@@ -67,14 +71,17 @@ struct TransposeArgs {
     long long out_stride = 0;
 };
 
-// Host-side plan: radices of n (largest-first 4, then 2, 3, 5, then other primes).
+// Host-side plan. A 2,3,5-smooth n gets the in-place engine's composite
+// radices (fewest passes, then the largest smallest radix, so the fewest
+// threads), odd radices first; any other n the ping-pong engine's radices (4 first, then 2, 3, 5,
+// then other primes).
 bool make_fft_plan(int n, FftPlanDev& plan);
 // exp(-2 pi i e / n) for e < n, then the per-pass twiddles of the plan, in
 // float2 pairs; and exp(-i pi k / 2n) for k < n.
 std::vector<float> fft_twiddles(int n);
 std::vector<float> dct_half_twiddles(int n);
 
-size_t dct_lds_bytes(int n, bool filter);
+size_t dct_lds_bytes(const FftPlanDev& plan);
 hipError_t launch_dct2_rows(const DctRowsArgs& a, int planes, hipStream_t s);
 hipError_t launch_dct3_rows(const DctRowsArgs& a, int planes, hipStream_t s);
 hipError_t launch_dct_filter_rows(const DctRowsArgs& a, const DctTransfer& tf, int planes, hipStream_t s);

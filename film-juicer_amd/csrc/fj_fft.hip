@@ -2,10 +2,17 @@
 //
 // Every line transform runs inside one workgroup: the line (a pair of real
 // lines packed as re/im of one complex line) is staged into LDS, transformed
-// with a self-sorting mixed-radix Stockham FFT (radix 4, 2, 3, 5 butterflies
-// in registers, any other prime factor through a generic LDS butterfly), and
-// written back. Twiddles come from a per-length table built on the host in
-// double precision.
+// with a self-sorting mixed-radix Stockham FFT, and written back. Twiddles come
+// from per-length tables built on the host in double precision.
+//
+// Lines whose length has only the prime factors 2, 3 and 5 (every padded
+// diffusion line, and the frame sides of the usual formats) run the in-place
+// engine: one LDS buffer (odd radices first, so the strided writes of the
+// first passes spread over the banks); each pass is one butterfly per thread, of a
+// composite radix up to 25 (16 = 4x4, 15 = 3x5, 25 = 5x5, ...) evaluated in
+// registers as a two-level Cooley-Tukey DFT, so a 3840- or 5625-point line
+// takes three passes. Other lengths run the ping-pong engine (radix 4, 2, 3, 5
+// in registers, other primes through a generic LDS butterfly) over two buffers.
 //
 // DCT-II / DCT-III of length N use Makhoul's reordering onto one length-N
 // complex FFT. The DCT-II basis diagonalizes convolution of the half-sample
@@ -137,6 +144,229 @@ __device__ float2* fft_lds(float2* a, float2* b, const FftPlanDev& plan, const f
     return src;
 }
 
+// ---------------------------------------------------------------------------
+// In-place engine
+// ---------------------------------------------------------------------------
+
+// cos / sin (2 pi e / N) of the composite radices' inner twiddles.
+template <int N>
+struct Roots;
+template <> struct Roots<6> {
+    static constexpr float c[6] = {1.000000000f, 5.000000000e-1f, -5.000000000e-1f, -1.000000000f, -5.000000000e-1f, 5.000000000e-1f};
+    static constexpr float s[6] = {0.000000000f, 8.660254038e-1f, 8.660254038e-1f, 0.000000000f, -8.660254038e-1f, -8.660254038e-1f};
+};
+template <> struct Roots<8> {
+    static constexpr float c[8] = {1.000000000f, 7.071067812e-1f, 0.000000000f, -7.071067812e-1f, -1.000000000f, -7.071067812e-1f, 0.000000000f, 7.071067812e-1f};
+    static constexpr float s[8] = {0.000000000f, 7.071067812e-1f, 1.000000000f, 7.071067812e-1f, 0.000000000f, -7.071067812e-1f, -1.000000000f, -7.071067812e-1f};
+};
+template <> struct Roots<9> {
+    static constexpr float c[9] = {1.000000000f, 7.660444431e-1f, 1.736481777e-1f, -5.000000000e-1f, -9.396926208e-1f, -9.396926208e-1f, -5.000000000e-1f, 1.736481777e-1f, 7.660444431e-1f};
+    static constexpr float s[9] = {0.000000000f, 6.427876097e-1f, 9.848077530e-1f, 8.660254038e-1f, 3.420201433e-1f, -3.420201433e-1f, -8.660254038e-1f, -9.848077530e-1f, -6.427876097e-1f};
+};
+template <> struct Roots<10> {
+    static constexpr float c[10] = {1.000000000f, 8.090169944e-1f, 3.090169944e-1f, -3.090169944e-1f, -8.090169944e-1f, -1.000000000f, -8.090169944e-1f, -3.090169944e-1f, 3.090169944e-1f, 8.090169944e-1f};
+    static constexpr float s[10] = {0.000000000f, 5.877852523e-1f, 9.510565163e-1f, 9.510565163e-1f, 5.877852523e-1f, 0.000000000f, -5.877852523e-1f, -9.510565163e-1f, -9.510565163e-1f, -5.877852523e-1f};
+};
+template <> struct Roots<12> {
+    static constexpr float c[12] = {1.000000000f, 8.660254038e-1f, 5.000000000e-1f, 0.000000000f, -5.000000000e-1f, -8.660254038e-1f, -1.000000000f, -8.660254038e-1f, -5.000000000e-1f, 0.000000000f, 5.000000000e-1f, 8.660254038e-1f};
+    static constexpr float s[12] = {0.000000000f, 5.000000000e-1f, 8.660254038e-1f, 1.000000000f, 8.660254038e-1f, 5.000000000e-1f, 0.000000000f, -5.000000000e-1f, -8.660254038e-1f, -1.000000000f, -8.660254038e-1f, -5.000000000e-1f};
+};
+template <> struct Roots<15> {
+    static constexpr float c[15] = {1.000000000f, 9.135454576e-1f, 6.691306064e-1f, 3.090169944e-1f, -1.045284633e-1f, -5.000000000e-1f, -8.090169944e-1f, -9.781476007e-1f, -9.781476007e-1f, -8.090169944e-1f, -5.000000000e-1f, -1.045284633e-1f, 3.090169944e-1f, 6.691306064e-1f, 9.135454576e-1f};
+    static constexpr float s[15] = {0.000000000f, 4.067366431e-1f, 7.431448255e-1f, 9.510565163e-1f, 9.945218954e-1f, 8.660254038e-1f, 5.877852523e-1f, 2.079116908e-1f, -2.079116908e-1f, -5.877852523e-1f, -8.660254038e-1f, -9.945218954e-1f, -9.510565163e-1f, -7.431448255e-1f, -4.067366431e-1f};
+};
+template <> struct Roots<16> {
+    static constexpr float c[16] = {1.000000000f, 9.238795325e-1f, 7.071067812e-1f, 3.826834324e-1f, 0.000000000f, -3.826834324e-1f, -7.071067812e-1f, -9.238795325e-1f, -1.000000000f, -9.238795325e-1f, -7.071067812e-1f, -3.826834324e-1f, 0.000000000f, 3.826834324e-1f, 7.071067812e-1f, 9.238795325e-1f};
+    static constexpr float s[16] = {0.000000000f, 3.826834324e-1f, 7.071067812e-1f, 9.238795325e-1f, 1.000000000f, 9.238795325e-1f, 7.071067812e-1f, 3.826834324e-1f, 0.000000000f, -3.826834324e-1f, -7.071067812e-1f, -9.238795325e-1f, -1.000000000f, -9.238795325e-1f, -7.071067812e-1f, -3.826834324e-1f};
+};
+template <> struct Roots<20> {
+    static constexpr float c[20] = {1.000000000f, 9.510565163e-1f, 8.090169944e-1f, 5.877852523e-1f, 3.090169944e-1f, 0.000000000f, -3.090169944e-1f, -5.877852523e-1f, -8.090169944e-1f, -9.510565163e-1f, -1.000000000f, -9.510565163e-1f, -8.090169944e-1f, -5.877852523e-1f, -3.090169944e-1f, 0.000000000f, 3.090169944e-1f, 5.877852523e-1f, 8.090169944e-1f, 9.510565163e-1f};
+    static constexpr float s[20] = {0.000000000f, 3.090169944e-1f, 5.877852523e-1f, 8.090169944e-1f, 9.510565163e-1f, 1.000000000f, 9.510565163e-1f, 8.090169944e-1f, 5.877852523e-1f, 3.090169944e-1f, 0.000000000f, -3.090169944e-1f, -5.877852523e-1f, -8.090169944e-1f, -9.510565163e-1f, -1.000000000f, -9.510565163e-1f, -8.090169944e-1f, -5.877852523e-1f, -3.090169944e-1f};
+};
+template <> struct Roots<25> {
+    static constexpr float c[25] = {1.000000000f, 9.685831611e-1f, 8.763066800e-1f, 7.289686274e-1f, 5.358267950e-1f, 3.090169944e-1f, 6.279051953e-2f, -1.873813146e-1f, -4.257792916e-1f, -6.374239897e-1f, -8.090169944e-1f, -9.297764859e-1f, -9.921147013e-1f, -9.921147013e-1f, -9.297764859e-1f, -8.090169944e-1f, -6.374239897e-1f, -4.257792916e-1f, -1.873813146e-1f, 6.279051953e-2f, 3.090169944e-1f, 5.358267950e-1f, 7.289686274e-1f, 8.763066800e-1f, 9.685831611e-1f};
+    static constexpr float s[25] = {0.000000000f, 2.486898872e-1f, 4.817536741e-1f, 6.845471059e-1f, 8.443279255e-1f, 9.510565163e-1f, 9.980267284e-1f, 9.822872507e-1f, 9.048270525e-1f, 7.705132428e-1f, 5.877852523e-1f, 3.681245527e-1f, 1.253332336e-1f, -1.253332336e-1f, -3.681245527e-1f, -5.877852523e-1f, -7.705132428e-1f, -9.048270525e-1f, -9.822872507e-1f, -9.980267284e-1f, -9.510565163e-1f, -8.443279255e-1f, -6.845471059e-1f, -4.817536741e-1f, -2.486898872e-1f};
+};
+
+// x * exp(-2 pi i e / N) for 0 <= e < N (e is a constant once unrolled).
+template <int N>
+__device__ __forceinline__ float2 rot(float2 x, int e) {
+    if (e == 0) return x;
+    if (4 * e == N) return mul_mi(x);
+    if (2 * e == N) return make_float2(-x.x, -x.y);
+    if (4 * e == 3 * N) return make_float2(-x.y, x.x);
+    return cmul(x, make_float2(Roots<N>::c[e], -Roots<N>::s[e]));
+}
+
+// Forward DFT of R points held in registers, in place: a[k] = sum_r a[r] w_R^(r k).
+template <int R>
+struct Dft;
+
+template <>
+struct Dft<2> {
+    static __device__ __forceinline__ void run(float2* a) {
+        const float2 t = a[0];
+        a[0] = cadd(t, a[1]);
+        a[1] = csub(t, a[1]);
+    }
+};
+
+template <>
+struct Dft<3> {
+    static __device__ __forceinline__ void run(float2* a) {
+        const float c = -0.5f, s = -0.86602540378443864676f;  // cos/sin(-2pi/3)
+        const float2 t = cadd(a[1], a[2]);
+        const float2 u = csub(a[1], a[2]);
+        const float2 m = make_float2(fmaf(c, t.x, a[0].x), fmaf(c, t.y, a[0].y));
+        const float2 v = make_float2(-s * u.y, s * u.x);  // i*s*u
+        a[0] = cadd(a[0], t);
+        a[1] = cadd(m, v);
+        a[2] = csub(m, v);
+    }
+};
+
+template <>
+struct Dft<4> {
+    static __device__ __forceinline__ void run(float2* a) {
+        const float2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+        const float2 s13 = cadd(a[1], a[3]), d13 = mul_mi(csub(a[1], a[3]));
+        a[0] = cadd(s02, s13);
+        a[2] = csub(s02, s13);
+        a[1] = cadd(d02, d13);
+        a[3] = csub(d02, d13);
+    }
+};
+
+template <>
+struct Dft<5> {
+    static __device__ __forceinline__ void run(float2* a) {
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = -0.95105651629515357212f, s2 = -0.58778525229247312917f;  // sin(-2pi/5), sin(-4pi/5)
+        const float2 t1 = cadd(a[1], a[4]), u1 = csub(a[1], a[4]);
+        const float2 t2 = cadd(a[2], a[3]), u2 = csub(a[2], a[3]);
+        const float2 m1 = make_float2(a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y);
+        const float2 m2 = make_float2(a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y);
+        const float2 v1 = make_float2(-(s1 * u1.y + s2 * u2.y), s1 * u1.x + s2 * u2.x);
+        const float2 v2 = make_float2(-(s2 * u1.y - s1 * u2.y), s2 * u1.x - s1 * u2.x);
+        a[0] = cadd(a[0], cadd(t1, t2));
+        a[1] = cadd(m1, v1);
+        a[4] = csub(m1, v1);
+        a[2] = cadd(m2, v2);
+        a[3] = csub(m2, v2);
+    }
+};
+
+// N = R1 R2: input n = R2 n1 + n2, output k = k1 + R1 k2;
+// X[k1 + R1 k2] = sum_n2 w_N^(n2 k1) [sum_n1 x[R2 n1 + n2] w_R1^(n1 k1)] w_R2^(n2 k2).
+template <int R1, int R2>
+struct DftCT {
+    static __device__ __forceinline__ void run(float2* a) {
+        constexpr int N = R1 * R2;
+        float2 y[N];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+            float2 t[R1];
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) t[n1] = a[R2 * n1 + n2];
+            Dft<R1>::run(t);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) y[n2 * R1 + k1] = rot<N>(t[k1], n2 * k1);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            float2 t[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) t[n2] = y[n2 * R1 + k1];
+            Dft<R2>::run(t);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) a[k1 + R1 * k2] = t[k2];
+        }
+    }
+};
+
+template <> struct Dft<6> : DftCT<2, 3> {};
+template <> struct Dft<8> : DftCT<2, 4> {};
+template <> struct Dft<9> : DftCT<3, 3> {};
+template <> struct Dft<10> : DftCT<2, 5> {};
+template <> struct Dft<12> : DftCT<4, 3> {};
+template <> struct Dft<15> : DftCT<3, 5> {};
+template <> struct Dft<16> : DftCT<4, 4> {};
+template <> struct Dft<20> : DftCT<4, 5> {};
+template <> struct Dft<25> : DftCT<5, 5> {};
+
+// One in-place Stockham pass of radix R: thread j < n / R owns butterfly j.
+// All reads happen before the first barrier and all writes after it, so the
+// butterfly may overwrite the slots other threads read.
+template <int R>
+__device__ __forceinline__ void pass_ip(float2* buf, int n, int ns, const float2* stw, unsigned magic) {
+    const int count = n / R;
+    const int j = threadIdx.x;
+    const bool active = j < count;
+    float2 v[R];
+    int base = 0;
+    if (active) {
+        int q = j, k = 0;
+        if (ns > 1) {
+            q = static_cast<int>(__umulhi(static_cast<unsigned>(j), magic));
+            k = j - q * ns;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = buf[j + r * count];
+        if (ns > 1) {
+            const float2* t = stw + k * (R - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], t[r - 1]);
+        }
+        Dft<R>::run(v);
+        base = q * ns * R + k;
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[base + r * ns] = v[r];
+    }
+    __syncthreads();
+}
+
+// Forward FFT of buf[0..n) in place. blockDim.x >= n / R for
+// every radix of the plan (the plan's thread count).
+__device__ void fft_inplace(float2* buf, const FftPlanDev& plan, const float2* table) {
+    int ns = 1;
+    for (int p = 0; p < plan.passes; ++p) {
+        const int r = plan.radix[p];
+        const float2* stw = table + plan.n + plan.tw_offset[p];
+        const unsigned magic = plan.ns_magic[p];
+        switch (r) {
+            case 2: pass_ip<2>(buf, plan.n, ns, stw, magic); break;
+            case 3: pass_ip<3>(buf, plan.n, ns, stw, magic); break;
+            case 4: pass_ip<4>(buf, plan.n, ns, stw, magic); break;
+            case 5: pass_ip<5>(buf, plan.n, ns, stw, magic); break;
+            case 6: pass_ip<6>(buf, plan.n, ns, stw, magic); break;
+            case 8: pass_ip<8>(buf, plan.n, ns, stw, magic); break;
+            case 9: pass_ip<9>(buf, plan.n, ns, stw, magic); break;
+            case 10: pass_ip<10>(buf, plan.n, ns, stw, magic); break;
+            case 12: pass_ip<12>(buf, plan.n, ns, stw, magic); break;
+            case 15: pass_ip<15>(buf, plan.n, ns, stw, magic); break;
+            case 16: pass_ip<16>(buf, plan.n, ns, stw, magic); break;
+            case 20: pass_ip<20>(buf, plan.n, ns, stw, magic); break;
+            case 25: pass_ip<25>(buf, plan.n, ns, stw, magic); break;
+            default: break;  // not produced by the planner for in-place plans
+        }
+        ns *= r;
+    }
+}
+
+// Forward FFT with the plan's engine; returns the buffer holding the result.
+template <bool kIP>
+__device__ __forceinline__ float2* fft_line(float2* a, float2* b, const FftPlanDev& plan, const float2* table) {
+    if constexpr (kIP) {
+        fft_inplace(a, plan, table);
+        return a;
+    } else {
+        return fft_lds(a, b, plan, table);
+    }
+}
+
 // Whole-sample mirror with period 2 (n - 1), n >= 2 (reflect-101).
 __device__ __forceinline__ int reflect101(int i, int n) {
     if (i >= 0 && i < n) return i;
@@ -149,12 +379,13 @@ __device__ __forceinline__ int reflect101(int i, int n) {
 // Makhoul permutation index: position of x[i] in v.
 __device__ __forceinline__ int makhoul_pos(int i, int n) { return (i & 1) ? n - 1 - (i >> 1) : (i >> 1); }
 
-// DCT-II of two real lines packed as (re, im) of one complex line.
-// On return out_a / out_b (global) hold X_a[k], X_b[k].
+// DCT-II of two real lines packed as (re, im) of one complex line held in
+// LDS (Makhoul order). On return out_a / out_b (global) hold X_a[k], X_b[k].
+template <bool kIP>
 __device__ void dct2_pair(float2* a, float2* b, const FftPlanDev& plan, const float2* table, const float2* half_tw,
-                          float* out_a, float* out_b, float scale_a, float scale_b) {
+                          float* out_a, float* out_b) {
     const int n = plan.n;
-    float2* V = fft_lds(a, b, plan, table);
+    float2* V = fft_line<kIP>(a, b, plan, table);
     // V = Va + i Vb with Va, Vb spectra of real sequences.
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const float2 z = V[k];
@@ -162,9 +393,18 @@ __device__ void dct2_pair(float2* a, float2* b, const FftPlanDev& plan, const fl
         const float2 va = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
         const float2 vb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
         const float2 w = half_tw[k];  // exp(-i*pi*k/(2n))
-        if (out_a) out_a[k] = (w.x * va.x - w.y * va.y) * scale_a;
-        if (out_b) out_b[k] = (w.x * vb.x - w.y * vb.y) * scale_b;
+        out_a[k] = w.x * va.x - w.y * va.y;
+        if (out_b) out_b[k] = w.x * vb.x - w.y * vb.y;
     }
+}
+
+// Packed inverse input for one pair position: conj(e^{i pi k/2n}(X_a[k] - i X_a[n-k])
+// + i e^{i pi k/2n}(X_b[k] - i X_b[n-k])); xam / xbm are 0 for k = 0.
+__device__ __forceinline__ float2 dct3_pack(float xa, float xam, float xb, float xbm, float2 w) {
+    const float2 wc = make_float2(w.x, -w.y);
+    const float2 va = cmul(wc, make_float2(xa, -xam));
+    const float2 vb = cmul(wc, make_float2(xb, -xbm));
+    return make_float2(va.x - vb.y, -(va.y + vb.x));
 }
 
 }  // namespace
@@ -174,12 +414,13 @@ __device__ void dct2_pair(float2* a, float2* b, const FftPlanDev& plan, const fl
 // ---------------------------------------------------------------------------
 
 // Rows [2*blockIdx.x, 2*blockIdx.x+1] of each plane (blockIdx.y) -> DCT-II rows.
+template <bool kIP>
 __global__ __launch_bounds__(kMaxRowThreads) void dct2_rows_kernel(DctRowsArgs args) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
     const int n = plan.n;
     float2* a = lds2;
-    float2* b = lds2 + n;
+    float2* b = kIP ? nullptr : lds2 + n;
     const int row0 = 2 * blockIdx.x;
     const int plane = blockIdx.y;
     if (row0 >= args.rows) return;
@@ -193,32 +434,28 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct2_rows_kernel(DctRowsArgs a
             const int si = reflect101(i - args.pad, args.src_w);
             a[makhoul_pos(i, n)] = make_float2(r0[si], has1 ? r1[si] : 0.0f);
         }
-        __syncthreads();
-        dct2_pair(a, b, plan, args.table, args.half_tw, out + static_cast<long long>(row0) * args.out_stride,
-                  has1 ? out + static_cast<long long>(row0 + 1) * args.out_stride : nullptr, 1.0f, 1.0f);
-        return;
-    }
-    const float* r0 = in + static_cast<long long>(row0) * args.in_stride;
-    const float* r1 = has1 ? in + static_cast<long long>(row0 + 1) * args.in_stride : nullptr;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const float v0 = r0[i];
-        const float v1 = has1 ? r1[i] : 0.0f;
-        a[makhoul_pos(i, n)] = make_float2(v0, v1);
+    } else {
+        const float* r0 = in + static_cast<long long>(row0) * args.in_stride;
+        const float* r1 = has1 ? in + static_cast<long long>(row0 + 1) * args.in_stride : nullptr;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const float v0 = r0[i];
+            const float v1 = has1 ? r1[i] : 0.0f;
+            a[makhoul_pos(i, n)] = make_float2(v0, v1);
+        }
     }
     __syncthreads();
-    dct2_pair(a, b, plan, args.table, args.half_tw, out + static_cast<long long>(row0) * args.out_stride,
-              has1 ? out + static_cast<long long>(row0 + 1) * args.out_stride : nullptr, 1.0f, 1.0f);
+    dct2_pair<kIP>(a, b, plan, args.table, args.half_tw, out + static_cast<long long>(row0) * args.out_stride,
+                   has1 ? out + static_cast<long long>(row0 + 1) * args.out_stride : nullptr);
 }
 
-// Inverse (DCT-III scaled so that it undoes dct2) of row pairs, optionally
-// multiplied by a separable-sum transfer function first:
-//   T(k, row) = sum_c w_c * tk[c][k] * trow[c][row]
+// Inverse (DCT-III scaled so that it undoes dct2) of row pairs.
+template <bool kIP>
 __global__ __launch_bounds__(kMaxRowThreads) void dct3_rows_kernel(DctRowsArgs args) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
     const int n = plan.n;
     float2* a = lds2;
-    float2* b = lds2 + n;
+    float2* b = kIP ? nullptr : lds2 + n;
     const int row0 = args.row_begin + 2 * blockIdx.x;
     const int plane = blockIdx.y;
     if (row0 >= args.rows) return;
@@ -227,21 +464,14 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct3_rows_kernel(DctRowsArgs a
     float* out = args.out[plane];
     const float* r0 = in + static_cast<long long>(row0) * args.in_stride;
     const float* r1 = has1 ? in + static_cast<long long>(row0 + 1) * args.in_stride : r0;
-    // V_a[k] = e^{i pi k/2n} (X_a[k] - i X_a[n-k]); pack V = V_a + i V_b.
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const int km = k == 0 ? -1 : n - k;
         const float xa = r0[k], xam = km < 0 ? 0.0f : r0[km];
         const float xb = has1 ? r1[k] : 0.0f, xbm = (has1 && km >= 0) ? r1[km] : 0.0f;
-        const float2 w = args.half_tw[k];  // e^{-i pi k/2n}; use conj
-        const float2 wc = make_float2(w.x, -w.y);
-        const float2 va = cmul(wc, make_float2(xa, -xam));
-        const float2 vb = cmul(wc, make_float2(xb, -xbm));
-        // conj(V) for an inverse via the forward FFT: conj(Va + i Vb)
-        const float2 v = make_float2(va.x - vb.y, va.y + vb.x);
-        a[k] = make_float2(v.x, -v.y);
+        a[k] = dct3_pack(xa, xam, xb, xbm, args.half_tw[k]);
     }
     __syncthreads();
-    float2* res = fft_lds(a, b, plan, args.table);
+    float2* res = fft_line<kIP>(a, b, plan, args.table);
     const float inv = 1.0f / static_cast<float>(n);
     if (args.pad >= 0) {
         // Rows [pad, pad + src_h) of the padded plane hold the plane's rows.
@@ -267,18 +497,28 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct3_rows_kernel(DctRowsArgs a
     }
 }
 
+// Transfer value T(row, k) of one row of a filter launch.
+template <bool kFullTable>
+__device__ __forceinline__ float transfer_at(const DctTransfer& tf, const float* full, const float* f, int row, int n,
+                                             int k) {
+    if (kFullTable) return full[static_cast<long long>(row) * n + k];
+    float t = 0.0f;
+    for (int c = 0; c < tf.count; ++c) t = fmaf(f[c], tf.t_cols[c][k], t);
+    return t;
+}
+
 // Forward DCT-II along rows, multiply by T, inverse along rows, in one pass.
 // Used on the transposed plane (rows = original columns): the row index is the
 // first-axis frequency kx, the in-row index the second-axis frequency.
 // kFullTable: T is a full per-plane table (diffusion); otherwise a weighted sum
 // of separable component factors (spatial DIR).
-template <bool kFullTable>
+template <bool kFullTable, bool kIP>
 __global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRowsArgs args, DctTransfer tf) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
     const int n = plan.n;
     float2* a = lds2;
-    float2* b = lds2 + n;
+    float2* b = kIP ? nullptr : lds2 + n;
     const int row0 = 2 * blockIdx.x;
     const int plane = blockIdx.y;
     if (row0 >= args.rows) return;
@@ -301,43 +541,59 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRows
             fb[c] = has1 ? tf.weight[c] * tf.t_rows[c][row0 + 1] : 0.0f;
         }
     }
-    float2* V = fft_lds(a, b, plan, args.table);
-    float2* other = V == a ? b : a;
-    float* spec = reinterpret_cast<float*>(other);  // n floats X_a, then n floats X_b
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const float2 z = V[k];
-        const float2 zc = V[k == 0 ? 0 : n - k];
-        const float2 va = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
-        const float2 vb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
-        const float2 w = args.half_tw[k];
-        float ta = 0.0f, tb = 0.0f;
-        if (kFullTable) {
-            ta = full[static_cast<long long>(row0) * n + k];
-            tb = has1 ? full[static_cast<long long>(row0 + 1) * n + k] : 0.0f;
-        } else {
-            for (int c = 0; c < tf.count; ++c) {
-                const float tk = tf.t_cols[c][k];
-                ta = fmaf(fa[c], tk, ta);
-                tb = fmaf(fb[c], tk, tb);
+    const int row1 = has1 ? row0 + 1 : row0;
+    float2* V = fft_line<kIP>(a, b, plan, args.table);
+    if constexpr (kIP) {
+        // Spectrum product and inverse packing, in place: positions k and
+        // n - k depend only on V[k] and V[n - k], so each thread owns a pair.
+        for (int p = threadIdx.x; p <= n / 2; p += blockDim.x) {
+            const int k1 = p, k2 = p == 0 ? 0 : n - p;
+            const float2 z1 = V[k1], z2 = V[k2];
+            const float2 w1 = args.half_tw[k1], w2 = args.half_tw[k2];
+            // DCT-II coefficients of both lines at k1 and k2, times T.
+            const float2 va1 = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+            const float2 vb1 = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
+            const float2 va2 = make_float2(0.5f * (z2.x + z1.x), 0.5f * (z2.y - z1.y));
+            const float2 vb2 = make_float2(0.5f * (z2.y + z1.y), -0.5f * (z2.x - z1.x));
+            const float ta1 = transfer_at<kFullTable>(tf, full, fa, row0, n, k1);
+            const float ta2 = transfer_at<kFullTable>(tf, full, fa, row0, n, k2);
+            const float tb1 = has1 ? transfer_at<kFullTable>(tf, full, fb, row1, n, k1) : 0.0f;
+            const float tb2 = has1 ? transfer_at<kFullTable>(tf, full, fb, row1, n, k2) : 0.0f;
+            const float xa1 = (w1.x * va1.x - w1.y * va1.y) * ta1;
+            const float xb1 = (w1.x * vb1.x - w1.y * vb1.y) * tb1;
+            const float xa2 = (w2.x * va2.x - w2.y * va2.y) * ta2;
+            const float xb2 = (w2.x * vb2.x - w2.y * vb2.y) * tb2;
+            if (p == 0) {
+                V[0] = dct3_pack(xa1, 0.0f, xb1, 0.0f, w1);
+            } else {
+                V[k1] = dct3_pack(xa1, xa2, xb1, xb2, w1);
+                if (k2 != k1) V[k2] = dct3_pack(xa2, xa1, xb2, xb1, w2);
             }
         }
-        spec[k] = (w.x * va.x - w.y * va.y) * ta;
-        spec[n + k] = (w.x * vb.x - w.y * vb.y) * tb;
+    } else {
+        float2* other = V == a ? b : a;
+        float* spec = reinterpret_cast<float*>(other);  // n floats X_a, then n floats X_b
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            const float2 z = V[k];
+            const float2 zc = V[k == 0 ? 0 : n - k];
+            const float2 va = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+            const float2 vb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+            const float2 w = args.half_tw[k];
+            const float ta = transfer_at<kFullTable>(tf, full, fa, row0, n, k);
+            const float tb = has1 ? transfer_at<kFullTable>(tf, full, fb, row1, n, k) : 0.0f;
+            spec[k] = (w.x * va.x - w.y * va.y) * ta;
+            spec[n + k] = (w.x * vb.x - w.y * vb.y) * tb;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            const int km = k == 0 ? -1 : n - k;
+            const float xam = km < 0 ? 0.0f : spec[km];
+            const float xbm = km < 0 ? 0.0f : spec[n + km];
+            V[k] = dct3_pack(spec[k], xam, spec[n + k], xbm, args.half_tw[k]);
+        }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const int km = k == 0 ? -1 : n - k;
-        const float xa = spec[k], xam = km < 0 ? 0.0f : spec[km];
-        const float xb = spec[n + k], xbm = km < 0 ? 0.0f : spec[n + km];
-        const float2 w = args.half_tw[k];
-        const float2 wc = make_float2(w.x, -w.y);
-        const float2 va = cmul(wc, make_float2(xa, -xam));
-        const float2 vb = cmul(wc, make_float2(xb, -xbm));
-        const float2 v = make_float2(va.x - vb.y, va.y + vb.x);
-        V[k] = make_float2(v.x, -v.y);
-    }
-    __syncthreads();
-    float2* res = fft_lds(V, other, plan, args.table);
+    float2* res = fft_line<kIP>(V, kIP ? nullptr : (V == a ? b : a), plan, args.table);
     const float inv = 1.0f / static_cast<float>(n);
     float* o0 = out + static_cast<long long>(row0) * args.out_stride;
     float* o1 = has1 ? out + static_cast<long long>(row0 + 1) * args.out_stride : nullptr;
@@ -370,38 +626,54 @@ __global__ __launch_bounds__(256) void transpose_kernel(TransposeArgs t) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-size_t dct_lds_bytes(int n, bool) { return static_cast<size_t>(n) * sizeof(float2) * 2; }
+size_t dct_lds_bytes(const FftPlanDev& plan) {
+    const size_t n = static_cast<size_t>(plan.n);
+    return (plan.inplace ? n : 2 * n) * sizeof(float2);
+}
 
-// Threads per line-pair block: about one radix-4 butterfly per thread, whole waves.
-static int row_threads(int n) {
-    const int t = (n / 4 + 63) / 64 * 64;
+// Threads per line-pair block: the in-place plan's butterfly count, else
+// about one radix-4 butterfly per thread; whole waves.
+static int row_threads(const FftPlanDev& plan) {
+    if (plan.inplace) return plan.threads;
+    const int t = (plan.n / 4 + 63) / 64 * 64;
     return t < 64 ? 64 : (t > kMaxRowThreads ? kMaxRowThreads : t);
 }
 
 hipError_t launch_dct2_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
-    const dim3 grid((a.rows + 1) / 2, planes);
-    hipLaunchKernelGGL(dct2_rows_kernel, grid, dim3(row_threads(a.plan.n)), dct_lds_bytes(a.plan.n, false), s, a);
+    const dim3 grid((a.rows + 1) / 2, planes), block(row_threads(a.plan));
+    const size_t lds = dct_lds_bytes(a.plan);
+    if (a.plan.inplace)
+        hipLaunchKernelGGL(dct2_rows_kernel<true>, grid, block, lds, s, a);
+    else
+        hipLaunchKernelGGL(dct2_rows_kernel<false>, grid, block, lds, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_dct3_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
     if (a.rows <= a.row_begin) return hipSuccess;
-    const dim3 grid((a.rows - a.row_begin + 1) / 2, planes);
-    hipLaunchKernelGGL(dct3_rows_kernel, grid, dim3(row_threads(a.plan.n)), dct_lds_bytes(a.plan.n, false), s, a);
+    const dim3 grid((a.rows - a.row_begin + 1) / 2, planes), block(row_threads(a.plan));
+    const size_t lds = dct_lds_bytes(a.plan);
+    if (a.plan.inplace)
+        hipLaunchKernelGGL(dct3_rows_kernel<true>, grid, block, lds, s, a);
+    else
+        hipLaunchKernelGGL(dct3_rows_kernel<false>, grid, block, lds, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_dct_filter_rows(const DctRowsArgs& a, const DctTransfer& tf, int planes, hipStream_t s) {
-    const dim3 grid((a.rows + 1) / 2, planes);
-    if (tf.full[0]) {
-        for (int p = 1; p < planes; ++p)
-            if (!tf.full[p]) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(dct_filter_rows_kernel<true>, grid, dim3(row_threads(a.plan.n)), dct_lds_bytes(a.plan.n, true),
-                           s, a, tf);
-    } else {
-        hipLaunchKernelGGL(dct_filter_rows_kernel<false>, grid, dim3(row_threads(a.plan.n)),
-                           dct_lds_bytes(a.plan.n, true), s, a, tf);
-    }
+    const dim3 grid((a.rows + 1) / 2, planes), block(row_threads(a.plan));
+    const size_t lds = dct_lds_bytes(a.plan);
+    const bool full = tf.full[0] != nullptr;
+    for (int p = 1; p < planes; ++p)
+        if ((tf.full[p] != nullptr) != full) return hipErrorInvalidValue;
+    if (full && a.plan.inplace)
+        hipLaunchKernelGGL((dct_filter_rows_kernel<true, true>), grid, block, lds, s, a, tf);
+    else if (full)
+        hipLaunchKernelGGL((dct_filter_rows_kernel<true, false>), grid, block, lds, s, a, tf);
+    else if (a.plan.inplace)
+        hipLaunchKernelGGL((dct_filter_rows_kernel<false, true>), grid, block, lds, s, a, tf);
+    else
+        hipLaunchKernelGGL((dct_filter_rows_kernel<false, false>), grid, block, lds, s, a, tf);
     return hipGetLastError();
 }
 
@@ -417,12 +689,17 @@ namespace fjx {
 
 hipError_t fft_kernels_init(int* max_lds_n) {
     // Line transforms beyond 4096 points need more than the default 64 KiB of
-    // LDS; fall back to 4096-point lines if the runtime refuses the request.
-    const int bytes = static_cast<int>(dct_lds_bytes(kMaxLdsFft, true));
-    const void* kernels[4] = {reinterpret_cast<const void*>(dct2_rows_kernel),
-                              reinterpret_cast<const void*>(dct3_rows_kernel),
-                              reinterpret_cast<const void*>(dct_filter_rows_kernel<false>),
-                              reinterpret_cast<const void*>(dct_filter_rows_kernel<true>)};
+    // LDS (the ping-pong engine holds two lines); fall back to 4096-point
+    // lines if the runtime refuses the request.
+    const int bytes = static_cast<int>(2 * kMaxLdsFft * sizeof(float2));
+    const void* kernels[8] = {reinterpret_cast<const void*>(dct2_rows_kernel<false>),
+                              reinterpret_cast<const void*>(dct2_rows_kernel<true>),
+                              reinterpret_cast<const void*>(dct3_rows_kernel<false>),
+                              reinterpret_cast<const void*>(dct3_rows_kernel<true>),
+                              reinterpret_cast<const void*>(dct_filter_rows_kernel<false, false>),
+                              reinterpret_cast<const void*>(dct_filter_rows_kernel<false, true>),
+                              reinterpret_cast<const void*>(dct_filter_rows_kernel<true, false>),
+                              reinterpret_cast<const void*>(dct_filter_rows_kernel<true, true>)};
     *max_lds_n = kMaxLdsFft;
     for (const void* k : kernels) {
         if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {

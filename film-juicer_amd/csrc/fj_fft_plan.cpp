@@ -1,34 +1,109 @@
 // fj_fft_plan.cpp — host planning for the LDS Stockham FFT: factorization
 // into butterfly radices and double-precision twiddle tables.
+#include <algorithm>
 #include <cmath>
+#include <map>
 #include <vector>
 
 #include "fj_fft.h"
 
 namespace fjx {
 
+namespace {
+
+struct Factoring {
+    bool ok = false;
+    int passes = 0;
+    int min_radix = 0;
+    std::vector<int> radices;
+};
+
+// Composite radices of the in-place engine (largest first).
+constexpr int kInplaceRadices[] = {25, 20, 16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+
+// Fewest passes, then the largest smallest radix, over factorings of m into
+// in-place radices no smaller than rmin.
+Factoring best_factoring(int m, int rmin, std::map<int, Factoring>& memo) {
+    if (m == 1) {
+        Factoring f;
+        f.ok = true;
+        f.min_radix = 1 << 30;
+        return f;
+    }
+    auto it = memo.find(m);
+    if (it != memo.end()) return it->second;
+    Factoring best;
+    for (int r : kInplaceRadices) {
+        if (r < rmin || m % r != 0) continue;
+        const Factoring sub = best_factoring(m / r, rmin, memo);
+        if (!sub.ok) continue;
+        const int passes = sub.passes + 1;
+        const int min_radix = std::min(sub.min_radix, r);
+        if (!best.ok || passes < best.passes || (passes == best.passes && min_radix > best.min_radix)) {
+            best.ok = true;
+            best.passes = passes;
+            best.min_radix = min_radix;
+            best.radices = sub.radices;
+            best.radices.push_back(r);
+        }
+    }
+    memo.emplace(m, best);
+    return best;
+}
+
+bool inplace_radices(int n, std::vector<int>& radices, int& threads) {
+    int m = n;
+    for (int p : {2, 3, 5})
+        while (m % p == 0) m /= p;
+    if (m != 1) return false;
+    std::map<int, Factoring> memo;
+    const int rmin = (n + kMaxRowThreads - 1) / kMaxRowThreads;
+    const Factoring f = best_factoring(n, rmin, memo);
+    if (!f.ok) return false;
+    radices = f.radices;
+    // Odd radices first: a first pass of odd radix R writes with an odd
+    // stride, which spreads a wave's stores over the LDS banks.
+    std::sort(radices.begin(), radices.end(), [](int a, int b) {
+        if ((a & 1) != (b & 1)) return (a & 1) != 0;
+        return a > b;
+    });
+    int count = 1;
+    for (int r : radices) count = std::max(count, n / r);
+    threads = std::max(64, (count + 63) / 64 * 64);
+    return threads <= kMaxRowThreads;
+}
+
+}  // namespace
+
 bool make_fft_plan(int n, FftPlanDev& plan) {
     plan = FftPlanDev{};
     if (n < 1 || n > kMaxLdsFft) return false;
     plan.n = n;
-    int m = n;
     std::vector<int> radices;
-    while (m % 4 == 0) {
-        radices.push_back(4);
-        m /= 4;
-    }
-    while (m % 2 == 0) {
-        radices.push_back(2);
-        m /= 2;
-    }
-    for (int p = 3; p * p <= m || p <= m; p += 2) {
-        while (m % p == 0) {
-            radices.push_back(p);
-            m /= p;
+    int threads = 0;
+    if (inplace_radices(n, radices, threads)) {
+        plan.inplace = 1;
+        plan.threads = threads;
+    } else {
+        radices.clear();
+        int m = n;
+        while (m % 4 == 0) {
+            radices.push_back(4);
+            m /= 4;
         }
-        if (m == 1) break;
+        while (m % 2 == 0) {
+            radices.push_back(2);
+            m /= 2;
+        }
+        for (int p = 3; p * p <= m || p <= m; p += 2) {
+            while (m % p == 0) {
+                radices.push_back(p);
+                m /= p;
+            }
+            if (m == 1) break;
+        }
+        if (m != 1) radices.push_back(m);
     }
-    if (m != 1) radices.push_back(m);
     if (static_cast<int>(radices.size()) > kMaxFftPasses) return false;
     plan.passes = static_cast<int>(radices.size());
     int ns = 1, offset = 0;
