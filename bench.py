@@ -35,8 +35,8 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the diffusion column filter (dct_filter_rows_kernel<true>)
 # at this workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
 # scaled by the factors a plain copy kernel of known size reads (x2.0 fetch, x1.0 write).
-FILTER_FETCH_KIB = 253442.9
-FILTER_WRITE_KIB = 253125.0
+FILTER_FETCH_KIB = 254183.3
+FILTER_WRITE_KIB = 142383.8
 
 
 def smooth_length(n: int) -> int:
@@ -235,14 +235,15 @@ def main():
         return sum(v for k, v in stage_ms.items() if k == prefix or k.startswith(prefix + "."))
 
     # Dominant kernel: the diffusion column filter, one launch per diffusion
-    # stage over three padded planes (forward DCT-II along padded columns,
-    # x T(u, v), inverse). Algorithmic bytes per launch: read the transposed
-    # padded plane (4 B), read the transfer table (4 B), write the plane (4 B)
-    # per padded sample and channel.
+    # stage over three padded planes (forward DCT-II along the pw padded
+    # columns of ph samples, x T(u, v), inverse, keeping the frame's HEIGHT
+    # rows). Algorithmic bytes per launch and channel: read the transposed
+    # padded plane (4 B) and the transfer table (4 B) per padded sample, write
+    # 4 B per kept sample.
     filt = [stage_ms[k] for k in ("camera_diffusion.filter", "enlarger_diffusion.filter") if k in stage_ms]
     r_px = p.record.optics.camera.radius_px
     pw, ph = smooth_length(WIDTH + 2 * r_px), smooth_length(HEIGHT + 2 * r_px)
-    filter_bytes = 3 * 12 * pw * ph
+    filter_bytes = 3 * (8 * pw * ph + 4 * pw * HEIGHT)
     filter_ms = float(np.mean(filt))
     filter_gbs = filter_bytes / (filter_ms * 1e-3) / 1e9
     traffic = None
@@ -298,8 +299,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "dct_filter_rows_kernel<true> (diffusion column filter: DCT-II, x T, DCT-III along "
-                      f"padded columns; 3 planes of {pw}x{ph})",
+            "kernel": "dct_filter_rows_kernel<true, 16> (diffusion column filter: DCT-II, x T, DCT-III along "
+                      f"the padded columns of 3 planes of {pw}x{ph}, keeping {HEIGHT} rows)",
             "achieved": filter_gbs,
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",

@@ -328,69 +328,78 @@ hipError_t run_diffusion(DeviceState& dev, const DiffusionStage& st, float* cons
         ext[c] = padded + c * pplane;
         tr[c] = padded + (3 + c) * pplane;
     }
-    // 1) reflect-101 extension + DCT-II along padded rows: planes -> ext
+    // 1) DCT-II of the plane's own rows along their reflect-101 extension to
+    //    pw samples: planes (height x width) -> ext (height x pw). The padded
+    //    plane's other rows are reflections of these, so they are not
+    //    transformed twice.
     DctRowsArgs a{};
     hipError_t e = fft_tables(dev, pw, a);
     if (e != hipSuccess) return e;
-    a.rows = ph;
+    a.rows = height;
     a.in_stride = width;
     a.out_stride = pw;
     a.pad = r;
     a.src_w = width;
-    a.src_h = height;
     for (int c = 0; c < 3; ++c) {
         a.in[c] = planes[c];
         a.out[c] = ext[c];
     }
     e = launch_dct2_rows(a, 3, stream);
     if (e != hipSuccess) return e;
-    // 2) ext (ph x pw) -> tr (pw x ph)
+    // 2) ext -> tr (pw x ph): column u of the padded plane, its rows gathered
+    //    through the reflect-101 row extension.
     TransposeArgs t{};
     t.width = pw;
     t.height = ph;
     t.in_stride = pw;
     t.out_stride = ph;
+    t.src_rows = height;
+    t.row_pad = r;
     for (int c = 0; c < 3; ++c) {
         t.in[c] = ext[c];
         t.out[c] = tr[c];
     }
     e = launch_transpose(t, 3, stream);
     if (e != hipSuccess) return e;
-    // 3) along padded columns: DCT-II, x T, inverse
+    // 3) along padded columns: DCT-II, x T, inverse; only the frame's rows
+    //    [r, r + height) are kept: tr -> ext (pw x height).
     DctRowsArgs b{};
     e = fft_tables(dev, ph, b);
     if (e != hipSuccess) return e;
     b.rows = pw;
     b.in_stride = ph;
-    b.out_stride = ph;
+    b.out_stride = height;
+    b.out_begin = r;
+    b.out_count = height;
     for (int c = 0; c < 3; ++c) {
         b.in[c] = tr[c];
-        b.out[c] = tr[c];
+        b.out[c] = ext[c];
     }
     if (mark) mark(1);
     e = launch_dct_filter_rows(b, st.transfer, 3, stream);
     if (e != hipSuccess) return e;
     if (mark) mark(2);
-    // 4) tr -> ext
-    t.width = ph;
+    // 4) ext (pw x height) -> tr (height x pw)
+    t = TransposeArgs{};
+    t.width = height;
     t.height = pw;
-    t.in_stride = ph;
+    t.in_stride = height;
     t.out_stride = pw;
     for (int c = 0; c < 3; ++c) {
-        t.in[c] = tr[c];
-        t.out[c] = ext[c];
+        t.in[c] = ext[c];
+        t.out[c] = tr[c];
     }
     e = launch_transpose(t, 3, stream);
     if (e != hipSuccess) return e;
-    // 5) inverse along the frame's rows of the padded plane, blended into the planes
+    // 5) inverse along the frame's rows, cropped to their own samples and
+    //    blended into the planes
     DctRowsArgs o = a;
-    o.row_begin = r;
-    o.rows = r + height;
+    o.rows = height;
     o.in_stride = pw;
     o.out_stride = width;
     o.fraction = st.fraction;
     for (int c = 0; c < 3; ++c) {
-        o.in[c] = ext[c];
+        o.in[c] = tr[c];
         o.out[c] = planes[c];
         o.mix[c] = planes[c];
     }

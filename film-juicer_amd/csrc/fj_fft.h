@@ -40,15 +40,20 @@ struct DctRowsArgs {
     int row_begin = 0;
     long long in_stride = 0;
     long long out_stride = 0;
-    // Padded-plane mode (pad >= 0) for a src_w x src_h plane:
-    //  forward rows read the reflect-101 extension: padded row y, element i
-    //    <- in[reflect101(y - pad, src_h) * in_stride + reflect101(i - pad, src_w)];
-    //  inverse rows keep only the plane's own samples and blend them with it:
-    //    out[(y - pad) * out_stride + i - pad] = (1 - fraction) * mix[same] + fraction * value.
+    // Padded-line mode (pad >= 0) for rows of src_w samples:
+    //  forward rows read the reflect-101 extension of row y, element i
+    //    <- in[y * in_stride + reflect101(i - pad, src_w)];
+    //  inverse rows keep only the row's own samples and blend them with it:
+    //    out[y * out_stride + i - pad] = (1 - fraction) * mix[same] + fraction * value.
     int pad = -1;
-    int src_w = 0, src_h = 0;
+    int src_w = 0;
     const float* mix[3] = {};
     float fraction = 0.0f;
+    // Filter rows: only elements [out_begin, out_begin + out_count) of each
+    // transformed row are written, at out[row * out_stride + i - out_begin]
+    // (out_count 0: the whole row at offset 0).
+    int out_begin = 0;
+    int out_count = 0;
 };
 
 // Separable-sum transfer function T(row, k) = sum_c weight[c] * t_rows[c][row] * t_cols[c][k],
@@ -69,6 +74,10 @@ struct TransposeArgs {
     int height = 0;  // of the input
     long long in_stride = 0;
     long long out_stride = 0;
+    // src_rows > 0: input row y is read from row reflect101(y - row_pad,
+    // src_rows) (the reflect-101 row extension of a src_rows-row plane).
+    int src_rows = 0;
+    int row_pad = 0;
 };
 
 // Host-side plan. A 2,3,5-smooth n gets the in-place engine's composite

@@ -328,8 +328,10 @@ __device__ __forceinline__ void pass_ip(float2* buf, int n, int ns, const float2
     __syncthreads();
 }
 
-// Forward FFT of buf[0..n) in place. blockDim.x >= n / R for
-// every radix of the plan (the plan's thread count).
+// Forward FFT of buf[0..n) in place. blockDim.x >= n / R for every radix of
+// the plan (the plan's thread count); kMaxR bounds the plan's radices, so a
+// plan without 20 or 25 runs code with fewer live registers.
+template <int kMaxR>
 __device__ void fft_inplace(float2* buf, const FftPlanDev& plan, const float2* table) {
     int ns = 1;
     for (int p = 0; p < plan.passes; ++p) {
@@ -348,19 +350,28 @@ __device__ void fft_inplace(float2* buf, const FftPlanDev& plan, const float2* t
             case 12: pass_ip<12>(buf, plan.n, ns, stw, magic); break;
             case 15: pass_ip<15>(buf, plan.n, ns, stw, magic); break;
             case 16: pass_ip<16>(buf, plan.n, ns, stw, magic); break;
-            case 20: pass_ip<20>(buf, plan.n, ns, stw, magic); break;
-            case 25: pass_ip<25>(buf, plan.n, ns, stw, magic); break;
+            case 20:
+                if constexpr (kMaxR >= 20) pass_ip<20>(buf, plan.n, ns, stw, magic);
+                break;
+            case 25:
+                if constexpr (kMaxR >= 25) pass_ip<25>(buf, plan.n, ns, stw, magic);
+                break;
             default: break;  // not produced by the planner for in-place plans
         }
         ns *= r;
     }
 }
 
+// Engines: 0 = ping-pong over two buffers; 16 / 25 = in place with radices up
+// to 16 / 25.
+template <int kEng>
+constexpr bool kInplace = kEng != 0;
+
 // Forward FFT with the plan's engine; returns the buffer holding the result.
-template <bool kIP>
+template <int kEng>
 __device__ __forceinline__ float2* fft_line(float2* a, float2* b, const FftPlanDev& plan, const float2* table) {
-    if constexpr (kIP) {
-        fft_inplace(a, plan, table);
+    if constexpr (kInplace<kEng>) {
+        fft_inplace<kEng>(a, plan, table);
         return a;
     } else {
         return fft_lds(a, b, plan, table);
@@ -381,11 +392,11 @@ __device__ __forceinline__ int makhoul_pos(int i, int n) { return (i & 1) ? n - 
 
 // DCT-II of two real lines packed as (re, im) of one complex line held in
 // LDS (Makhoul order). On return out_a / out_b (global) hold X_a[k], X_b[k].
-template <bool kIP>
+template <int kEng>
 __device__ void dct2_pair(float2* a, float2* b, const FftPlanDev& plan, const float2* table, const float2* half_tw,
                           float* out_a, float* out_b) {
     const int n = plan.n;
-    float2* V = fft_line<kIP>(a, b, plan, table);
+    float2* V = fft_line<kEng>(a, b, plan, table);
     // V = Va + i Vb with Va, Vb spectra of real sequences.
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const float2 z = V[k];
@@ -414,13 +425,13 @@ __device__ __forceinline__ float2 dct3_pack(float xa, float xam, float xb, float
 // ---------------------------------------------------------------------------
 
 // Rows [2*blockIdx.x, 2*blockIdx.x+1] of each plane (blockIdx.y) -> DCT-II rows.
-template <bool kIP>
+template <int kEng>
 __global__ __launch_bounds__(kMaxRowThreads) void dct2_rows_kernel(DctRowsArgs args) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
     const int n = plan.n;
     float2* a = lds2;
-    float2* b = kIP ? nullptr : lds2 + n;
+    float2* b = kInplace<kEng> ? nullptr : lds2 + n;
     const int row0 = 2 * blockIdx.x;
     const int plane = blockIdx.y;
     if (row0 >= args.rows) return;
@@ -428,8 +439,8 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct2_rows_kernel(DctRowsArgs a
     const float* in = args.in[plane];
     float* out = args.out[plane];
     if (args.pad >= 0) {
-        const float* r0 = in + static_cast<long long>(reflect101(row0 - args.pad, args.src_h)) * args.in_stride;
-        const float* r1 = in + static_cast<long long>(reflect101(row0 + 1 - args.pad, args.src_h)) * args.in_stride;
+        const float* r0 = in + static_cast<long long>(row0) * args.in_stride;
+        const float* r1 = has1 ? r0 + args.in_stride : r0;
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             const int si = reflect101(i - args.pad, args.src_w);
             a[makhoul_pos(i, n)] = make_float2(r0[si], has1 ? r1[si] : 0.0f);
@@ -444,18 +455,18 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct2_rows_kernel(DctRowsArgs a
         }
     }
     __syncthreads();
-    dct2_pair<kIP>(a, b, plan, args.table, args.half_tw, out + static_cast<long long>(row0) * args.out_stride,
+    dct2_pair<kEng>(a, b, plan, args.table, args.half_tw, out + static_cast<long long>(row0) * args.out_stride,
                    has1 ? out + static_cast<long long>(row0 + 1) * args.out_stride : nullptr);
 }
 
 // Inverse (DCT-III scaled so that it undoes dct2) of row pairs.
-template <bool kIP>
+template <int kEng>
 __global__ __launch_bounds__(kMaxRowThreads) void dct3_rows_kernel(DctRowsArgs args) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
     const int n = plan.n;
     float2* a = lds2;
-    float2* b = kIP ? nullptr : lds2 + n;
+    float2* b = kInplace<kEng> ? nullptr : lds2 + n;
     const int row0 = args.row_begin + 2 * blockIdx.x;
     const int plane = blockIdx.y;
     if (row0 >= args.rows) return;
@@ -471,13 +482,13 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct3_rows_kernel(DctRowsArgs a
         a[k] = dct3_pack(xa, xam, xb, xbm, args.half_tw[k]);
     }
     __syncthreads();
-    float2* res = fft_line<kIP>(a, b, plan, args.table);
+    float2* res = fft_line<kEng>(a, b, plan, args.table);
     const float inv = 1.0f / static_cast<float>(n);
     if (args.pad >= 0) {
-        // Rows [pad, pad + src_h) of the padded plane hold the plane's rows.
+        // Elements [pad, pad + src_w) of the padded line are the row's own.
         const int pad = args.pad;
         const float f = args.fraction, keep = 1.0f - args.fraction;
-        const long long y0 = row0 - pad;
+        const long long y0 = row0;
         const float* m = args.mix[plane];
         for (int i = threadIdx.x; i < args.src_w; i += blockDim.x) {
             const float2 z = res[makhoul_pos(i + pad, n)];
@@ -512,13 +523,13 @@ __device__ __forceinline__ float transfer_at(const DctTransfer& tf, const float*
 // first-axis frequency kx, the in-row index the second-axis frequency.
 // kFullTable: T is a full per-plane table (diffusion); otherwise a weighted sum
 // of separable component factors (spatial DIR).
-template <bool kFullTable, bool kIP>
+template <bool kFullTable, int kEng>
 __global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRowsArgs args, DctTransfer tf) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
     const int n = plan.n;
     float2* a = lds2;
-    float2* b = kIP ? nullptr : lds2 + n;
+    float2* b = kInplace<kEng> ? nullptr : lds2 + n;
     const int row0 = 2 * blockIdx.x;
     const int plane = blockIdx.y;
     if (row0 >= args.rows) return;
@@ -542,8 +553,8 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRows
         }
     }
     const int row1 = has1 ? row0 + 1 : row0;
-    float2* V = fft_line<kIP>(a, b, plan, args.table);
-    if constexpr (kIP) {
+    float2* V = fft_line<kEng>(a, b, plan, args.table);
+    if constexpr (kInplace<kEng>) {
         // Spectrum product and inverse packing, in place: positions k and
         // n - k depend only on V[k] and V[n - k], so each thread owns a pair.
         for (int p = threadIdx.x; p <= n / 2; p += blockDim.x) {
@@ -593,11 +604,12 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRows
         }
     }
     __syncthreads();
-    float2* res = fft_line<kIP>(V, kIP ? nullptr : (V == a ? b : a), plan, args.table);
+    float2* res = fft_line<kEng>(V, kInplace<kEng> ? nullptr : (V == a ? b : a), plan, args.table);
     const float inv = 1.0f / static_cast<float>(n);
-    float* o0 = out + static_cast<long long>(row0) * args.out_stride;
-    float* o1 = has1 ? out + static_cast<long long>(row0 + 1) * args.out_stride : nullptr;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int begin = args.out_begin, count = args.out_count > 0 ? args.out_count : n;
+    float* o0 = out + static_cast<long long>(row0) * args.out_stride - begin;
+    float* o1 = has1 ? o0 + args.out_stride : nullptr;
+    for (int i = begin + threadIdx.x; i < begin + count; i += blockDim.x) {
         const float2 z = res[makhoul_pos(i, n)];
         o0[i] = z.x * inv;
         if (o1) o1[i] = -z.y * inv;
@@ -614,7 +626,10 @@ __global__ __launch_bounds__(256) void transpose_kernel(TransposeArgs t) {
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
     for (int j = ty; j < 64; j += 4) {
         const int x = bx + tx, y = by + j;
-        if (x < t.width && y < t.height) tile[j][tx] = in[static_cast<long long>(y) * t.in_stride + x];
+        if (x < t.width && y < t.height) {
+            const int sy = t.src_rows > 0 ? reflect101(y - t.row_pad, t.src_rows) : y;
+            tile[j][tx] = in[static_cast<long long>(sy) * t.in_stride + x];
+        }
     }
     __syncthreads();
     for (int j = ty; j < 64; j += 4) {
@@ -639,13 +654,21 @@ static int row_threads(const FftPlanDev& plan) {
     return t < 64 ? 64 : (t > kMaxRowThreads ? kMaxRowThreads : t);
 }
 
+static int engine(const FftPlanDev& plan) {
+    if (!plan.inplace) return 0;
+    int max_r = 0;
+    for (int p = 0; p < plan.passes; ++p) max_r = plan.radix[p] > max_r ? plan.radix[p] : max_r;
+    return max_r <= 16 ? 16 : 25;
+}
+
 hipError_t launch_dct2_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
     const dim3 grid((a.rows + 1) / 2, planes), block(row_threads(a.plan));
     const size_t lds = dct_lds_bytes(a.plan);
-    if (a.plan.inplace)
-        hipLaunchKernelGGL(dct2_rows_kernel<true>, grid, block, lds, s, a);
-    else
-        hipLaunchKernelGGL(dct2_rows_kernel<false>, grid, block, lds, s, a);
+    switch (engine(a.plan)) {
+        case 16: hipLaunchKernelGGL(dct2_rows_kernel<16>, grid, block, lds, s, a); break;
+        case 25: hipLaunchKernelGGL(dct2_rows_kernel<25>, grid, block, lds, s, a); break;
+        default: hipLaunchKernelGGL(dct2_rows_kernel<0>, grid, block, lds, s, a); break;
+    }
     return hipGetLastError();
 }
 
@@ -653,11 +676,22 @@ hipError_t launch_dct3_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
     if (a.rows <= a.row_begin) return hipSuccess;
     const dim3 grid((a.rows - a.row_begin + 1) / 2, planes), block(row_threads(a.plan));
     const size_t lds = dct_lds_bytes(a.plan);
-    if (a.plan.inplace)
-        hipLaunchKernelGGL(dct3_rows_kernel<true>, grid, block, lds, s, a);
-    else
-        hipLaunchKernelGGL(dct3_rows_kernel<false>, grid, block, lds, s, a);
+    switch (engine(a.plan)) {
+        case 16: hipLaunchKernelGGL(dct3_rows_kernel<16>, grid, block, lds, s, a); break;
+        case 25: hipLaunchKernelGGL(dct3_rows_kernel<25>, grid, block, lds, s, a); break;
+        default: hipLaunchKernelGGL(dct3_rows_kernel<0>, grid, block, lds, s, a); break;
+    }
     return hipGetLastError();
+}
+
+template <bool kFull>
+static void launch_filter(const DctRowsArgs& a, const DctTransfer& tf, dim3 grid, dim3 block, size_t lds,
+                          hipStream_t s) {
+    switch (engine(a.plan)) {
+        case 16: hipLaunchKernelGGL((dct_filter_rows_kernel<kFull, 16>), grid, block, lds, s, a, tf); break;
+        case 25: hipLaunchKernelGGL((dct_filter_rows_kernel<kFull, 25>), grid, block, lds, s, a, tf); break;
+        default: hipLaunchKernelGGL((dct_filter_rows_kernel<kFull, 0>), grid, block, lds, s, a, tf); break;
+    }
 }
 
 hipError_t launch_dct_filter_rows(const DctRowsArgs& a, const DctTransfer& tf, int planes, hipStream_t s) {
@@ -666,14 +700,10 @@ hipError_t launch_dct_filter_rows(const DctRowsArgs& a, const DctTransfer& tf, i
     const bool full = tf.full[0] != nullptr;
     for (int p = 1; p < planes; ++p)
         if ((tf.full[p] != nullptr) != full) return hipErrorInvalidValue;
-    if (full && a.plan.inplace)
-        hipLaunchKernelGGL((dct_filter_rows_kernel<true, true>), grid, block, lds, s, a, tf);
-    else if (full)
-        hipLaunchKernelGGL((dct_filter_rows_kernel<true, false>), grid, block, lds, s, a, tf);
-    else if (a.plan.inplace)
-        hipLaunchKernelGGL((dct_filter_rows_kernel<false, true>), grid, block, lds, s, a, tf);
+    if (full)
+        launch_filter<true>(a, tf, grid, block, lds, s);
     else
-        hipLaunchKernelGGL((dct_filter_rows_kernel<false, false>), grid, block, lds, s, a, tf);
+        launch_filter<false>(a, tf, grid, block, lds, s);
     return hipGetLastError();
 }
 
@@ -692,14 +722,18 @@ hipError_t fft_kernels_init(int* max_lds_n) {
     // LDS (the ping-pong engine holds two lines); fall back to 4096-point
     // lines if the runtime refuses the request.
     const int bytes = static_cast<int>(2 * kMaxLdsFft * sizeof(float2));
-    const void* kernels[8] = {reinterpret_cast<const void*>(dct2_rows_kernel<false>),
-                              reinterpret_cast<const void*>(dct2_rows_kernel<true>),
-                              reinterpret_cast<const void*>(dct3_rows_kernel<false>),
-                              reinterpret_cast<const void*>(dct3_rows_kernel<true>),
-                              reinterpret_cast<const void*>(dct_filter_rows_kernel<false, false>),
-                              reinterpret_cast<const void*>(dct_filter_rows_kernel<false, true>),
-                              reinterpret_cast<const void*>(dct_filter_rows_kernel<true, false>),
-                              reinterpret_cast<const void*>(dct_filter_rows_kernel<true, true>)};
+    const void* kernels[] = {reinterpret_cast<const void*>(dct2_rows_kernel<0>),
+                             reinterpret_cast<const void*>(dct2_rows_kernel<16>),
+                             reinterpret_cast<const void*>(dct2_rows_kernel<25>),
+                             reinterpret_cast<const void*>(dct3_rows_kernel<0>),
+                             reinterpret_cast<const void*>(dct3_rows_kernel<16>),
+                             reinterpret_cast<const void*>(dct3_rows_kernel<25>),
+                             reinterpret_cast<const void*>(dct_filter_rows_kernel<false, 0>),
+                             reinterpret_cast<const void*>(dct_filter_rows_kernel<false, 16>),
+                             reinterpret_cast<const void*>(dct_filter_rows_kernel<false, 25>),
+                             reinterpret_cast<const void*>(dct_filter_rows_kernel<true, 0>),
+                             reinterpret_cast<const void*>(dct_filter_rows_kernel<true, 16>),
+                             reinterpret_cast<const void*>(dct_filter_rows_kernel<true, 25>)};
     *max_lds_n = kMaxLdsFft;
     for (const void* k : kernels) {
         if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {
