@@ -25,24 +25,19 @@
 
 #include "../../include/fj/fj_mi355x_ext.h"
 #include "fj_host.h"
+#include "fj_owner.h"
 #include "fj_plan.h"
 
 using namespace fjx;
 
 #define FJ_EXPORT __attribute__((visibility("default")))
 
-struct FjCuda {
-    std::mutex mu;
-    bool open = true;
-    std::map<int, std::unique_ptr<DeviceState>> devices;
-};
-
-namespace {
+namespace fjx {
 
 std::atomic<FjCuda*> g_owner{nullptr};
 thread_local bool t_in_call = false;
 
-FjStatus make_status(uint32_t category, uint32_t api = FJ_API_NONE, int32_t code = 0) {
+FjStatus make_status(uint32_t category, uint32_t api, int32_t code) {
     FjStatus s;
     s.category = category;
     s.api = api;
@@ -62,8 +57,7 @@ void write_error(FjErrorBuffer* error, const std::string& text) {
     error->length = n;
 }
 
-FjStatus fail(FjErrorBuffer* error, uint32_t category, const std::string& text, uint32_t api = FJ_API_NONE,
-              int32_t code = 0) {
+FjStatus fail(FjErrorBuffer* error, uint32_t category, const std::string& text, uint32_t api, int32_t code) {
     write_error(error, text);
     return make_status(category, api, code);
 }
@@ -75,15 +69,7 @@ FjStatus hip_fail(FjErrorBuffer* error, hipError_t e, const char* where) {
     return fail(error, category, text, FJ_API_CUDA_RUNTIME, static_cast<int32_t>(e));
 }
 
-struct CallGuard {
-    bool ok;
-    CallGuard() : ok(!t_in_call) {
-        if (ok) t_in_call = true;
-    }
-    ~CallGuard() {
-        if (ok) t_in_call = false;
-    }
-};
+namespace {
 
 bool image_device(const FjImage& img, int& device) {
     if (img.address == 0) return false;
@@ -96,6 +82,8 @@ bool image_device(const FjImage& img, int& device) {
     device = attr.device;
     return true;
 }
+
+}  // namespace
 
 FjStatus inspect_frame(const FjFrame* frame, FjCudaContext* out, FjErrorBuffer* error) {
     if (!frame) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "frame is null");
@@ -111,7 +99,59 @@ FjStatus inspect_frame(const FjFrame* frame, FjCudaContext* out, FjErrorBuffer* 
     return make_status(FJ_STATUS_SUCCESS);
 }
 
-}  // namespace
+FjRenderOutcome render_locked(FjCuda* cuda, const FjCudaContext* context, const FjFrame* frame,
+                              const FjSubmission* submission, const FjPreparedHostData* prepared,
+                              FjAbortCallback abort_callback, FjErrorBuffer* error) {
+    FjRenderOutcome outcome{};
+    if (!cuda->open) {
+        outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+        return outcome;
+    }
+    if (!context || !frame || !submission || !prepared) {
+        outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null render argument");
+        return outcome;
+    }
+    FjCudaContext current{};
+    FjStatus s = inspect_frame(frame, &current, error);
+    if (s.category != FJ_STATUS_SUCCESS) {
+        outcome.status = s;
+        return outcome;
+    }
+    if (current.device_id != context->device_id || current.context != context->context) {
+        outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "render context differs from the frame's device");
+        return outcome;
+    }
+    hipError_t he = hipSetDevice(current.device_id);
+    if (he != hipSuccess) {
+        outcome.status = hip_fail(error, he, "hipSetDevice");
+        return outcome;
+    }
+    auto& slot = cuda->devices[current.device_id];
+    if (!slot) {
+        slot = std::make_unique<DeviceState>();
+        he = slot->init(current.device_id);
+        if (he != hipSuccess) {
+            slot.reset();
+            outcome.status = hip_fail(error, he, "device state init");
+            return outcome;
+        }
+    }
+    outcome = render_frame(*slot, *frame, *submission, *prepared, abort_callback, error);
+    if (outcome.status.category == FJ_STATUS_SUCCESS) write_error(error, "");
+    return outcome;
+}
+
+bool retire_device_locked(FjCuda* cuda, int32_t device_id) {
+    auto it = cuda->devices.find(device_id);
+    if (it == cuda->devices.end()) return true;
+    bool ok = true;
+    if (it->second) ok = it->second->release() == hipSuccess;
+    (void)hipGetLastError();
+    cuda->devices.erase(it);
+    return ok;
+}
+
+}  // namespace fjx
 
 extern "C" {
 
@@ -175,43 +215,7 @@ FJ_EXPORT FjRenderOutcome fj_cuda_render(FjCuda* cuda, const FjCudaContext* cont
             return outcome;
         }
         std::lock_guard<std::mutex> lock(cuda->mu);
-        if (!cuda->open) {
-            outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
-            return outcome;
-        }
-        if (!context || !frame || !submission || !prepared) {
-            outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null render argument");
-            return outcome;
-        }
-        FjCudaContext current{};
-        FjStatus s = inspect_frame(frame, &current, error);
-        if (s.category != FJ_STATUS_SUCCESS) {
-            outcome.status = s;
-            return outcome;
-        }
-        if (current.device_id != context->device_id || current.context != context->context) {
-            outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "render context differs from the frame's device");
-            return outcome;
-        }
-        hipError_t he = hipSetDevice(current.device_id);
-        if (he != hipSuccess) {
-            outcome.status = hip_fail(error, he, "hipSetDevice");
-            return outcome;
-        }
-        auto& slot = cuda->devices[current.device_id];
-        if (!slot) {
-            slot = std::make_unique<DeviceState>();
-            std::string diag;
-            he = slot->init(current.device_id);
-            if (he != hipSuccess) {
-                slot.reset();
-                outcome.status = hip_fail(error, he, "device state init");
-                return outcome;
-            }
-        }
-        outcome = render_frame(*slot, *frame, *submission, *prepared, abort_callback, error);
-        if (outcome.status.category == FJ_STATUS_SUCCESS) write_error(error, "");
-        return outcome;
+        return render_locked(cuda, context, frame, submission, prepared, abort_callback, error);
     } catch (const std::bad_alloc&) {
         outcome.status = fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
         return outcome;

@@ -261,5 +261,79 @@ FjPreparedHostData = _s("FjPreparedHostData", [
     ("optics", FjOptics), ("spatial_dir", FjSpatialDir), ("grain", FjVisualGrain),
     ("effects", FjFilmEffects), ("noise", FjStaticNoise)])
 
+# ---- frame values and the call scope (juicer_cuda_frame_values.h,
+# juicer_cuda_call_scope.h) ----------------------------------------------------
+FJ_SCANNER_LUT_REASON_NONE, FJ_SCANNER_LUT_REASON_RAW, FJ_SCANNER_LUT_REASON_RESOLUTION = 0, 1, 2
+FJ_SCANNER_LUT_REASON_SIZE, FJ_SCANNER_LUT_REASON_ALLOCATION = 3, 4
+FJ_SCANNER_LUT_REASON_NONFINITE, FJ_SCANNER_LUT_REASON_PANIC = 5, 6
+FJ_GAUSSIAN_KERNEL_NONE, FJ_GAUSSIAN_KERNEL_RAW, FJ_GAUSSIAN_KERNEL_RADIUS = 0, 1, 2
+FJ_GAUSSIAN_KERNEL_SIGMA, FJ_GAUSSIAN_KERNEL_ALLOCATION, FJ_GAUSSIAN_KERNEL_PANIC = 3, 4, 5
+FjScannerLutInput = _s("FjScannerLutInput", [
+    ("eps_c", FjFloatSpan), ("eps_m", FjFloatSpan), ("eps_y", FjFloatSpan), ("weighted_x", FjFloatSpan),
+    ("weighted_y", FjFloatSpan), ("weighted_z", FjFloatSpan), ("baseline_min", FjFloatSpan),
+    ("range_min_cmy", f32 * 3), ("inverse_span_cmy", f32 * 3), ("sample_count", i32), ("inv_yn", f32),
+    ("has_baseline", u32), ("medium", u32), ("resolution", u32)])
+FjScannerLutView = _s("FjScannerLutView", [
+    ("resolution", u32), ("log2_xyz", FjFloatSpan), ("slope_c", FjFloatSpan), ("slope_m", FjFloatSpan),
+    ("slope_y", FjFloatSpan), ("cell_min", FjFloatSpan), ("cell_max", FjFloatSpan)])
+FjScannerLutFailure = _s("FjScannerLutFailure", [
+    ("reason", u32), ("channel", u32), ("index", u64), ("buffer", u32)])
+FjGaussianKernelInput = _s("FjGaussianKernelInput", [("radius", i32), ("sigma_px", f32)])
+FjGaussianKernelView = _s("FjGaussianKernelView", [("radius", i32), ("sigma_px", f32), ("weights", FjFloatSpan)])
+FjGaussianKernelFailure = _s("FjGaussianKernelFailure", [("reason", u32)])
+FjFrameGeometry = _s("FjFrameGeometry", [("full_frame", FjExtent), ("pixel_size_um", f32)])
+FjDirFrameMath = _s("FjDirFrameMath", [
+    ("present", u32), ("component_count", u32), ("recipe_hash", u64), ("render_extent", FjExtent),
+    ("full_frame_extent", FjExtent), ("filter_domain_extent", FjExtent), ("components", FjDirGaussian * 4)])
+FjGrainFrameMath = _s("FjGrainFrameMath", [
+    ("present", u32), ("capture_polarity", u32), ("render_extent", FjExtent), ("full_frame_extent", FjExtent),
+    ("pixel_size_um", f32), ("frame0", i64), ("frame_alpha", f32), ("seed_base", u64), ("seed_base_next", u64),
+    ("session_seed", u64), ("clip_token", u64), ("pitch_px", i32), ("breathing_period_frames", i32),
+    ("clump_morph_period_frames", i32), ("wang_cell_mm", f32), ("breathing_amplitude", f32),
+    ("breathing_cell_um_small", f32), ("breathing_cell_um_large", f32), ("breathing_mix", f32),
+    ("breathing_drift_um_per_frame", f32), ("debug_scale", f32), ("density_max_cmy", f32 * 3),
+    ("n_particles_cmy", f32 * 3), ("od_particle_cmy", f32 * 3), ("density_min_layers", (f32 * 3) * 3),
+    ("density_max_layers", (f32 * 3) * 3), ("n_particles_layers", (f32 * 3) * 3),
+    ("od_particle_layers", (f32 * 3) * 3), ("correlation", FjGrainGaussian * 3),
+    ("dye_cloud", (FjGrainGaussian * 3) * 3), ("size_mix_weights", f32 * 3), ("size_mix_gain", f32),
+    ("requires_full_frame", u32), ("recipe_hash", u64), ("density_layers_hash", u64)])
+FjEffectsFrameMath = _s("FjEffectsFrameMath", [
+    ("present", u32), ("flags", u32), ("render_extent", FjExtent), ("full_frame_extent", FjExtent),
+    ("origins", FjDefectOrigin * 4), ("sample_step_x_mm", f32), ("sample_step_y_mm", f32),
+    ("roi_offset_x", i32), ("roi_offset_y", i32), ("gate_width", i32), ("gate_height", i32),
+    ("session_seed", u64), ("clip_token", u64), ("weave_dx_px", f32), ("weave_dy_px", f32),
+    ("weave_cos_rot", f32), ("weave_sin_rot", f32), ("recipe_hash", u64), ("hash", u64)])
+FjPreparedFramePolicy = _s("FjPreparedFramePolicy", [
+    ("geometry", FjFrameGeometry), ("optics", FjOptics), ("recipe_hash", u64)])
+FjGrainCoefficients = _s("FjGrainCoefficients", [
+    ("flags", u32), ("debug_view", u32), ("sublayer_count", size_t), ("micro_structure", f32 * 2),
+    ("clump_temporal_mix", f32), ("size_mix_scale", f32), ("amplitude", f32), ("chroma_mix", f32),
+    ("chroma_shared_weight", f32), ("chroma_independent_weight", f32), ("particle_density_min_cmy", f32 * 3),
+    ("uniformity_cmy", f32 * 3), ("layer_axis_block_prefix_max", (f32 * 16) * 3)])
+FjFrameMathInput = _s("FjFrameMathInput", [
+    ("dir", FjDirFrameMath), ("grain", FjGrainFrameMath), ("effects", FjEffectsFrameMath),
+    ("grain_coefficients", FjGrainCoefficients), ("film_dust", FjDust), ("gate_dust", FjDust),
+    ("film_scratch", FjScratch), ("gate_scratch", FjScratch)])
+FjNativeFrameDescriptors = _s("FjNativeFrameDescriptors", [
+    ("spatial_dir", FjSpatialDir), ("grain", FjVisualGrain), ("effects", FjFilmEffects)])
+FjScannerLutCallback = C.CFUNCTYPE(FjStatus, C.c_void_p, C.POINTER(FjScannerLutInput), C.POINTER(FjScannerLutView),
+                                   C.POINTER(FjScannerLutFailure), C.POINTER(FjErrorBuffer))
+FjGaussianCallback = C.CFUNCTYPE(FjStatus, C.c_void_p, C.POINTER(FjGaussianKernelInput),
+                                 C.POINTER(FjGaussianKernelView), C.POINTER(FjGaussianKernelFailure),
+                                 C.POINTER(FjErrorBuffer))
+FjHostPreparation = _s("FjHostPreparation", [
+    ("user", C.c_void_p), ("scanner_lut", C.c_void_p), ("gaussian", C.c_void_p)])
+FjCallCompletionInput = _s("FjCallCompletionInput", [
+    ("outcome", FjRenderOutcome), ("diagnostic", FjStringView), ("render_outcome_valid", u32),
+    ("abort_failed", u32), ("recovery_scope_entered", u32), ("has_inspected_context", u32),
+    ("inspected_context", FjCudaContext), ("force_fatal", u32)])
+FjCallCompletionResult = _s("FjCallCompletionResult", [("outcome", FjRenderOutcome), ("fatal", u32)])
+FjCudaCallAccess = _s("FjCudaCallAccess", [
+    ("user", C.c_void_p), ("precheck", C.c_void_p), ("admit", C.c_void_p), ("release", C.c_void_p),
+    ("inspect", C.c_void_p), ("preview", C.c_void_p), ("frame_descriptors", C.c_void_p), ("render", C.c_void_p),
+    ("complete", C.c_void_p), ("trace_submission", C.c_void_p)])
+FjCudaCompletion = _s("FjCudaCompletion", [
+    ("result", FjCallCompletionResult), ("retirement", u32), ("invalidation", FjStatus), ("latch_cleared", u32)])
+
 ALL_TYPES = {name: obj for name, obj in globals().items()
              if isinstance(obj, type) and issubclass(obj, C.Structure)}

@@ -49,6 +49,17 @@ def lib():
         L.fjx_grain_apply.restype = abi.FjStatus
         L.fjx_grain_apply.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjPreparedHostData), C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, eb]
+        L.fjx_frame_descriptors.restype = abi.FjStatus
+        L.fjx_frame_descriptors.argtypes = [C.c_uint32, C.POINTER(abi.FjFrame), C.POINTER(abi.FjFrameMathInput),
+                                            C.POINTER(abi.FjNativeFrameDescriptors), C.POINTER(abi.FjErrorBuffer)]
+        L.fj_cuda_call_retire_context.restype = C.c_uint32
+        L.fj_cuda_call_retire_context.argtypes = [C.c_void_p, C.POINTER(abi.FjCallCompletionInput)]
+        L.fj_cuda_call_trace_flags.restype = C.c_uint32
+        L.fj_cuda_call_trace_flags.argtypes = []
+        L.fj_cuda_call_trace_cutoff.restype = None
+        L.fj_cuda_call_trace_cutoff.argtypes = [abi.FjStringView]
+        L.fj_cuda_call_trace_failure.restype = None
+        L.fj_cuda_call_trace_failure.argtypes = [abi.FjStringView]
         L.fjx_stage_timings.restype = C.c_int32
         L.fjx_stage_timings.argtypes = [C.c_void_p, C.c_int32, C.POINTER(FjxStageTiming), C.c_int32]
         L.fj_cuda_destroy.restype = abi.FjStatus

@@ -7,6 +7,7 @@
 #define FJ_MI355X_EXT_H
 
 #include "juicer_cuda_api.h"
+#include "juicer_cuda_frame_values.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -36,6 +37,15 @@ int32_t fjx_stage_timings(FjCuda* cuda, int32_t device_id, FjxStageTiming* out, 
 FjStatus fjx_grain_apply(FjCuda* cuda, int32_t device_id, const FjPreparedHostData* prepared, float* density_c,
                          float* density_m, float* density_y, int32_t width, int32_t height, uint64_t stream,
                          FjErrorBuffer* error);
+
+/* The spatial-DIR, visual-grain and film-effects descriptors of one frame,
+ * composed from the host's frame math without a device: the composition
+ * FjCudaCallAccess.frame_descriptors performs for an inspected frame
+ * (native/juicer_cuda_call_scope.cpp:425-551), for callers outside a call
+ * scope and for host-only tests. Uses the frame's render window, full-frame
+ * extent, pixel size, session seed, clip token and effects geometry only. */
+FjStatus fjx_frame_descriptors(uint32_t route, const FjFrame* frame, const FjFrameMathInput* input,
+                               FjNativeFrameDescriptors* out, FjErrorBuffer* error);
 
 #ifdef __cplusplus
 }

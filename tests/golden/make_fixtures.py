@@ -6,6 +6,8 @@ committed under tests/golden/ and nothing at run time reads the reference.
 Sources (all data files held by the reference's tests):
   * tests/ffi/cuda_abi_facts.inc                       -> abi_facts.json
       x86-64 sizes/offsets/values of every FjPreparedHostData record.
+  * tests/ffi/prepared_call_abi_facts.inc              -> call_abi_facts.json
+      x86-64 sizes/offsets/values of the call-scope and frame-value records.
   * tests/ffi/fixtures/prepared-state-projection/linux-release.json
       -> prepared/<case>.npz + prepared/<case>.json
       Complete prepared products (exact f32 words) for named recipes:
@@ -46,10 +48,10 @@ def f32(word: int) -> float:
     return struct.unpack("<f", struct.pack("<I", word & 0xFFFFFFFF))[0]
 
 
-def abi_facts() -> None:
+def abi_facts(source: str = "cuda_abi_facts.inc", target: str = "abi_facts.json") -> None:
     facts = {"types": {}, "fields": [], "values": {}}
     pat = re.compile(r"^FJ_ABI_(TYPE|FIELD|VALUE)\((.*)\)\s*$")
-    for line in open(f"{REF}/tests/ffi/cuda_abi_facts.inc"):
+    for line in open(f"{REF}/tests/ffi/{source}"):
         m = pat.match(line.strip())
         if not m:
             continue
@@ -60,7 +62,7 @@ def abi_facts() -> None:
             facts["fields"].append([args[0], args[1], int(args[2])])
         else:
             facts["values"][args[0]] = int(args[1], 0)
-    with open(os.path.join(HERE, "abi_facts.json"), "w") as fh:
+    with open(os.path.join(HERE, target), "w") as fh:
         json.dump(facts, fh, indent=0, sort_keys=True)
 
 
@@ -270,6 +272,7 @@ if __name__ == "__main__":
     what = sys.argv[1:] or ["abi", "prepared", "lut", "dir", "halation", "diffusion"]
     if "abi" in what:
         abi_facts()
+        abi_facts("prepared_call_abi_facts.inc", "call_abi_facts.json")
     if "prepared" in what:
         prepared_cases()
     if "lut" in what:

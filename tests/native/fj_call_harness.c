@@ -1,0 +1,175 @@
+/* fj_call_harness.c — a host-side driver of the call scope for the tests.
+ *
+ * fj_cuda_call's continuation and completion return records by value, which
+ * Python callbacks cannot, so the tests describe a call as a script of
+ * operations and this C driver plays it inside the scope, recording every
+ * status, diagnostic and record the scope returns. Test infrastructure only.
+ */
+#include <stdio.h>
+#include <string.h>
+
+#include "fj/juicer_cuda_call_scope.h"
+
+enum {
+    FJH_PRECHECK = 1,
+    FJH_ADMIT = 2,
+    FJH_INSPECT = 3,
+    FJH_DESCRIPTORS = 4,
+    FJH_PREVIEW = 5,
+    FJH_RENDER = 6,
+    FJH_RELEASE = 7,
+    FJH_COMPLETE = 8,
+    FJH_PUBLIC_INSPECT = 9, /* fj_cuda_inspect from inside the scope (re-entry) */
+    FJH_TRACE = 10,
+    FJH_RENDER_OTHER_CONTEXT = 11,
+};
+
+#define FJH_MAX_OPS 16
+#define FJH_TEXT 256
+
+typedef struct FjhScript {
+    /* inputs */
+    uint32_t route;
+    uint32_t op_count;
+    uint32_t ops[FJH_MAX_OPS];
+    const FjFrame* frame;
+    const FjFrameMathInput* math;
+    FjPreparedHostData* prepared; /* receives the composed descriptors when copy_descriptors */
+    uint32_t copy_descriptors;
+    const FjSubmission* submission;
+    FjRect preview_source, preview_meter;
+    uint32_t preview_method;
+    uint32_t recovery_scope_entered;
+    uint32_t force_fatal;
+    uint32_t retire_on_completion; /* call fj_cuda_call_retire_context from the completion */
+    uint32_t report_context_loss;  /* the completion input reports FJ_STATUS_CONTEXT_LOSS */
+    /* results */
+    FjStatus status[FJH_MAX_OPS];
+    char text[FJH_MAX_OPS][FJH_TEXT];
+    FjCudaContext context;
+    FjNativeFrameDescriptors descriptors;
+    FjAutoExposure preview;
+    FjRenderOutcome outcome;
+    FjCallCompletionResult completed;
+    uint32_t completion_calls;
+    FjCallCompletionInput completion_seen;
+    char completion_diagnostic[FJH_TEXT];
+    uint32_t retirement;
+    FjCallCompletionResult call_result;
+} FjhScript;
+
+typedef struct FjhState {
+    FjCuda* cuda;
+    FjhScript* s;
+} FjhState;
+
+static FjCudaCompletion fjh_completion(void* user, void* scope, const FjCallCompletionInput* input) {
+    FjhState* st = (FjhState*)user;
+    FjhScript* s = st->s;
+    FjCudaCompletion out;
+    memset(&out, 0, sizeof(out));
+    s->completion_calls++;
+    s->completion_seen = *input;
+    s->completion_seen.diagnostic.data = NULL;
+    s->completion_diagnostic[0] = '\0';
+    if (input->diagnostic.data) {
+        size_t n = input->diagnostic.count < FJH_TEXT - 1 ? input->diagnostic.count : FJH_TEXT - 1;
+        memcpy(s->completion_diagnostic, input->diagnostic.data, n);
+        s->completion_diagnostic[n] = '\0';
+    }
+    if (s->retire_on_completion) {
+        out.retirement = fj_cuda_call_retire_context(scope, input);
+        s->retirement = out.retirement;
+    }
+    out.result.outcome = input->outcome;
+    out.result.fatal = input->force_fatal;
+    return out;
+}
+
+static FjCallCompletionResult fjh_continuation(void* user, FjCudaCallAccess a) {
+    FjhState* st = (FjhState*)user;
+    FjhScript* s = st->s;
+    FjCallCompletionResult last;
+    memset(&last, 0, sizeof(last));
+    int rendered = 0;
+    for (uint32_t i = 0; i < s->op_count && i < FJH_MAX_OPS; ++i) {
+        FjErrorBuffer err = {s->text[i], FJH_TEXT, 0};
+        FjStatus r;
+        memset(&r, 0, sizeof(r));
+        switch (s->ops[i]) {
+            case FJH_PRECHECK: r = a.precheck(a.user, &err); break;
+            case FJH_ADMIT: r = a.admit(a.user, &err); break;
+            case FJH_INSPECT: r = a.inspect(a.user, s->frame, &s->context, &err); break;
+            case FJH_DESCRIPTORS:
+                r = a.frame_descriptors(a.user, s->math, &s->descriptors, &err);
+                if (r.category == FJ_STATUS_SUCCESS && s->copy_descriptors && s->prepared) {
+                    s->prepared->spatial_dir = s->descriptors.spatial_dir;
+                    s->prepared->grain = s->descriptors.grain;
+                    s->prepared->effects = s->descriptors.effects;
+                }
+                break;
+            case FJH_PREVIEW:
+                r = a.preview(a.user, &s->preview_source, &s->preview_meter, s->preview_method, &s->preview, &err);
+                break;
+            case FJH_RENDER:
+            case FJH_RENDER_OTHER_CONTEXT: {
+                FjAbortCallback abort_cb = {NULL, NULL};
+                FjCudaContext ctx = s->context;
+                if (s->ops[i] == FJH_RENDER_OTHER_CONTEXT) ctx.context += 1000;
+                s->outcome = a.render(a.user, &ctx, s->frame, s->submission, s->prepared, NULL, abort_cb, &err);
+                r = s->outcome.status;
+                rendered = 1;
+                break;
+            }
+            case FJH_RELEASE:
+                a.release(a.user);
+                break;
+            case FJH_COMPLETE: {
+                FjCallCompletionInput in;
+                memset(&in, 0, sizeof(in));
+                if (rendered) {
+                    in.outcome = s->outcome;
+                    in.render_outcome_valid = 1;
+                }
+                if (s->report_context_loss) in.outcome.status.category = FJ_STATUS_CONTEXT_LOSS;
+                in.recovery_scope_entered = s->recovery_scope_entered;
+                in.has_inspected_context = s->context.context != 0;
+                in.inspected_context = s->context;
+                in.force_fatal = s->force_fatal;
+                s->completed = a.complete(a.user, &in);
+                r = s->completed.outcome.status;
+                last = s->completed;
+                break;
+            }
+            case FJH_PUBLIC_INSPECT: {
+                FjCudaContext ctx;
+                r = fj_cuda_inspect(st->cuda, s->frame, &ctx, &err);
+                break;
+            }
+            case FJH_TRACE:
+                if (a.trace_submission) a.trace_submission(a.user, s->submission, 0);
+                break;
+            default: break;
+        }
+        s->status[i] = r;
+    }
+    return last;
+}
+
+/* Plays the script inside fj_cuda_call(cuda, route, ...). */
+int fjh_run(FjCuda* cuda, FjhScript* s) {
+    FjhState st = {cuda, s};
+    s->call_result = fj_cuda_call(cuda, s->route, fjh_continuation, fjh_completion, &st);
+    return 0;
+}
+
+/* fj_cuda_call with caller-chosen null arguments (argument validation). */
+FjCallCompletionResult fjh_call_raw(FjCuda* cuda, uint32_t route, int with_continuation, int with_completion,
+                                    int with_user) {
+    static FjhScript empty;
+    FjhState st = {cuda, &empty};
+    return fj_cuda_call(cuda, route, with_continuation ? fjh_continuation : NULL,
+                        with_completion ? fjh_completion : NULL, with_user ? (void*)&st : NULL);
+}
+
+size_t fjh_script_size(void) { return sizeof(FjhScript); }

@@ -57,3 +57,41 @@ def test_header_compiles_as_cpp():
         r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-c", src, "-o",
                             os.path.join(d, "abi.o")], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr[-4000:]
+
+
+# Call scope and frame values (tests/ffi/prepared_call_abi_facts.inc, extracted
+# to tests/golden/call_abi_facts.json). Its two values name the reference's
+# test-only API (juicer_test_api.h) and are not part of the boundary.
+CALL_FACTS = json.load(open(os.path.join(HERE, "golden", "call_abi_facts.json")))
+
+
+def test_call_scope_ctypes_mirror_matches_reference_facts():
+    for name, (size, align) in CALL_FACTS["types"].items():
+        t = abi.ALL_TYPES[name]
+        assert ctypes.sizeof(t) == size, name
+        assert ctypes.alignment(t) == align, name
+    for name, field, off in CALL_FACTS["fields"]:
+        assert getattr(abi.ALL_TYPES[name], field).offset == off, (name, field)
+    assert len(CALL_FACTS["types"]) == 8 and len(CALL_FACTS["fields"]) == 50
+
+
+def test_call_scope_header_matches_reference_facts():
+    lines = ['#include <stddef.h>', '#include "fj/juicer_cuda_call_scope.h"', '#include "fj/fj_mi355x_ext.h"']
+    for name, (size, align) in CALL_FACTS["types"].items():
+        lines.append(f'_Static_assert(sizeof({name}) == {size} && _Alignof({name}) == {align}, "{name}");')
+    for name, field, off in CALL_FACTS["fields"]:
+        lines.append(f'_Static_assert(offsetof({name}, {field}) == {off}, "{name}.{field}");')
+    # The frame-math records' sizes follow from the offsets above.
+    lines.append('_Static_assert(sizeof(FjDirFrameMath) == 512, "FjDirFrameMath");')
+    lines.append('_Static_assert(sizeof(FjGrainFrameMath) == 552, "FjGrainFrameMath");')
+    lines.append('_Static_assert(sizeof(FjEffectsFrameMath) == 208, "FjEffectsFrameMath");')
+    lines.append("int main(void) { return 0; }")
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "abi.c")
+        open(src, "w").write("\n".join(lines))
+        r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", src,
+                            "-o", os.path.join(d, "abi.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-4000:]
+    for name in ("FjDirFrameMath", "FjGrainFrameMath", "FjEffectsFrameMath"):
+        assert ctypes.sizeof(abi.ALL_TYPES[name]) == {"FjDirFrameMath": 512, "FjGrainFrameMath": 552,
+                                                      "FjEffectsFrameMath": 208}[name]
