@@ -310,6 +310,40 @@ FJ_EXPORT FjStatus fjx_grain_apply(FjCuda* cuda, int32_t device_id, const FjPrep
     }
 }
 
+FJ_EXPORT FjStatus fjx_diffusion_apply(FjCuda* cuda, int32_t device_id, const FjDiffusion* stage, float* red,
+                                       float* green, float* blue, int32_t width, int32_t height, uint64_t stream,
+                                       FjErrorBuffer* error) {
+    try {
+        if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
+        CallGuard guard;
+        if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
+        std::lock_guard<std::mutex> lock(cuda->mu);
+        if (!cuda->open) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+        if (!stage) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null diffusion stage");
+        int count = 0;
+        hipError_t he = hipGetDeviceCount(&count);
+        if (he != hipSuccess) return hip_fail(error, he, "hipGetDeviceCount");
+        if (device_id < 0 || device_id >= count) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "device id out of range");
+        he = hipSetDevice(device_id);
+        if (he != hipSuccess) return hip_fail(error, he, "hipSetDevice");
+        auto& slot = cuda->devices[device_id];
+        if (!slot) {
+            slot = std::make_unique<DeviceState>();
+            he = slot->init(device_id);
+            if (he != hipSuccess) {
+                slot.reset();
+                return hip_fail(error, he, "device state init");
+            }
+        }
+        float* const planes[3] = {red, green, blue};
+        return diffusion_apply_planes(*slot, *stage, planes, width, height, reinterpret_cast<hipStream_t>(stream), error);
+    } catch (const std::bad_alloc&) {
+        return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fjx_diffusion_apply");
+    }
+}
+
 FJ_EXPORT int32_t fjx_stage_timings(FjCuda* cuda, int32_t device_id, FjxStageTiming* out, int32_t capacity) {
     try {
         if (!cuda || g_owner.load() != cuda || capacity < 0 || (capacity > 0 && !out)) return -1;

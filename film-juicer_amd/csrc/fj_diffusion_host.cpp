@@ -212,18 +212,9 @@ bool diffusion_quadrant(const FjDiffusion& d, int channel, std::vector<double>& 
     return ok;
 }
 
-int smooth_length(int n) {
-    for (int m = std::max(n, 1);; ++m) {
-        int v = m;
-        for (int p : {2, 3, 5})
-            while (v % p == 0) v /= p;
-        if (v == 1) return m;
-    }
-}
-
 void diffusion_padded(const FjDiffusion& d, int width, int height, int& pw, int& ph) {
-    pw = smooth_length(width + 2 * d.radius_px);
-    ph = smooth_length(height + 2 * d.radius_px);
+    pw = inplace_fft_length(width + 2 * d.radius_px);
+    ph = inplace_fft_length(height + 2 * d.radius_px);
 }
 
 size_t diffusion_build_floats(const FjDiffusion& d, int pw, int ph) {

@@ -23,8 +23,6 @@ bool diffusion_components(const FjDiffusion& d, double lambda_px[kDiffusionCompo
 bool diffusion_decode(const FjOptics& o, bool print_route, std::string& field);
 // Normalized PSF quadrant q[dy][dx], dx, dy in [0, r], of channel `channel`.
 bool diffusion_quadrant(const FjDiffusion& d, int channel, std::vector<double>& q);
-// Smallest 2^a 3^b 5^c >= n.
-int smooth_length(int n);
 
 struct DiffusionStage {
     int radius = 0;
@@ -33,7 +31,9 @@ struct DiffusionStage {
     DctTransfer transfer;     // full[c] = T of channel c (R, G, B), pw x ph
 };
 
-// Padded size of a stage on a width x height frame.
+// Padded size of a stage on a width x height frame: each side extended by the
+// radius on both ends, rounded up to the next length with an in-place LDS FFT
+// plan (0 when none fits).
 void diffusion_padded(const FjDiffusion& d, int width, int height, int& pw, int& ph);
 // Scratch floats the transfer build needs.
 size_t diffusion_build_floats(const FjDiffusion& d, int pw, int ph);

@@ -10,7 +10,8 @@ namespace fjx {
 
 constexpr int kMaxFftPasses = 24;
 constexpr int kMaxDctComponents = 8;
-constexpr int kMaxLdsFft = 8192;  // 2 x 8192 float2 = 128 KiB of the 160 KiB LDS
+constexpr int kMaxLdsFft = 16384;      // in-place engine: 16384 float2 = 128 KiB of the 160 KiB LDS
+constexpr int kMaxPingPongFft = 8192;  // ping-pong engine: 2 x 8192 float2
 constexpr int kMaxRowThreads = 1024;
 
 struct FftPlanDev {
@@ -24,7 +25,7 @@ struct FftPlanDev {
     int tw_offset[kMaxFftPasses] = {};
     unsigned ns_magic[kMaxFftPasses] = {};
     // In-place engine (2,3,5-smooth n): radices from {2,3,4,5,6,8,9,10,12,15,
-    // 16,20,25}, one butterfly per thread, `threads` >= n / radix of every pass.
+    // 16,18,20,24,25}, one butterfly per thread, `threads` >= n / radix of every pass.
     int inplace = 0;
     int threads = 0;
 };
@@ -85,6 +86,8 @@ struct TransposeArgs {
 // threads), odd radices first; any other n the ping-pong engine's radices (4 first, then 2, 3, 5,
 // then other primes).
 bool make_fft_plan(int n, FftPlanDev& plan);
+// Smallest length >= n that has an in-place plan (0 if none up to kMaxLdsFft).
+int inplace_fft_length(int n);
 // exp(-2 pi i e / n) for e < n, then the per-pass twiddles of the plan, in
 // float2 pairs; and exp(-i pi k / 2n) for k < n.
 std::vector<float> fft_twiddles(int n);

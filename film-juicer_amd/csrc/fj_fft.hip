@@ -9,7 +9,7 @@
 // diffusion line, and the frame sides of the usual formats) run the in-place
 // engine: one LDS buffer (odd radices first, so the strided writes of the
 // first passes spread over the banks); each pass is one butterfly per thread, of a
-// composite radix up to 25 (16 = 4x4, 15 = 3x5, 25 = 5x5, ...) evaluated in
+// composite radix up to 25 (16 = 4x4, 15 = 3x5, 24 = 3x8, 25 = 5x5, ...) evaluated in
 // registers as a two-level Cooley-Tukey DFT, so a 3840- or 5625-point line
 // takes three passes. Other lengths run the ping-pong engine (radix 4, 2, 3, 5
 // in registers, other primes through a generic LDS butterfly) over two buffers.
@@ -179,6 +179,14 @@ template <> struct Roots<16> {
     static constexpr float c[16] = {1.000000000f, 9.238795325e-1f, 7.071067812e-1f, 3.826834324e-1f, 0.000000000f, -3.826834324e-1f, -7.071067812e-1f, -9.238795325e-1f, -1.000000000f, -9.238795325e-1f, -7.071067812e-1f, -3.826834324e-1f, 0.000000000f, 3.826834324e-1f, 7.071067812e-1f, 9.238795325e-1f};
     static constexpr float s[16] = {0.000000000f, 3.826834324e-1f, 7.071067812e-1f, 9.238795325e-1f, 1.000000000f, 9.238795325e-1f, 7.071067812e-1f, 3.826834324e-1f, 0.000000000f, -3.826834324e-1f, -7.071067812e-1f, -9.238795325e-1f, -1.000000000f, -9.238795325e-1f, -7.071067812e-1f, -3.826834324e-1f};
 };
+template <> struct Roots<18> {
+    static constexpr float c[18] = {1.000000000f, 9.396926208e-1f, 7.660444431e-1f, 5.000000000e-1f, 1.736481777e-1f, -1.736481777e-1f, -5.000000000e-1f, -7.660444431e-1f, -9.396926208e-1f, -1.000000000f, -9.396926208e-1f, -7.660444431e-1f, -5.000000000e-1f, -1.736481777e-1f, 1.736481777e-1f, 5.000000000e-1f, 7.660444431e-1f, 9.396926208e-1f};
+    static constexpr float s[18] = {0.000000000f, 3.420201433e-1f, 6.427876097e-1f, 8.660254038e-1f, 9.848077530e-1f, 9.848077530e-1f, 8.660254038e-1f, 6.427876097e-1f, 3.420201433e-1f, 0.000000000f, -3.420201433e-1f, -6.427876097e-1f, -8.660254038e-1f, -9.848077530e-1f, -9.848077530e-1f, -8.660254038e-1f, -6.427876097e-1f, -3.420201433e-1f};
+};
+template <> struct Roots<24> {
+    static constexpr float c[24] = {1.000000000f, 9.659258263e-1f, 8.660254038e-1f, 7.071067812e-1f, 5.000000000e-1f, 2.588190451e-1f, 0.000000000f, -2.588190451e-1f, -5.000000000e-1f, -7.071067812e-1f, -8.660254038e-1f, -9.659258263e-1f, -1.000000000f, -9.659258263e-1f, -8.660254038e-1f, -7.071067812e-1f, -5.000000000e-1f, -2.588190451e-1f, 0.000000000f, 2.588190451e-1f, 5.000000000e-1f, 7.071067812e-1f, 8.660254038e-1f, 9.659258263e-1f};
+    static constexpr float s[24] = {0.000000000f, 2.588190451e-1f, 5.000000000e-1f, 7.071067812e-1f, 8.660254038e-1f, 9.659258263e-1f, 1.000000000f, 9.659258263e-1f, 8.660254038e-1f, 7.071067812e-1f, 5.000000000e-1f, 2.588190451e-1f, 0.000000000f, -2.588190451e-1f, -5.000000000e-1f, -7.071067812e-1f, -8.660254038e-1f, -9.659258263e-1f, -1.000000000f, -9.659258263e-1f, -8.660254038e-1f, -7.071067812e-1f, -5.000000000e-1f, -2.588190451e-1f};
+};
 template <> struct Roots<20> {
     static constexpr float c[20] = {1.000000000f, 9.510565163e-1f, 8.090169944e-1f, 5.877852523e-1f, 3.090169944e-1f, 0.000000000f, -3.090169944e-1f, -5.877852523e-1f, -8.090169944e-1f, -9.510565163e-1f, -1.000000000f, -9.510565163e-1f, -8.090169944e-1f, -5.877852523e-1f, -3.090169944e-1f, 0.000000000f, 3.090169944e-1f, 5.877852523e-1f, 8.090169944e-1f, 9.510565163e-1f};
     static constexpr float s[20] = {0.000000000f, 3.090169944e-1f, 5.877852523e-1f, 8.090169944e-1f, 9.510565163e-1f, 1.000000000f, 9.510565163e-1f, 8.090169944e-1f, 5.877852523e-1f, 3.090169944e-1f, 0.000000000f, -3.090169944e-1f, -5.877852523e-1f, -8.090169944e-1f, -9.510565163e-1f, -1.000000000f, -9.510565163e-1f, -8.090169944e-1f, -5.877852523e-1f, -3.090169944e-1f};
@@ -291,7 +299,9 @@ template <> struct Dft<10> : DftCT<2, 5> {};
 template <> struct Dft<12> : DftCT<4, 3> {};
 template <> struct Dft<15> : DftCT<3, 5> {};
 template <> struct Dft<16> : DftCT<4, 4> {};
+template <> struct Dft<18> : DftCT<2, 9> {};
 template <> struct Dft<20> : DftCT<4, 5> {};
+template <> struct Dft<24> : DftCT<3, 8> {};
 template <> struct Dft<25> : DftCT<5, 5> {};
 
 // One in-place Stockham pass of radix R: thread j < n / R owns butterfly j.
@@ -350,8 +360,14 @@ __device__ void fft_inplace(float2* buf, const FftPlanDev& plan, const float2* t
             case 12: pass_ip<12>(buf, plan.n, ns, stw, magic); break;
             case 15: pass_ip<15>(buf, plan.n, ns, stw, magic); break;
             case 16: pass_ip<16>(buf, plan.n, ns, stw, magic); break;
+            case 18:
+                if constexpr (kMaxR >= 18) pass_ip<18>(buf, plan.n, ns, stw, magic);
+                break;
             case 20:
                 if constexpr (kMaxR >= 20) pass_ip<20>(buf, plan.n, ns, stw, magic);
+                break;
+            case 24:
+                if constexpr (kMaxR >= 24) pass_ip<24>(buf, plan.n, ns, stw, magic);
                 break;
             case 25:
                 if constexpr (kMaxR >= 25) pass_ip<25>(buf, plan.n, ns, stw, magic);
@@ -524,7 +540,7 @@ __device__ __forceinline__ float transfer_at(const DctTransfer& tf, const float*
 // kFullTable: T is a full per-plane table (diffusion); otherwise a weighted sum
 // of separable component factors (spatial DIR).
 template <bool kFullTable, int kEng>
-__global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRowsArgs args, DctTransfer tf) {
+__device__ __forceinline__ void dct_filter_rows_body(const DctRowsArgs& args, const DctTransfer& tf) {
     extern __shared__ float2 lds2[];
     const FftPlanDev& plan = args.plan;
     const int n = plan.n;
@@ -614,6 +630,20 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRows
         o0[i] = z.x * inv;
         if (o1) o1[i] = -z.y * inv;
     }
+}
+
+template <bool kFullTable, int kEng>
+__global__ __launch_bounds__(kMaxRowThreads) void dct_filter_rows_kernel(DctRowsArgs args, DctTransfer tf) {
+    dct_filter_rows_body<kFullTable, kEng>(args, tf);
+}
+
+// The diffusion column filter on radix <= 16 plans: capped at 96 VGPRs
+// (five waves per SIMD) it still needs no scratch, and five 3840-point
+// blocks then fit a CU by registers as they do by LDS.
+template <>
+__global__ __launch_bounds__(kMaxRowThreads) __attribute__((amdgpu_waves_per_eu(5, 8))) void
+dct_filter_rows_kernel<true, 16>(DctRowsArgs args, DctTransfer tf) {
+    dct_filter_rows_body<true, 16>(args, tf);
 }
 
 // 64x64 tiled transpose through LDS (padded against bank conflicts).
@@ -721,7 +751,7 @@ hipError_t fft_kernels_init(int* max_lds_n) {
     // Line transforms beyond 4096 points need more than the default 64 KiB of
     // LDS (the ping-pong engine holds two lines); fall back to 4096-point
     // lines if the runtime refuses the request.
-    const int bytes = static_cast<int>(2 * kMaxLdsFft * sizeof(float2));
+    const int bytes = static_cast<int>(kMaxLdsFft * sizeof(float2));  // = 2 x kMaxPingPongFft float2
     const void* kernels[] = {reinterpret_cast<const void*>(dct2_rows_kernel<0>),
                              reinterpret_cast<const void*>(dct2_rows_kernel<16>),
                              reinterpret_cast<const void*>(dct2_rows_kernel<25>),

@@ -19,7 +19,7 @@ struct Factoring {
 };
 
 // Composite radices of the in-place engine (largest first).
-constexpr int kInplaceRadices[] = {25, 20, 16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+constexpr int kInplaceRadices[] = {25, 24, 20, 18, 16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
 
 // Fewest passes, then the largest smallest radix, over factorings of m into
 // in-place radices no smaller than rmin.
@@ -85,6 +85,7 @@ bool make_fft_plan(int n, FftPlanDev& plan) {
         plan.inplace = 1;
         plan.threads = threads;
     } else {
+        if (n > kMaxPingPongFft) return false;
         radices.clear();
         int m = n;
         while (m % 4 == 0) {
@@ -115,6 +116,14 @@ bool make_fft_plan(int n, FftPlanDev& plan) {
         ns *= radices[i];
     }
     return true;
+}
+
+int inplace_fft_length(int n) {
+    std::vector<int> radices;
+    int threads = 0;
+    for (int m = std::max(n, 1); m <= kMaxLdsFft; ++m)
+        if (inplace_radices(m, radices, threads)) return m;
+    return 0;
 }
 
 std::vector<float> fft_twiddles(int n) {

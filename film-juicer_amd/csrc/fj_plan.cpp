@@ -386,7 +386,9 @@ bool check_spatial_dir(const FjPreparedHostData& p, const FjFrame& f, int max_li
         !same_extent(sd.full_frame_extent, f.full_frame_extent))
         return d.malformed("spatial_dir.frame");
     const int w = f.render_window.x2 - f.render_window.x1, h = f.render_window.y2 - f.render_window.y1;
-    if (w > max_line || h > max_line) return d.unsupported("spatial DIR frame side above the LDS line limit");
+    FftPlanDev plan;
+    if (w > max_line || h > max_line || !make_fft_plan(w, plan) || !make_fft_plan(h, plan))
+        return d.unsupported("spatial DIR frame side above the LDS line limit");
     return true;
 }
 
@@ -873,7 +875,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
             return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput diffusion frame side below 2", error);
         int pw = 0, ph = 0;
         diffusion_padded(*dd, pp.io.width, pp.io.height, pw, ph);
-        if (pw > dev.max_line || ph > dev.max_line)
+        if (pw == 0 || ph == 0 || pw > dev.max_line || ph > dev.max_line)
             return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT,
                                 "UnsupportedInput diffusion padded frame side above the LDS line limit", error);
         padded_floats = std::max({padded_floats, 6LL * pw * ph, static_cast<long long>(diffusion_build_floats(*dd, pw, ph))});
@@ -1153,6 +1155,44 @@ FjStatus grain_apply_planes(DeviceState& dev, const FjPreparedHostData& prepared
     if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
     e = run_grain(gp, density, width, height, static_cast<float*>(dev.workspace), stream);
     if (e != hipSuccess) return outcome_hip(e, "visual grain launch", error).status;
+    e = dev.release_workspace(stream);
+    if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
+    put_error(error, "");
+    return status_of(FJ_STATUS_SUCCESS);
+}
+
+FjStatus diffusion_apply_planes(DeviceState& dev, const FjDiffusion& stage, float* const planes[3], int width,
+                                int height, hipStream_t stream, FjErrorBuffer* error) {
+    if (stage.family > 3 || stage.hash == 0 || stage.sample_hash == 0)
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "MalformedPreparedHostData field=optics.diffusion", error).status;
+    if (width < 2 || height < 2 || !planes[0] || !planes[1] || !planes[2])
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput diffusion frame side below 2", error).status;
+    double lam[kDiffusionComponents], wts[3][kDiffusionComponents];
+    std::string field;
+    if (!diffusion_components(stage, lam, wts, field))
+        return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "InvalidDiffusionHostBehavior field=" + field, error).status;
+    int pw = 0, ph = 0;
+    diffusion_padded(stage, width, height, pw, ph);
+    if (pw == 0 || ph == 0 || pw > dev.max_line || ph > dev.max_line)
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT,
+                            "UnsupportedInput diffusion padded frame side above the LDS line limit", error)
+            .status;
+    ++dev.frame_serial;
+    const size_t floats =
+        std::max(static_cast<size_t>(6) * pw * ph, diffusion_build_floats(stage, pw, ph));
+    hipError_t e = dev.ensure_workspace(floats * sizeof(float));
+    if (e != hipSuccess) return outcome_hip(e, "workspace", error).status;
+    e = dev.acquire_workspace(stream);
+    if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
+    float* ws = static_cast<float*>(dev.workspace);
+    DiffusionStage st;
+    e = diffusion_prepare(dev, stage, width, height, ws, stream, st);
+    if (e == hipErrorInvalidValue)
+        return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "InvalidDiffusionHostBehavior field=component_output", error)
+            .status;
+    if (e != hipSuccess) return outcome_hip(e, "diffusion transfer", error).status;
+    e = run_diffusion(dev, st, planes, width, height, ws, stream);
+    if (e != hipSuccess) return outcome_hip(e, "diffusion stage", error).status;
     e = dev.release_workspace(stream);
     if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
     put_error(error, "");

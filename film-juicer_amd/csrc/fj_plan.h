@@ -81,6 +81,10 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
 // density planes (device memory, width x height each) on `stream`.
 FjStatus grain_apply_planes(DeviceState& dev, const FjPreparedHostData& prepared, float* const density[3], int width,
                             int height, hipStream_t stream, FjErrorBuffer* error);
+// One diffusion stage (camera or enlarger descriptor) applied in place to
+// three caller-owned R, G, B planes of a width x height frame.
+FjStatus diffusion_apply_planes(DeviceState& dev, const FjDiffusion& stage, float* const planes[3], int width,
+                                int height, hipStream_t stream, FjErrorBuffer* error);
 
 uint64_t hash_bytes(const void* data, size_t bytes, uint64_t seed);
 

@@ -49,6 +49,9 @@ def lib():
         L.fjx_grain_apply.restype = abi.FjStatus
         L.fjx_grain_apply.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjPreparedHostData), C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, eb]
+        L.fjx_diffusion_apply.restype = abi.FjStatus
+        L.fjx_diffusion_apply.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjDiffusion), C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(abi.FjErrorBuffer)]
         L.fjx_frame_descriptors.restype = abi.FjStatus
         L.fjx_frame_descriptors.argtypes = [C.c_uint32, C.POINTER(abi.FjFrame), C.POINTER(abi.FjFrameMathInput),
                                             C.POINTER(abi.FjNativeFrameDescriptors), C.POINTER(abi.FjErrorBuffer)]
@@ -190,6 +193,14 @@ class Renderer:
         err = ErrorBuffer()
         st = lib().fjx_grain_apply(self.handle, device_id, C.byref(prepared_record), planes[0], planes[1], planes[2],
                                    width, height, stream, err.ptr())
+        return st, err.text
+
+    def diffusion_apply(self, stage, planes, width: int, height: int, stream: int = 0, device_id: int = 0):
+        """Runs one diffusion stage on three device planes (addresses R, G, B);
+        returns (FjStatus, diagnostic text)."""
+        err = ErrorBuffer()
+        st = lib().fjx_diffusion_apply(self.handle, device_id, C.byref(stage), planes[0], planes[1], planes[2], width,
+                                       height, stream, err.ptr())
         return st, err.text
 
     def stage_timings(self, device_id: int = 0) -> list:

@@ -38,6 +38,15 @@ FjStatus fjx_grain_apply(FjCuda* cuda, int32_t device_id, const FjPreparedHostDa
                          float* density_m, float* density_y, int32_t width, int32_t height, uint64_t stream,
                          FjErrorBuffer* error);
 
+/* One camera or enlarger diffusion stage (launch_stage,
+ * src/Cuda/Diffusion/JuicerCudaDiffusion.cu:779-806, which the reference keeps
+ * internal) applied in place to three caller-owned R, G, B float planes of a
+ * width x height frame, asynchronously on `stream`: out = (1 - f) x + f (psf
+ * conv reflect101(x)). For callers with planes of their own and stage-level
+ * tests. */
+FjStatus fjx_diffusion_apply(FjCuda* cuda, int32_t device_id, const FjDiffusion* stage, float* red, float* green,
+                             float* blue, int32_t width, int32_t height, uint64_t stream, FjErrorBuffer* error);
+
 /* The spatial-DIR, visual-grain and film-effects descriptors of one frame,
  * composed from the host's frame math without a device: the composition
  * FjCudaCallAccess.frame_descriptors performs for an inspected frame
