@@ -24,6 +24,10 @@ struct FftPlanDev {
     // ns_magic[p] = ceil(2^32 / ns) divides butterfly indices by ns exactly.
     int tw_offset[kMaxFftPasses] = {};
     unsigned ns_magic[kMaxFftPasses] = {};
+    // In-place engine: pass p's base twiddles w_n^(k n/(ns R)) for k < ns at
+    // table[n + tw1_offset[p] + k]; a butterfly loads one and forms the other
+    // powers by squaring and multiplying (at most 2 log2 R products).
+    int tw1_offset[kMaxFftPasses] = {};
     // In-place engine (2,3,5-smooth n): radices from {2,3,4,5,6,8,9,10,12,15,
     // 16,18,20,24,25}, one butterfly per thread, `threads` >= n / radix of every pass.
     int inplace = 0;

@@ -307,8 +307,24 @@ template <> struct Dft<25> : DftCT<5, 5> {};
 // One in-place Stockham pass of radix R: thread j < n / R owns butterfly j.
 // All reads happen before the first barrier and all writes after it, so the
 // butterfly may overwrite the slots other threads read.
+// Powers w^1 .. w^(R-1) of a loaded w, applied to v[1..R-1]: even powers
+// square w^(r/2), odd ones multiply w^(r-1) by w.
 template <int R>
-__device__ __forceinline__ void pass_ip(float2* buf, int n, int ns, const float2* stw, unsigned magic) {
+__device__ __forceinline__ void twiddle_powers(float2* v, float2 w) {
+    float2 t[R];
+    t[1] = w;
+#pragma unroll
+    for (int r = 2; r < R; ++r) t[r] = (r & 1) ? cmul(t[r - 1], w) : cmul(t[r / 2], t[r / 2]);
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], t[r]);
+}
+
+// Radices up to 16 form their twiddles from one loaded base power (fewer
+// loads, registers to spare); larger radices load all R - 1 (`stw`, R - 1
+// per butterfly) to stay within the register budget.
+template <int R>
+__device__ __forceinline__ void pass_ip(float2* buf, int n, int ns, const float2* stw, const float2* stw1,
+                                        unsigned magic) {
     const int count = n / R;
     const int j = threadIdx.x;
     const bool active = j < count;
@@ -323,9 +339,13 @@ __device__ __forceinline__ void pass_ip(float2* buf, int n, int ns, const float2
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = buf[j + r * count];
         if (ns > 1) {
-            const float2* t = stw + k * (R - 1);
+            if constexpr (R <= 16) {
+                twiddle_powers<R>(v, stw1[k]);
+            } else {
+                const float2* t = stw + k * (R - 1);
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], t[r - 1]);
+                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], t[r - 1]);
+            }
         }
         Dft<R>::run(v);
         base = q * ns * R + k;
@@ -347,30 +367,31 @@ __device__ void fft_inplace(float2* buf, const FftPlanDev& plan, const float2* t
     for (int p = 0; p < plan.passes; ++p) {
         const int r = plan.radix[p];
         const float2* stw = table + plan.n + plan.tw_offset[p];
+        const float2* stw1 = table + plan.n + plan.tw1_offset[p];
         const unsigned magic = plan.ns_magic[p];
         switch (r) {
-            case 2: pass_ip<2>(buf, plan.n, ns, stw, magic); break;
-            case 3: pass_ip<3>(buf, plan.n, ns, stw, magic); break;
-            case 4: pass_ip<4>(buf, plan.n, ns, stw, magic); break;
-            case 5: pass_ip<5>(buf, plan.n, ns, stw, magic); break;
-            case 6: pass_ip<6>(buf, plan.n, ns, stw, magic); break;
-            case 8: pass_ip<8>(buf, plan.n, ns, stw, magic); break;
-            case 9: pass_ip<9>(buf, plan.n, ns, stw, magic); break;
-            case 10: pass_ip<10>(buf, plan.n, ns, stw, magic); break;
-            case 12: pass_ip<12>(buf, plan.n, ns, stw, magic); break;
-            case 15: pass_ip<15>(buf, plan.n, ns, stw, magic); break;
-            case 16: pass_ip<16>(buf, plan.n, ns, stw, magic); break;
+            case 2: pass_ip<2>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 3: pass_ip<3>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 4: pass_ip<4>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 5: pass_ip<5>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 6: pass_ip<6>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 8: pass_ip<8>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 9: pass_ip<9>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 10: pass_ip<10>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 12: pass_ip<12>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 15: pass_ip<15>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 16: pass_ip<16>(buf, plan.n, ns, stw, stw1, magic); break;
             case 18:
-                if constexpr (kMaxR >= 18) pass_ip<18>(buf, plan.n, ns, stw, magic);
+                if constexpr (kMaxR >= 18) pass_ip<18>(buf, plan.n, ns, stw, stw1, magic);
                 break;
             case 20:
-                if constexpr (kMaxR >= 20) pass_ip<20>(buf, plan.n, ns, stw, magic);
+                if constexpr (kMaxR >= 20) pass_ip<20>(buf, plan.n, ns, stw, stw1, magic);
                 break;
             case 24:
-                if constexpr (kMaxR >= 24) pass_ip<24>(buf, plan.n, ns, stw, magic);
+                if constexpr (kMaxR >= 24) pass_ip<24>(buf, plan.n, ns, stw, stw1, magic);
                 break;
             case 25:
-                if constexpr (kMaxR >= 25) pass_ip<25>(buf, plan.n, ns, stw, magic);
+                if constexpr (kMaxR >= 25) pass_ip<25>(buf, plan.n, ns, stw, stw1, magic);
                 break;
             default: break;  // not produced by the planner for in-place plans
         }

@@ -115,6 +115,12 @@ bool make_fft_plan(int n, FftPlanDev& plan) {
         offset += ns * (radices[i] - 1);
         ns *= radices[i];
     }
+    ns = 1;
+    for (int i = 0; i < plan.passes; ++i) {
+        plan.tw1_offset[i] = offset;
+        offset += ns;
+        ns *= radices[i];
+    }
     return true;
 }
 
@@ -130,7 +136,7 @@ std::vector<float> fft_twiddles(int n) {
     FftPlanDev plan;
     make_fft_plan(n, plan);
     size_t stage = 0;
-    for (int p = 0, ns = 1; p < plan.passes; ns *= plan.radix[p], ++p) stage += static_cast<size_t>(ns) * (plan.radix[p] - 1);
+    for (int p = 0, ns = 1; p < plan.passes; ns *= plan.radix[p], ++p) stage += static_cast<size_t>(ns) * plan.radix[p];
     std::vector<float> t(2 * (static_cast<size_t>(n) + stage));
     const double two_pi = 6.283185307179586476925286766559;
     auto put = [&](size_t i, long long e) {
@@ -146,6 +152,8 @@ std::vector<float> fft_twiddles(int n) {
             for (int r = 1; r < R; ++r)
                 put(static_cast<size_t>(n) + plan.tw_offset[p] + static_cast<size_t>(k) * (R - 1) + (r - 1),
                     static_cast<long long>(r) * k * stride);
+        for (int k = 0; k < ns; ++k)
+            put(static_cast<size_t>(n) + plan.tw1_offset[p] + static_cast<size_t>(k), static_cast<long long>(k) * stride);
     }
     return t;
 }
