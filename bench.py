@@ -150,12 +150,132 @@ def cpu_baseline(p, src, budget_s: float = 10.0):
                       f"per-pixel costs scaled to whole frames ({stages} diffusion stages)"}
 
 
+def config1(with_cpu: bool):
+    """512 x 512 negative direct route (Portra 400, Hanatos TC LUT, no spatial
+    stages, EV 0): the reference's CPU-runnable case. GPU frames/s over 50
+    frames; with `with_cpu` also the oracle's FP64 frames/s on the host cores
+    (1 warm-up + 3 timed) and the GPU's max |error| against it (linear out)."""
+    import numpy as np
+    import torch
+
+    from fj import prepared, runtime, synth
+    n = 512
+    p = prepared.simple_view(prepared.load("NegativeDirectHanatos"), encode=False)
+    src_np = synth.frame(n, n, seed=synth.DEFAULT_SEED)
+    src = torch.from_numpy(src_np).to("cuda:0")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    f = runtime.frame_for(src.data_ptr(), dst.data_ptr(), n, n, 4, stream=stream.cuda_stream)
+    r = runtime.shared_renderer()
+    ctx = r.inspect(f)
+    for _ in range(5):
+        r.render(f, p.record, context=ctx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    frames = 50
+    for _ in range(frames):
+        r.render(f, p.record, context=ctx)
+    torch.cuda.synchronize()
+    out = {"workload": "512x512 RGBA f32 negative direct route, Portra 400 (Hanatos), no spatial stages",
+           "gpu_frames_per_s": frames / (time.perf_counter() - t0)}
+    if with_cpu:
+        from oracle import oracle
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        plan = oracle.Plan(p)
+        _, lin, _ = plan.render(src_np, threads=cores)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            plan.render(src_np, threads=cores)
+        out["cpu_frames_per_s"] = 3 / (time.perf_counter() - t0)
+        out["cpu_cores"] = cores
+        got = dst.cpu().numpy()
+        out["max_abs_err_linear"] = float(np.abs(got[..., :3].astype(np.float64) - np.clip(lin, 0, 1)).max())
+    return out
+
+
+BATCH_SIZES = [(1920, 1080)] * 16 + [(3840, 2160)] * 16 + [(4096, 1716)] * 8 + [(6144, 3160)] * 8
+
+
+def batched(reps: int = 2):
+    """48 frames of mixed sizes (16 x 1920x1080, 16 x UHD, 8 x 4096x1716,
+    8 x 6144x3160; 377.4 Mpx) in the UHD workload's stage set, each frame with
+    its own seeded exposure (EV in [-2, 2]) and grain session seed, alternating
+    the negative print routes of the prepared set. Plain: 48 fj_cuda_render
+    calls on one stream; batched: one fjx_render_batch call (4 worker streams).
+    Both after one untimed pass; the batched frames must equal the plain ones
+    bit for bit."""
+    import numpy as np
+    import torch
+
+    from fj import descriptors, prepared, runtime, synth
+    rng = np.random.default_rng(20261203)
+    stream = torch.cuda.current_stream()
+    sources = {}
+    frames, records, dst_plain, dst_batch = [], [], [], []
+    for i, (w, h) in enumerate(BATCH_SIZES):
+        if (w, h) not in sources:
+            sources[(w, h)] = torch.from_numpy(synth.frame(w, h, seed=synth.DEFAULT_SEED + len(sources))).to("cuda:0")
+        src = sources[(w, h)]
+        px = descriptors.pixel_size_for(w, h)
+        case = "PrintPreflash" if i % 2 == 0 else "NegativePrintHanatos"
+        p = prepared.simple_view(prepared.load(case), unsharp=True)
+        p.record.film_exposure.manual_exposure_ev = float(rng.uniform(-2.0, 2.0))
+        descriptors.spatial_dir(p.record, 20.0, 200.0, 0.03, px, w, h)
+        prepared.with_halation(p, w, h, px)
+        mist = descriptors.diffusion_recipe(1, 0.5)
+        prepared.with_diffusion(p, w, h, px, camera=mist, enlarger=mist)
+        seed = int(rng.integers(1, 2 ** 31))
+        prepared.with_grain(p, w, h, px, session_seed=seed, noise_shape=(512, 512, 256))
+        records.append(p)
+        d0, d1 = torch.empty_like(src), torch.empty_like(src)
+        dst_plain.append(d0)
+        dst_batch.append(d1)
+        frames.append((src, d0, d1, w, h, px, seed))
+    plain_f = [runtime.frame_for(s.data_ptr(), d0.data_ptr(), w, h, 4, stream=stream.cuda_stream, pixel_size_um=px,
+                                 session_seed=seed) for s, d0, d1, w, h, px, seed in frames]
+    batch_f = [runtime.frame_for(s.data_ptr(), d1.data_ptr(), w, h, 4, stream=stream.cuda_stream, pixel_size_um=px,
+                                 session_seed=seed) for s, d0, d1, w, h, px, seed in frames]
+    r = runtime.shared_renderer()
+    ctx = r.inspect(plain_f[0])
+    subs = [runtime.submission_for(p.record) for p in records]
+
+    def plain():
+        for f, p, sb in zip(plain_f, records, subs):
+            r.render(f, p.record, submission=sb, context=ctx)
+
+    def batch():
+        st, text, _ = r.render_batch(batch_f, [p.record for p in records], streams=4, context=ctx,
+                                     submissions=subs)
+        if st.category != 0:
+            raise RuntimeError(text)
+
+    plain()
+    batch()
+    torch.cuda.synchronize()
+    timed = {}
+    for name, fn in (("plain", plain), ("batched", batch)):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        timed[name] = (time.perf_counter() - t0) / reps
+    same = all(torch.equal(a, b) for a, b in zip(dst_plain, dst_batch))
+    pixels = sum(w * h for w, h in BATCH_SIZES)
+    return {"workload": "48 mixed frames (16 x 1920x1080, 16 x 3840x2160, 8 x 4096x1716, 8 x 6144x3160), the UHD "
+                        "stage set, per-frame seeded EV in [-2, 2] and grain seed, Portra Endura / Portra print "
+                        "routes alternating",
+            "pixels": pixels, "plain_ms": timed["plain"] * 1e3, "batched_ms": timed["batched"] * 1e3,
+            "plain_frames_per_s": 48 / timed["plain"], "batched_frames_per_s": 48 / timed["batched"],
+            "batched_over_plain": timed["plain"] / timed["batched"], "batched_equals_plain": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 48-frame plain vs batched measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB float32 buffer three times (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -350,6 +470,9 @@ def main():
         },
         "frame_event_ms": ms_event,
     }
+    result["config1"] = config1(not args.no_cpu_baseline)
+    if not args.no_batch:
+        result["batched"] = batched()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(p, src_np)
     print(json.dumps(result))

@@ -344,6 +344,61 @@ FJ_EXPORT FjStatus fjx_diffusion_apply(FjCuda* cuda, int32_t device_id, const Fj
     }
 }
 
+FJ_EXPORT FjStatus fjx_render_batch(FjCuda* cuda, const FjCudaContext* context, int32_t count, const FjFrame* frames,
+                                    const FjSubmission* submissions, const FjPreparedHostData* const* prepared,
+                                    int32_t streams, FjAbortCallback abort_callback, FjRenderOutcome* outcomes,
+                                    FjErrorBuffer* error) {
+    try {
+        if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
+        CallGuard guard;
+        if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
+        std::lock_guard<std::mutex> lock(cuda->mu);
+        if (!cuda->open) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+        if (!context || count < 0 || (count > 0 && (!frames || !submissions || !prepared || !outcomes)))
+            return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null batch argument");
+        if (count == 0) {
+            write_error(error, "");
+            return make_status(FJ_STATUS_SUCCESS);
+        }
+        // One device and one caller stream for the whole batch.
+        const bool has_stream = (frames[0].flags & FJ_FRAME_STREAM_PRESENT) != 0;
+        for (int32_t i = 0; i < count; ++i) {
+            FjCudaContext c{};
+            const FjStatus st = inspect_frame(&frames[i], &c, error);
+            if (st.category != FJ_STATUS_SUCCESS) return st;
+            if (c.device_id != context->device_id || c.context != context->context)
+                return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "batch frame on another device");
+            if (!prepared[i]) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null prepared record in batch");
+            const bool hs = (frames[i].flags & FJ_FRAME_STREAM_PRESENT) != 0;
+            if (hs != has_stream || (hs && frames[i].stream != frames[0].stream))
+                return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "batch frames name different streams");
+        }
+        hipError_t he = hipSetDevice(context->device_id);
+        if (he != hipSuccess) return hip_fail(error, he, "hipSetDevice");
+        auto& slot = cuda->devices[context->device_id];
+        if (!slot) {
+            slot = std::make_unique<DeviceState>();
+            he = slot->init(context->device_id);
+            if (he != hipSuccess) {
+                slot.reset();
+                return hip_fail(error, he, "device state init");
+            }
+        }
+        hipStream_t caller = has_stream ? reinterpret_cast<hipStream_t>(frames[0].stream) : nullptr;
+        const FjStatus st =
+            render_batch(*slot, count, frames, submissions, prepared, streams, caller, abort_callback, outcomes, error);
+        if (!has_stream) {
+            he = hipStreamSynchronize(caller);
+            if (he != hipSuccess && st.category == FJ_STATUS_SUCCESS) return hip_fail(error, he, "batch synchronize");
+        }
+        return st;
+    } catch (const std::bad_alloc&) {
+        return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fjx_render_batch");
+    }
+}
+
 FJ_EXPORT int32_t fjx_stage_timings(FjCuda* cuda, int32_t device_id, FjxStageTiming* out, int32_t capacity) {
     try {
         if (!cuda || g_owner.load() != cuda || capacity < 0 || (capacity > 0 && !out)) return -1;

@@ -61,8 +61,14 @@ hipError_t DeviceState::init(int dev) {
         e = hipEventCreate(&stage_events[i]);
         if (e != hipSuccess) return e;
     }
-    e = hipEventCreateWithFlags(&workspace_done, hipEventDisableTiming);
-    if (e != hipSuccess) return e;
+    for (WorkspaceSlot& w : slots) {
+        e = hipEventCreateWithFlags(&w.done, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    for (hipEvent_t& b : batch_events) {
+        e = hipEventCreateWithFlags(&b, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
     e = fft_kernels_init(&max_line);
     if (e != hipSuccess) return e;
     if (halation_kernels_init() != hipSuccess) {
@@ -85,12 +91,20 @@ hipError_t DeviceState::release() {
     transfers.clear();
     lru.clear();
     table_bytes = 0;
-    if (workspace) keep(hipFree(workspace));
-    workspace = nullptr;
-    workspace_bytes = 0;
-    if (workspace_done) keep(hipEventDestroy(workspace_done));
-    workspace_done = nullptr;
-    workspace_used = false;
+    for (WorkspaceSlot& w : slots) {
+        if (w.ptr) keep(hipFree(w.ptr));
+        if (w.done) keep(hipEventDestroy(w.done));
+        w = WorkspaceSlot{};
+    }
+    active_slot = 0;
+    for (hipStream_t& st : workers) {
+        if (st) keep(hipStreamDestroy(st));
+        st = nullptr;
+    }
+    for (hipEvent_t& b : batch_events) {
+        if (b) keep(hipEventDestroy(b));
+        b = nullptr;
+    }
     for (int i = 0; i < kRing; ++i)
         if (error_events[i]) keep(hipEventDestroy(error_events[i]));
     for (int i = 0; i <= kMaxStages; ++i)
@@ -148,26 +162,31 @@ hipError_t DeviceState::table(uint64_t key, const void* host, size_t bytes, cons
 }
 
 hipError_t DeviceState::acquire_workspace(hipStream_t stream) {
-    if (!workspace_used || workspace_stream == stream) return hipSuccess;
-    return hipStreamWaitEvent(stream, workspace_done, 0);
+    WorkspaceSlot& w = slots[active_slot];
+    if (!w.used || w.stream == stream) return hipSuccess;
+    return hipStreamWaitEvent(stream, w.done, 0);
 }
 
 hipError_t DeviceState::release_workspace(hipStream_t stream) {
-    workspace_stream = stream;
-    workspace_used = true;
-    return hipEventRecord(workspace_done, stream);
+    WorkspaceSlot& w = slots[active_slot];
+    w.stream = stream;
+    w.used = true;
+    return hipEventRecord(w.done, stream);
 }
 
 hipError_t DeviceState::ensure_workspace(size_t bytes) {
-    if (workspace_bytes >= bytes) return hipSuccess;
-    if (workspace) {
-        const hipError_t e = hipFree(workspace);
-        workspace = nullptr;
-        workspace_bytes = 0;
+    WorkspaceSlot& w = slots[active_slot];
+    if (w.bytes >= bytes) return hipSuccess;
+    if (w.ptr) {
+        // hipFree waits for the device, so no queued user of the old
+        // allocation is still running.
+        const hipError_t e = hipFree(w.ptr);
+        w.ptr = nullptr;
+        w.bytes = 0;
         if (e != hipSuccess) return e;
     }
-    const hipError_t e = hipMalloc(&workspace, bytes);
-    if (e == hipSuccess) workspace_bytes = bytes;
+    const hipError_t e = hipMalloc(&w.ptr, bytes);
+    if (e == hipSuccess) w.bytes = bytes;
     return e;
 }
 
@@ -925,7 +944,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     if (ws_floats) {
         e = dev.ensure_workspace(static_cast<size_t>(ws_floats) * sizeof(float));
         if (e != hipSuccess) return outcome_hip(e, "workspace", error);
-        ws = static_cast<float*>(dev.workspace);
+        ws = static_cast<float*>(dev.workspace());
         e = dev.acquire_workspace(stream);
         if (e != hipSuccess) return outcome_hip(e, "workspace order", error);
     }
@@ -1122,6 +1141,51 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     return out;
 }
 
+FjStatus render_batch(DeviceState& dev, int count, const FjFrame* frames, const FjSubmission* submissions,
+                      const FjPreparedHostData* const* prepared, int streams, hipStream_t caller,
+                      FjAbortCallback abort_callback, FjRenderOutcome* outcomes, FjErrorBuffer* error) {
+    streams = std::max(1, std::min(streams, DeviceState::kWorkspaceSlots));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < streams; ++k) {
+        if (!dev.workers[k]) {
+            e = hipStreamCreateWithFlags(&dev.workers[k], hipStreamNonBlocking);
+            if (e != hipSuccess) return outcome_hip(e, "batch streams", error).status;
+        }
+    }
+    // Everything the caller queued before the batch precedes every frame.
+    e = hipEventRecord(dev.batch_events[DeviceState::kWorkspaceSlots], caller);
+    if (e != hipSuccess) return outcome_hip(e, "batch order", error).status;
+    for (int k = 0; k < streams; ++k) {
+        e = hipStreamWaitEvent(dev.workers[k], dev.batch_events[DeviceState::kWorkspaceSlots], 0);
+        if (e != hipSuccess) return outcome_hip(e, "batch order", error).status;
+    }
+    FjStatus first = status_of(FJ_STATUS_SUCCESS);
+    std::string first_text;
+    for (int i = 0; i < count; ++i) {
+        const int k = i % streams;
+        FjFrame f = frames[i];
+        f.stream = reinterpret_cast<uintptr_t>(dev.workers[k]);
+        f.flags = (f.flags | FJ_FRAME_STREAM_PRESENT) & ~FJ_FRAME_TRACE_VERBOSE;
+        dev.active_slot = k;
+        char text[512];
+        FjErrorBuffer local{text, sizeof(text), 0};
+        outcomes[i] = render_frame(dev, f, submissions[i], *prepared[i], abort_callback, &local);
+        if (outcomes[i].status.category != FJ_STATUS_SUCCESS && first.category == FJ_STATUS_SUCCESS) {
+            first = outcomes[i].status;
+            first_text = "frame " + std::to_string(i) + ": " + std::string(text, local.length);
+        }
+    }
+    dev.active_slot = 0;
+    // The caller's stream continues after every worker's last frame.
+    for (int k = 0; k < streams; ++k) {
+        e = hipEventRecord(dev.batch_events[k], dev.workers[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(caller, dev.batch_events[k], 0);
+        if (e != hipSuccess) return outcome_hip(e, "batch join", error).status;
+    }
+    put_error(error, first_text);
+    return first;
+}
+
 FjStatus grain_apply_planes(DeviceState& dev, const FjPreparedHostData& prepared, float* const density[3], int width,
                             int height, hipStream_t stream, FjErrorBuffer* error) {
     Diag d;
@@ -1153,7 +1217,7 @@ FjStatus grain_apply_planes(DeviceState& dev, const FjPreparedHostData& prepared
     if (e != hipSuccess) return outcome_hip(e, "workspace", error).status;
     e = dev.acquire_workspace(stream);
     if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
-    e = run_grain(gp, density, width, height, static_cast<float*>(dev.workspace), stream);
+    e = run_grain(gp, density, width, height, static_cast<float*>(dev.workspace()), stream);
     if (e != hipSuccess) return outcome_hip(e, "visual grain launch", error).status;
     e = dev.release_workspace(stream);
     if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
@@ -1184,7 +1248,7 @@ FjStatus diffusion_apply_planes(DeviceState& dev, const FjDiffusion& stage, floa
     if (e != hipSuccess) return outcome_hip(e, "workspace", error).status;
     e = dev.acquire_workspace(stream);
     if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
-    float* ws = static_cast<float*>(dev.workspace);
+    float* ws = static_cast<float*>(dev.workspace());
     DiffusionStage st;
     e = diffusion_prepare(dev, stage, width, height, ws, stream, st);
     if (e == hipErrorInvalidValue)

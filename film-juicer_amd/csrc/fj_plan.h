@@ -43,14 +43,24 @@ struct DeviceState {
     bool error_pending[kRing] = {};
     int error_next = 0;
 
-    // Frame workspace for staged (spatial) schedules. Work that uses it on
-    // one stream is ordered after the previous user's stream through
-    // `workspace_done` (recorded after each user's last launch).
-    void* workspace = nullptr;
-    size_t workspace_bytes = 0;
-    hipEvent_t workspace_done = nullptr;
-    hipStream_t workspace_stream = nullptr;
-    bool workspace_used = false;
+    // Frame workspaces for staged (spatial) schedules. Work that uses a slot
+    // on one stream is ordered after the slot's previous user's stream
+    // through `done` (recorded after each user's last launch). Single frames
+    // use slot 0; a batch spreads its frames over the slots, one worker
+    // stream per slot, so that frames overlap.
+    struct WorkspaceSlot {
+        void* ptr = nullptr;
+        size_t bytes = 0;
+        hipEvent_t done = nullptr;
+        hipStream_t stream = nullptr;
+        bool used = false;
+    };
+    static constexpr int kWorkspaceSlots = 4;
+    WorkspaceSlot slots[kWorkspaceSlots];
+    int active_slot = 0;
+    hipStream_t workers[kWorkspaceSlots] = {};  // batch worker streams, created on first use
+    hipEvent_t batch_events[kWorkspaceSlots + 1] = {};
+    void* workspace() const { return slots[active_slot].ptr; }
 
     // Stage timing of the last traced frame (FJ_FRAME_TRACE_VERBOSE).
     static constexpr int kMaxStages = 32;
@@ -64,7 +74,7 @@ struct DeviceState {
     // (host == nullptr: uninitialized device bytes the caller fills). Tables
     // used by the current frame are never evicted during it.
     hipError_t table(uint64_t key, const void* host, size_t bytes, const void** out);
-    hipError_t ensure_workspace(size_t bytes);
+    hipError_t ensure_workspace(size_t bytes);  // of the active slot
     // Orders `stream` after the workspace's previous user; call before the
     // first launch that touches the workspace, then release after the last.
     hipError_t acquire_workspace(hipStream_t stream);
@@ -76,6 +86,14 @@ struct DeviceState {
 FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSubmission& submission,
                              const FjPreparedHostData& prepared, FjAbortCallback abort_callback,
                              FjErrorBuffer* error);
+
+// Renders `count` frames of one device, frame i on worker stream i % streams
+// with workspace slot i % streams, all ordered after the work already queued
+// on `caller` and joined back into it. outcomes[i] is frame i's outcome;
+// returns the first failing frame's status, else success.
+FjStatus render_batch(DeviceState& dev, int count, const FjFrame* frames, const FjSubmission* submissions,
+                      const FjPreparedHostData* const* prepared, int streams, hipStream_t caller,
+                      FjAbortCallback abort_callback, FjRenderOutcome* outcomes, FjErrorBuffer* error);
 
 // The visual-grain stage of `prepared` applied to three caller-owned C/M/Y
 // density planes (device memory, width x height each) on `stream`.

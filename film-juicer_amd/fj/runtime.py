@@ -49,6 +49,11 @@ def lib():
         L.fjx_grain_apply.restype = abi.FjStatus
         L.fjx_grain_apply.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjPreparedHostData), C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, eb]
+        L.fjx_render_batch.restype = abi.FjStatus
+        L.fjx_render_batch.argtypes = [C.c_void_p, C.POINTER(abi.FjCudaContext), C.c_int32, C.POINTER(abi.FjFrame),
+                                       C.POINTER(abi.FjSubmission), C.POINTER(C.POINTER(abi.FjPreparedHostData)),
+                                       C.c_int32, abi.FjAbortCallback, C.POINTER(abi.FjRenderOutcome),
+                                       C.POINTER(abi.FjErrorBuffer)]
         L.fjx_diffusion_apply.restype = abi.FjStatus
         L.fjx_diffusion_apply.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjDiffusion), C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(abi.FjErrorBuffer)]
@@ -194,6 +199,21 @@ class Renderer:
         st = lib().fjx_grain_apply(self.handle, device_id, C.byref(prepared_record), planes[0], planes[1], planes[2],
                                    width, height, stream, err.ptr())
         return st, err.text
+
+    def render_batch(self, frames, records, streams: int = 4, context=None, submissions=None):
+        """Renders frames[i] with records[i] in one call; returns (FjStatus,
+        text, [FjRenderOutcome])."""
+        n = len(frames)
+        fr = (abi.FjFrame * n)(*frames)
+        subs = submissions or [submission_for(r, f.time_frames) for f, r in zip(frames, records)]
+        sb = (abi.FjSubmission * n)(*subs)
+        ptrs = (C.POINTER(abi.FjPreparedHostData) * n)(*[C.pointer(r) for r in records])
+        outs = (abi.FjRenderOutcome * n)()
+        ctx = context if context is not None else (self.inspect(frames[0]) if frames else abi.FjCudaContext())
+        err = ErrorBuffer()
+        st = lib().fjx_render_batch(self.handle, C.byref(ctx), n, fr, sb, ptrs, streams,
+                                    abi.FjAbortCallback(abi.FjAbortQuery(0), None), outs, err.ptr())
+        return st, err.text, list(outs)
 
     def diffusion_apply(self, stage, planes, width: int, height: int, stream: int = 0, device_id: int = 0):
         """Runs one diffusion stage on three device planes (addresses R, G, B);

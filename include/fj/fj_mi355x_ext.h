@@ -38,6 +38,19 @@ FjStatus fjx_grain_apply(FjCuda* cuda, int32_t device_id, const FjPreparedHostDa
                          float* density_m, float* density_y, int32_t width, int32_t height, uint64_t stream,
                          FjErrorBuffer* error);
 
+/* Renders `count` frames of one device in one call (no reference
+ * counterpart: the reference renders one frame per fj_cuda_render). Each
+ * frame keeps fj_cuda_render's semantics and validation; frame i runs on
+ * internal worker stream i % streams (1..4) with its own workspace, so that
+ * consecutive frames overlap on the GPU; all are ordered after the work
+ * queued on the frames' stream before the call and the stream continues after
+ * all of them (every frame must name the same stream, or none: then the call
+ * synchronizes). outcomes[i] receives frame i's outcome; the return value is
+ * the first failing frame's status, with its index in the error text. */
+FjStatus fjx_render_batch(FjCuda* cuda, const FjCudaContext* context, int32_t count, const FjFrame* frames,
+                          const FjSubmission* submissions, const FjPreparedHostData* const* prepared, int32_t streams,
+                          FjAbortCallback abort_callback, FjRenderOutcome* outcomes, FjErrorBuffer* error);
+
 /* One camera or enlarger diffusion stage (launch_stage,
  * src/Cuda/Diffusion/JuicerCudaDiffusion.cu:779-806, which the reference keeps
  * internal) applied in place to three caller-owned R, G, B float planes of a

@@ -161,3 +161,14 @@ def test_synchronous_call_without_stream():
     runtime.shared_renderer().render(f, p.record)
     _, lin, _ = oracle.Plan(p).render(src)
     check(d.cpu().numpy(), None, lin, src, True)
+
+
+def test_config1_negative_direct_512():
+    """The reference's CPU-runnable case: 512 x 512, Portra 400 negative,
+    Hanatos TC LUT, direct scan, no spatial stages, EV 0."""
+    p = prepared.simple_view(prepared.load("NegativeDirectHanatos"), encode=False)
+    src = synth.frame(512, 512, seed=synth.DEFAULT_SEED)
+    got, _ = render_gpu(p, src)
+    _, lin, _ = oracle.Plan(p).render(src, threads=8)
+    err = check(got, None, lin, src, True)
+    print(f"512x512 negative direct: max {err.max():.3e} mean {err.mean():.3e}")
