@@ -411,11 +411,23 @@ def visual_grain(plan: dict, r: dict) -> abi.FjVisualGrain:
     return g
 
 
+_NOISE = {}
+
+
 def synthetic_noise(stbn_shape=(128, 128, 64), wang_size: int = 64, seed: int = 2024):
     """(stbn u8[frames][h][w], wang tiles u8[16][s][s], wang lut u8[16]) of the
     reference assets' layout: uniform bytes (a placeholder for the blue-noise
     volume; the sampler reads any byte volume the same way) and a two-colour
-    Wang set whose LUT maps each edge colouring to a tile."""
+    Wang set whose LUT maps each edge colouring to a tile. One set per
+    (shape, size, seed) per process, as the plug-in loads its assets once:
+    records built with the same arguments share the same arrays."""
+    key = (tuple(stbn_shape), wang_size, seed)
+    if key not in _NOISE:
+        _NOISE[key] = _make_noise(stbn_shape, wang_size, seed)
+    return _NOISE[key]
+
+
+def _make_noise(stbn_shape, wang_size, seed):
     w, h, frames = stbn_shape
     rng = np.random.default_rng(seed)
     stbn = rng.integers(0, 256, size=(frames, h, w), dtype=np.uint8)
