@@ -55,8 +55,14 @@ __device__ __forceinline__ void film_exposure(const PixelParams& p, const float4
     camera_exposure(p.ex, mallett, in, E);
 }
 
-template <bool kPrint, bool kLds, int kMode>
-__global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int tiles) {
+// What a development stage writes: the full chain to the frame (or linear
+// planes), the density planes only (grain follows), or the print-linear
+// planes only (enlarger diffusion follows). Fixed per launch, so a split
+// stage is compiled without the code of the stages after it.
+enum PixelOut : int { kOutFull = 0, kOutDensity = 1, kOutPrintLinear = 2 };
+
+template <bool kPrint, bool kLds, int kMode, int kOut>
+__device__ __forceinline__ void pixel_stage_body(const PixelParams& p, int tiles) {
     extern __shared__ float4 lds_raw[];
     float* lds = reinterpret_cast<float*>(lds_raw);
     LdsLayout L;
@@ -126,21 +132,17 @@ __global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int til
             film_exposure(p, L.mallett, in, idx, E);
             errors |= develop_film(p.dev, L.film, E, D);
         }
-        if constexpr (kMode == kFused || kMode == kDirFinal) {
-            if (p.dens[0]) {
+        if constexpr (kOut == kOutDensity) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) p.dens[c][idx] = D[c];
-                continue;
-            }
-        }
-        if constexpr (kPrint && kMode != kEnlargerFinal) {
-            if (p.plin[0]) {
-                float lin[3];
-                print_linear(p.pr, L.spec, D, lin);
+            for (int c = 0; c < 3; ++c) p.dens[c][idx] = D[c];
+            continue;
+        } else if constexpr (kOut == kOutPrintLinear) {
+            float lin[3];
+            print_linear(p.pr, L.spec, D, lin);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) p.plin[c][idx] = lin[c];
-                continue;
-            }
+            for (int c = 0; c < 3; ++c) p.plin[c][idx] = lin[c];
+            continue;
+        } else if constexpr (kPrint && kMode != kEnlargerFinal) {
             print_develop(p.pr, L.spec, L.print, D);
         }
         float rgb[3];
@@ -171,6 +173,24 @@ __global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int til
     if (errors && p.error_word) atomicOr(p.error_word, errors);
 }
 
+template <bool kPrint, bool kLds, int kMode, int kOut = kOutFull>
+__global__ __launch_bounds__(256) void pixel_stage_kernel(PixelParams p, int tiles) {
+    pixel_stage_body<kPrint, kLds, kMode, kOut>(p, tiles);
+}
+
+// The print + scan stage after enlarger diffusion: capped at five waves per
+// SIMD it fits 96 VGPRs without scratch (from three waves at 129).
+template <>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void
+pixel_stage_kernel<true, true, kEnlargerFinal, kOutFull>(PixelParams p, int tiles) {
+    pixel_stage_body<true, true, kEnlargerFinal, kOutFull>(p, tiles);
+}
+template <>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void
+pixel_stage_kernel<true, false, kEnlargerFinal, kOutFull>(PixelParams p, int tiles) {
+    pixel_stage_body<true, false, kEnlargerFinal, kOutFull>(p, tiles);
+}
+
 __global__ void clear_word_kernel(unsigned int* word) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *word = 0u;
 }
@@ -195,18 +215,41 @@ hipError_t launch_clear_word(unsigned int* word, hipStream_t stream) {
     return hipGetLastError();
 }
 
-template <int kMode>
-static hipError_t launch_mode(const PixelParams& p, int grid, int tiles, size_t lds, bool use_lds, hipStream_t stream) {
+// Output kind of a development stage (the order of the checks is the
+// chain's: densities for grain first, then print-linear planes).
+static int out_kind(const PixelParams& p, int mode) {
+    if ((mode == kFused || mode == kDirFinal) && p.dens[0]) return kOutDensity;
+    if (p.print && mode != kEnlargerFinal && p.plin[0]) return kOutPrintLinear;
+    return kOutFull;
+}
+
+template <int kMode, int kOut>
+static void launch_out(const PixelParams& p, int grid, int tiles, size_t lds, bool use_lds, hipStream_t stream) {
     if (p.print) {
         if (use_lds)
-            hipLaunchKernelGGL((pixel_stage_kernel<true, true, kMode>), dim3(grid), dim3(256), lds, stream, p, tiles);
+            hipLaunchKernelGGL((pixel_stage_kernel<true, true, kMode, kOut>), dim3(grid), dim3(256), lds, stream, p, tiles);
         else
-            hipLaunchKernelGGL((pixel_stage_kernel<true, false, kMode>), dim3(grid), dim3(256), 0, stream, p, tiles);
+            hipLaunchKernelGGL((pixel_stage_kernel<true, false, kMode, kOut>), dim3(grid), dim3(256), 0, stream, p, tiles);
     } else {
         if (use_lds)
-            hipLaunchKernelGGL((pixel_stage_kernel<false, true, kMode>), dim3(grid), dim3(256), lds, stream, p, tiles);
+            hipLaunchKernelGGL((pixel_stage_kernel<false, true, kMode, kOut>), dim3(grid), dim3(256), lds, stream, p, tiles);
         else
-            hipLaunchKernelGGL((pixel_stage_kernel<false, false, kMode>), dim3(grid), dim3(256), 0, stream, p, tiles);
+            hipLaunchKernelGGL((pixel_stage_kernel<false, false, kMode, kOut>), dim3(grid), dim3(256), 0, stream, p, tiles);
+    }
+}
+
+template <int kMode>
+static hipError_t launch_mode(const PixelParams& p, int grid, int tiles, size_t lds, bool use_lds, hipStream_t stream) {
+    switch (out_kind(p, kMode)) {
+        case kOutDensity:
+            if constexpr (kMode == kFused || kMode == kDirFinal)
+                launch_out<kMode, kOutDensity>(p, grid, tiles, lds, use_lds, stream);
+            break;
+        case kOutPrintLinear:
+            if constexpr (kMode != kEnlargerFinal)
+                launch_out<kMode, kOutPrintLinear>(p, grid, tiles, lds, use_lds, stream);
+            break;
+        default: launch_out<kMode, kOutFull>(p, grid, tiles, lds, use_lds, stream); break;
     }
     return hipGetLastError();
 }
@@ -255,18 +298,7 @@ hipError_t launch_pixel_stage(const PixelParams& p, int mode, int grid_limit, hi
             q.dev.dir_mode = 0;
             const size_t lq = fused_lds_bytes(q);
             const bool lds_ok = lq <= 64 * 1024 && (!q.print || q.pr.n <= kMaxLdsCurve);
-            if (q.print) {
-                if (lds_ok)
-                    hipLaunchKernelGGL((pixel_stage_kernel<true, true, kDensityFinal>), dim3(grid), dim3(block), lq, stream, q, tiles);
-                else
-                    hipLaunchKernelGGL((pixel_stage_kernel<true, false, kDensityFinal>), dim3(grid), dim3(block), 0, stream, q, tiles);
-            } else {
-                if (lds_ok)
-                    hipLaunchKernelGGL((pixel_stage_kernel<false, true, kDensityFinal>), dim3(grid), dim3(block), lq, stream, q, tiles);
-                else
-                    hipLaunchKernelGGL((pixel_stage_kernel<false, false, kDensityFinal>), dim3(grid), dim3(block), 0, stream, q, tiles);
-            }
-            return hipGetLastError();
+            return launch_mode<kDensityFinal>(q, grid, tiles, lq, lds_ok, stream);
         }
         case kEnlargerFinal: {
             // Print curves, scanner and colour only: no film tables.
