@@ -326,7 +326,10 @@ template <int R>
 __device__ __forceinline__ void pass_ip(float2* buf, int n, int ns, const float2* stw, const float2* stw1,
                                         unsigned magic) {
     const int count = n / R;
-    const int j = threadIdx.x;
+    // A fresh value per pass: the pass's R addresses are computed inside it
+    // rather than hoisted out of the pass loop and held live across passes.
+    int j = threadIdx.x;
+    asm volatile("" : "+v"(j));
     const bool active = j < count;
     float2 v[R];
     int base = 0;
@@ -427,6 +430,27 @@ __device__ __forceinline__ int reflect101(int i, int n) {
 // Makhoul permutation index: position of x[i] in v.
 __device__ __forceinline__ int makhoul_pos(int i, int n) { return (i & 1) ? n - 1 - (i >> 1) : (i >> 1); }
 
+// The strided loop i = first, first + step, ... < end in groups of kU: a
+// group's global loads (load(i) -> T) are all issued before any of them is
+// used (use(i, T)), so a thread waits for memory once per group rather than
+// once per element.
+template <int kU, typename T, typename Load, typename Use>
+__device__ __forceinline__ void batched(int first, int end, int step, Load load, Use use) {
+    for (int i0 = first; i0 < end; i0 += kU * step) {
+        T v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 + u * step;
+            if (i < end) v[u] = load(i);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 + u * step;
+            if (i < end) use(i, v[u]);
+        }
+    }
+}
+
 // DCT-II of two real lines packed as (re, im) of one complex line held in
 // LDS (Makhoul order). On return out_a / out_b (global) hold X_a[k], X_b[k].
 template <int kEng>
@@ -435,15 +459,16 @@ __device__ void dct2_pair(float2* a, float2* b, const FftPlanDev& plan, const fl
     const int n = plan.n;
     float2* V = fft_line<kEng>(a, b, plan, table);
     // V = Va + i Vb with Va, Vb spectra of real sequences.
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const float2 z = V[k];
-        const float2 zc = V[k == 0 ? 0 : n - k];
-        const float2 va = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
-        const float2 vb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
-        const float2 w = half_tw[k];  // exp(-i*pi*k/(2n))
-        out_a[k] = w.x * va.x - w.y * va.y;
-        if (out_b) out_b[k] = w.x * vb.x - w.y * vb.y;
-    }
+    batched<8, float2>(
+        threadIdx.x, n, blockDim.x, [&](int k) { return half_tw[k]; },  // exp(-i*pi*k/(2n))
+        [&](int k, float2 w) {
+            const float2 z = V[k];
+            const float2 zc = V[k == 0 ? 0 : n - k];
+            const float2 va = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+            const float2 vb = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+            out_a[k] = w.x * va.x - w.y * va.y;
+            if (out_b) out_b[k] = w.x * vb.x - w.y * vb.y;
+        });
 }
 
 // Packed inverse input for one pair position: conj(e^{i pi k/2n}(X_a[k] - i X_a[n-k])
@@ -475,21 +500,22 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct2_rows_kernel(DctRowsArgs a
     const bool has1 = row0 + 1 < args.rows;
     const float* in = args.in[plane];
     float* out = args.out[plane];
+    auto put = [&](int i, float2 v) { a[makhoul_pos(i, n)] = v; };
     if (args.pad >= 0) {
         const float* r0 = in + static_cast<long long>(row0) * args.in_stride;
         const float* r1 = has1 ? r0 + args.in_stride : r0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const int si = reflect101(i - args.pad, args.src_w);
-            a[makhoul_pos(i, n)] = make_float2(r0[si], has1 ? r1[si] : 0.0f);
-        }
+        batched<8, float2>(
+            threadIdx.x, n, blockDim.x,
+            [&](int i) {
+                const int si = reflect101(i - args.pad, args.src_w);
+                return make_float2(r0[si], has1 ? r1[si] : 0.0f);
+            },
+            put);
     } else {
         const float* r0 = in + static_cast<long long>(row0) * args.in_stride;
         const float* r1 = has1 ? in + static_cast<long long>(row0 + 1) * args.in_stride : nullptr;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const float v0 = r0[i];
-            const float v1 = has1 ? r1[i] : 0.0f;
-            a[makhoul_pos(i, n)] = make_float2(v0, v1);
-        }
+        batched<8, float2>(
+            threadIdx.x, n, blockDim.x, [&](int i) { return make_float2(r0[i], has1 ? r1[i] : 0.0f); }, put);
     }
     __syncthreads();
     dct2_pair<kEng>(a, b, plan, args.table, args.half_tw, out + static_cast<long long>(row0) * args.out_stride,
@@ -512,12 +538,23 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct3_rows_kernel(DctRowsArgs a
     float* out = args.out[plane];
     const float* r0 = in + static_cast<long long>(row0) * args.in_stride;
     const float* r1 = has1 ? in + static_cast<long long>(row0 + 1) * args.in_stride : r0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const int km = k == 0 ? -1 : n - k;
-        const float xa = r0[k], xam = km < 0 ? 0.0f : r0[km];
-        const float xb = has1 ? r1[k] : 0.0f, xbm = (has1 && km >= 0) ? r1[km] : 0.0f;
-        a[k] = dct3_pack(xa, xam, xb, xbm, args.half_tw[k]);
-    }
+    struct Pack {
+        float xa, xam, xb, xbm;
+        float2 w;
+    };
+    batched<8, Pack>(
+        threadIdx.x, n, blockDim.x,
+        [&](int k) {
+            const int km = k == 0 ? -1 : n - k;
+            Pack v;
+            v.xa = r0[k];
+            v.xam = km < 0 ? 0.0f : r0[km];
+            v.xb = has1 ? r1[k] : 0.0f;
+            v.xbm = (has1 && km >= 0) ? r1[km] : 0.0f;
+            v.w = args.half_tw[k];
+            return v;
+        },
+        [&](int k, const Pack& v) { a[k] = dct3_pack(v.xa, v.xam, v.xb, v.xbm, v.w); });
     __syncthreads();
     float2* res = fft_line<kEng>(a, b, plan, args.table);
     const float inv = 1.0f / static_cast<float>(n);
@@ -527,12 +564,16 @@ __global__ __launch_bounds__(kMaxRowThreads) void dct3_rows_kernel(DctRowsArgs a
         const float f = args.fraction, keep = 1.0f - args.fraction;
         const long long y0 = row0;
         const float* m = args.mix[plane];
-        for (int i = threadIdx.x; i < args.src_w; i += blockDim.x) {
-            const float2 z = res[makhoul_pos(i + pad, n)];
-            const long long o = y0 * args.out_stride + i;
-            out[o] = keep * m[o] + f * (z.x * inv);
-            if (has1) out[o + args.out_stride] = keep * m[o + args.out_stride] + f * (-z.y * inv);
-        }
+        const float* m0 = m + y0 * args.out_stride;
+        float* q0 = out + y0 * args.out_stride;
+        batched<8, float2>(
+            threadIdx.x, args.src_w, blockDim.x,
+            [&](int i) { return make_float2(m0[i], has1 ? m0[i + args.out_stride] : 0.0f); },
+            [&](int i, float2 mv) {
+                const float2 z = res[makhoul_pos(i + pad, n)];
+                q0[i] = keep * mv.x + f * (z.x * inv);
+                if (has1) q0[i + args.out_stride] = keep * mv.y + f * (-z.y * inv);
+            });
         return;
     }
     float* o0 = out + static_cast<long long>(row0) * args.out_stride;
@@ -575,9 +616,9 @@ __device__ __forceinline__ void dct_filter_rows_body(const DctRowsArgs& args, co
     float* out = args.out[plane];
     const float* r0 = in + static_cast<long long>(row0) * args.in_stride;
     const float* r1 = has1 ? in + static_cast<long long>(row0 + 1) * args.in_stride : nullptr;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        a[makhoul_pos(i, n)] = make_float2(r0[i], has1 ? r1[i] : 0.0f);
-    }
+    batched<8, float2>(
+        threadIdx.x, n, blockDim.x, [&](int i) { return make_float2(r0[i], has1 ? r1[i] : 0.0f); },
+        [&](int i, float2 v) { a[makhoul_pos(i, n)] = v; });
     __syncthreads();
     // Per-row component factors for rows row0 and row0 + 1, or the plane's
     // full transfer table.
@@ -594,30 +635,43 @@ __device__ __forceinline__ void dct_filter_rows_body(const DctRowsArgs& args, co
     if constexpr (kInplace<kEng>) {
         // Spectrum product and inverse packing, in place: positions k and
         // n - k depend only on V[k] and V[n - k], so each thread owns a pair.
-        for (int p = threadIdx.x; p <= n / 2; p += blockDim.x) {
-            const int k1 = p, k2 = p == 0 ? 0 : n - p;
-            const float2 z1 = V[k1], z2 = V[k2];
-            const float2 w1 = args.half_tw[k1], w2 = args.half_tw[k2];
-            // DCT-II coefficients of both lines at k1 and k2, times T.
-            const float2 va1 = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
-            const float2 vb1 = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
-            const float2 va2 = make_float2(0.5f * (z2.x + z1.x), 0.5f * (z2.y - z1.y));
-            const float2 vb2 = make_float2(0.5f * (z2.y + z1.y), -0.5f * (z2.x - z1.x));
-            const float ta1 = transfer_at<kFullTable>(tf, full, fa, row0, n, k1);
-            const float ta2 = transfer_at<kFullTable>(tf, full, fa, row0, n, k2);
-            const float tb1 = has1 ? transfer_at<kFullTable>(tf, full, fb, row1, n, k1) : 0.0f;
-            const float tb2 = has1 ? transfer_at<kFullTable>(tf, full, fb, row1, n, k2) : 0.0f;
-            const float xa1 = (w1.x * va1.x - w1.y * va1.y) * ta1;
-            const float xb1 = (w1.x * vb1.x - w1.y * vb1.y) * tb1;
-            const float xa2 = (w2.x * va2.x - w2.y * va2.y) * ta2;
-            const float xb2 = (w2.x * vb2.x - w2.y * vb2.y) * tb2;
-            if (p == 0) {
-                V[0] = dct3_pack(xa1, 0.0f, xb1, 0.0f, w1);
-            } else {
-                V[k1] = dct3_pack(xa1, xa2, xb1, xb2, w1);
-                if (k2 != k1) V[k2] = dct3_pack(xa2, xa1, xb2, xb1, w2);
-            }
-        }
+        struct Coef {
+            float2 w1, w2;
+            float ta1, ta2, tb1, tb2;
+        };
+        batched<4, Coef>(
+            threadIdx.x, n / 2 + 1, blockDim.x,
+            [&](int p) {
+                const int k1 = p, k2 = p == 0 ? 0 : n - p;
+                Coef c;
+                c.w1 = args.half_tw[k1];
+                c.w2 = args.half_tw[k2];
+                c.ta1 = transfer_at<kFullTable>(tf, full, fa, row0, n, k1);
+                c.ta2 = transfer_at<kFullTable>(tf, full, fa, row0, n, k2);
+                c.tb1 = has1 ? transfer_at<kFullTable>(tf, full, fb, row1, n, k1) : 0.0f;
+                c.tb2 = has1 ? transfer_at<kFullTable>(tf, full, fb, row1, n, k2) : 0.0f;
+                return c;
+            },
+            [&](int p, const Coef& c) {
+                const int k1 = p, k2 = p == 0 ? 0 : n - p;
+                const float2 z1 = V[k1], z2 = V[k2];
+                const float2 w1 = c.w1, w2 = c.w2;
+                // DCT-II coefficients of both lines at k1 and k2, times T.
+                const float2 va1 = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+                const float2 vb1 = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
+                const float2 va2 = make_float2(0.5f * (z2.x + z1.x), 0.5f * (z2.y - z1.y));
+                const float2 vb2 = make_float2(0.5f * (z2.y + z1.y), -0.5f * (z2.x - z1.x));
+                const float xa1 = (w1.x * va1.x - w1.y * va1.y) * c.ta1;
+                const float xb1 = (w1.x * vb1.x - w1.y * vb1.y) * c.tb1;
+                const float xa2 = (w2.x * va2.x - w2.y * va2.y) * c.ta2;
+                const float xb2 = (w2.x * vb2.x - w2.y * vb2.y) * c.tb2;
+                if (p == 0) {
+                    V[0] = dct3_pack(xa1, 0.0f, xb1, 0.0f, w1);
+                } else {
+                    V[k1] = dct3_pack(xa1, xa2, xb1, xb2, w1);
+                    if (k2 != k1) V[k2] = dct3_pack(xa2, xa1, xb2, xb1, w2);
+                }
+            });
     } else {
         float2* other = V == a ? b : a;
         float* spec = reinterpret_cast<float*>(other);  // n floats X_a, then n floats X_b

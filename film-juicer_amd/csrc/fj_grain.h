@@ -92,6 +92,7 @@ enum GrainEltOp : int {
     kEltMixShared,        // a = san(amp (sw shared + iw ind))
     kEltReconstruct,      // a = san((add_mean ? a : 0) + s b)
     kEltCopy,             // a = b
+    kEltMixReconstruct,   // kEltMixShared into a temporary m (ind = b), then a = san((add_mean ? a : 0) + s m)
 };
 
 struct GrainEltArgs {

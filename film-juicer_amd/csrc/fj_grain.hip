@@ -56,12 +56,32 @@ __device__ __forceinline__ uint32_t hash_u32(int x, int y, uint64_t seed, uint64
     return static_cast<uint32_t>(h);
 }
 
+// True when every active lane holds the same (a, b); ua, ub are then that
+// pair as wave-uniform values, so integer work on them runs on the scalar
+// unit once per wave instead of once per lane (the bits are the same).
+__device__ __forceinline__ bool wave_uniform(int a, int b, int& ua, int& ub) {
+    ua = __builtin_amdgcn_readfirstlane(a);
+    ub = __builtin_amdgcn_readfirstlane(b);
+    return __all(a == ua && b == ub);
+}
+
 __device__ float value_noise(float x, float y, uint64_t seed) {
     const int ix = static_cast<int>(floorf(x)), iy = static_cast<int>(floorf(y));
     const float fx = x - static_cast<float>(ix), fy = y - static_cast<float>(iy);
     const float sx = smooth(fx), sy = smooth(fy);
-    const float v00 = spatial_hash01(ix, iy, seed), v10 = spatial_hash01(ix + 1, iy, seed);
-    const float v01 = spatial_hash01(ix, iy + 1, seed), v11 = spatial_hash01(ix + 1, iy + 1, seed);
+    float v00, v10, v01, v11;
+    int ux, uy;
+    if (wave_uniform(ix, iy, ux, uy)) {  // the wave lies in one lattice cell
+        v00 = spatial_hash01(ux, uy, seed);
+        v10 = spatial_hash01(ux + 1, uy, seed);
+        v01 = spatial_hash01(ux, uy + 1, seed);
+        v11 = spatial_hash01(ux + 1, uy + 1, seed);
+    } else {
+        v00 = spatial_hash01(ix, iy, seed);
+        v10 = spatial_hash01(ix + 1, iy, seed);
+        v01 = spatial_hash01(ix, iy + 1, seed);
+        v11 = spatial_hash01(ix + 1, iy + 1, seed);
+    }
     const float v0 = v00 + (v10 - v00) * sx;
     const float v1 = v01 + (v11 - v01) * sx;
     return v0 + (v1 - v0) * sy;
@@ -118,11 +138,17 @@ __device__ void wang_offsets(const GrainDev& g, uint64_t ax, uint64_t ay, int& o
     const uint64_t s = g.clip_token ? g.clip_token : g.session_seed;
     const uint64_t base = s ? s : 1ull;
     const uint32_t c = static_cast<uint32_t>(g.wang_colors);
-    const uint32_t l = hash_u32(static_cast<int>(mx), static_cast<int>(my), base, 0xA5A5A5A5u) % c;
-    const uint32_t r = hash_u32(static_cast<int>(mx + 1), static_cast<int>(my), base, 0x5A5A5A5Au) % c;
-    const uint32_t t = hash_u32(static_cast<int>(mx), static_cast<int>(my), base, 0xC3C3C3C3u) % c;
-    const uint32_t b = hash_u32(static_cast<int>(mx), static_cast<int>(my + 1), base, 0x3C3C3C3Cu) % c;
-    const int tile = g.wang_lut[((static_cast<size_t>(l) * c + r) * c + t) * c + b];
+    auto cell_tile = [&](int cx, int cy) {
+        const uint32_t l = hash_u32(cx, cy, base, 0xA5A5A5A5u) % c;
+        const uint32_t r = hash_u32(cx + 1, cy, base, 0x5A5A5A5Au) % c;
+        const uint32_t t = hash_u32(cx, cy, base, 0xC3C3C3C3u) % c;
+        const uint32_t b = hash_u32(cx, cy + 1, base, 0x3C3C3C3Cu) % c;
+        return static_cast<int>(g.wang_lut[((static_cast<size_t>(l) * c + r) * c + t) * c + b]);
+    };
+    int ux, uy;
+    const int tile = wave_uniform(static_cast<int>(mx), static_cast<int>(my), ux, uy)
+                         ? cell_tile(ux, uy)
+                         : cell_tile(static_cast<int>(mx), static_cast<int>(my));
     const int w = g.wang_w, h = g.wang_h;
     int tx = static_cast<int>(floorf(fx * static_cast<float>(w)));
     int ty = static_cast<int>(floorf(fy * static_cast<float>(h)));
@@ -187,12 +213,27 @@ __device__ __forceinline__ Stream make_stream(const GrainDev& g, uint64_t seed, 
 __device__ int poisson(float lambda, Stream& s) {
     if (!fin(lambda) || !(lambda > 0.0f)) return 0;
     if (lambda < 10.0f) {
+        // Knuth's product loop (fast_poisson_device,
+        // JuicerCudaFilmPipeline.cu:1997-2025). A draw's hash depends only on
+        // the seed and the draw index, the same for every pixel of the wave,
+        // so four draws' STBN gathers are issued together and consumed in
+        // order; a pixel stops at the same draw as the one-at-a-time loop.
         const float L = det_expf(-lambda);
         float p = 1.0f;
         int k = 0;
-        while (p > L && k < 1024) {
-            ++k;
-            p *= fminf(fmaxf(s.uniform(), 1e-7f), 1.0f);
+        bool live = p > L;
+        while (live) {
+            float u[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) u[j] = s.uniform();
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (live) {
+                    ++k;
+                    p *= fminf(fmaxf(u[j], 1e-7f), 1.0f);
+                    live = p > L && k < 1024;
+                }
+            }
         }
         return k - 1;
     }
@@ -246,9 +287,26 @@ __device__ float interp_layer(float density, const float* x, const float* y, int
             else
                 be = mid;
         }
-        i1 = max(1, bb * 16);
-        const int search_end = min(end, (bb + 1) * 16 - 1);
-        while (i1 <= search_end && ax(i1) < q) ++i1;
+        // First i in [start, search_end] with !(ax(i) < q), else
+        // search_end + 1: the 16 entries of block bb are read at once and
+        // compared in parallel (x is 16-byte aligned in LDS).
+        const int base = bb * 16;
+        const int start = max(1, base);
+        const int search_end = min(end, base + 15);
+        const float4* blk = reinterpret_cast<const float4*>(x + base);
+        unsigned hit = 0;
+#pragma unroll
+        for (int qd = 0; qd < 4; ++qd) {
+            const float4 v4 = blk[qd];
+            const float e4[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int idx = base + qd * 4 + t;
+                const float av = positive ? -e4[t] : e4[t];
+                if (idx >= start && idx <= search_end && !(av < q)) hit |= 1u << (qd * 4 + t);
+            }
+        }
+        i1 = hit ? base + __builtin_ctz(hit) : max(start, search_end + 1);
     } else {
         while (i1 <= end && ax(i1) < q) ++i1;
     }
@@ -279,8 +337,9 @@ __device__ __forceinline__ float retime(const GrainDev& g, float density, float 
 
 template <int kMode>
 __global__ __launch_bounds__(256) void grain_sample_kernel(GrainSampleArgs a) {
-    __shared__ float s_axis[kGrainCurveMax];
+    __shared__ __align__(16) float s_axis[kGrainCurveMax];
     __shared__ float s_layer[3][kGrainCurveMax];
+    __shared__ float s_prefix[16];
     const GrainDev& g = a.g;
     const int c = a.channel;
     const int n = g.curve_n;
@@ -289,6 +348,7 @@ __global__ __launch_bounds__(256) void grain_sample_kernel(GrainSampleArgs a) {
             s_axis[i] = g.base_curve[c][i];
             for (int l = 0; l < 3; ++l) s_layer[l][i] = (kMode == kGrainTriplet || l == a.layer) ? g.layer_curve[l][c][i] : 0.0f;
         }
+        if (threadIdx.x < 16) s_prefix[threadIdx.x] = g.axis_prefix[c][threadIdx.x];
         __syncthreads();
     }
     const long long total = static_cast<long long>(a.width) * a.height;
@@ -326,8 +386,7 @@ __global__ __launch_bounds__(256) void grain_sample_kernel(GrainSampleArgs a) {
         const int l0 = kMode == kGrainTriplet ? 0 : a.layer;
         const int l1 = kMode == kGrainTriplet ? 3 : a.layer + 1;
         for (int l = l0; l < l1; ++l) {
-            float density = interp_layer(in_density, s_axis, s_layer[l], n, g.positive_film != 0, bounded,
-                                         g.axis_prefix[c]);
+            float density = interp_layer(in_density, s_axis, s_layer[l], n, g.positive_film != 0, bounded, s_prefix);
             density += g.dmin_l[l][c];
             const uint64_t salt = static_cast<uint64_t>(c) + static_cast<uint64_t>(l) * 10ull;
             const float s0 = particle(g, density, g.dmax_l[l][c], g.n_l[l][c], g.od_l[l][c], g.uniformity[c],
@@ -444,6 +503,15 @@ __global__ __launch_bounds__(256) void grain_elt_kernel(GrainEltArgs a) {
             case kEltReconstruct: {
                 const float s = isfinite(a.s) ? a.s : 0.0f;
                 a.a[i] = san((a.add_mean ? a.a[i] : 0.0f) + s * a.b[i]);
+                break;
+            }
+            case kEltMixReconstruct: {
+                const float sw = clampf(a.sw, 0.0f, 1.0f), iw = clampf(a.iw, 0.0f, 1.0f);
+                const float shared = (a.shared && sw > 0.0f) ? a.shared[i] : 0.0f;
+                const float ind = (a.use_ind && iw > 0.0f) ? a.b[i] : 0.0f;
+                const float mixed = san(a.amp * (sw * shared + iw * ind));
+                const float s = isfinite(a.s) ? a.s : 0.0f;
+                a.a[i] = san((a.add_mean ? a.a[i] : 0.0f) + s * mixed);
                 break;
             }
             default: a.a[i] = a.b[i]; break;

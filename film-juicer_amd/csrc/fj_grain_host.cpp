@@ -405,7 +405,7 @@ hipError_t elt(int op, float* a, const float* b, long long n, hipStream_t s, int
 }
 
 hipError_t blur_pair(const float* in, float* tmp, float* out, const GrainKernel& k, const Work& wk, int weighted,
-                     float weight, int init, const float* sub, float bias, hipStream_t s) {
+                     float weight, int init, const float* sub, float bias, hipStream_t s, float gain = 1.0f) {
     GrainBlurArgs h;
     h.in = in;
     h.out = tmp;
@@ -423,12 +423,15 @@ hipError_t blur_pair(const float* in, float* tmp, float* out, const GrainKernel&
     v.initialize = init;
     v.sub = sub;
     v.bias = bias;
+    v.gain = gain;
     return launch_grain_blur_v(v, s);
 }
 
+// One grain scale accumulated (weighted) into `out`; when the correlation
+// blur runs, its epilogue also multiplies by `gain` (1: none).
 hipError_t generate_scale(const GrainPlan& plan, const GrainDev& base, const float* dens, int c, uint64_t salt,
                           float factor, const GrainKernel& corr, bool apply_corr, float weight, bool init,
-                          const Work& wk, hipStream_t s) {
+                          const Work& wk, float* out, hipStream_t s, float gain = 1.0f) {
     if (!std::isfinite(factor) || !(factor > 0.0f)) return hipErrorInvalidValue;
     GrainSampleArgs a{};
     a.g = base;
@@ -500,32 +503,45 @@ hipError_t generate_scale(const GrainPlan& plan, const GrainDev& base, const flo
         if (e != hipSuccess) return e;
     }
     if (apply_corr && corr.radius > 0)
-        return blur_pair(wk.scale, wk.tmp, wk.delta, corr, wk, 1, weight, init ? 1 : 0, nullptr, 0.0f, s);
-    return elt(kEltAccWeighted, wk.delta, wk.scale, wk.n, s, init ? 1 : 0, weight);
+        return blur_pair(wk.scale, wk.tmp, out, corr, wk, 1, weight, init ? 1 : 0, nullptr, 0.0f, s, gain);
+    return elt(kEltAccWeighted, out, wk.scale, wk.n, s, init ? 1 : 0, weight);
 }
 
+// The fine / mid / coarse scales summed into `out`, times the size-mix gain.
+// The gain rides on the last scale's correlation blur when that blur runs
+// (san(v * gain) there is the separate scale pass, bit for bit).
 hipError_t generate_mix(const GrainPlan& plan, const GrainDev& base, const float* dens, int c, const Work& wk,
-                        hipStream_t s) {
-    bool init = true;
-    hipError_t e;
-    if (plan.w_fine > 0.0f) {
-        e = generate_scale(plan, base, dens, c, kSaltFine, 1.0f, plan.corr[0], plan.corr[0].radius > 0, plan.w_fine,
-                           init, wk, s);
+                        float* out, hipStream_t s) {
+    struct Scale {
+        bool on;
+        uint64_t salt;
+        float factor;
+        const GrainKernel& corr;
+        bool apply;
+        float weight;
+    };
+    const Scale scales[3] = {
+        {plan.w_fine > 0.0f, kSaltFine, 1.0f, plan.corr[0], plan.corr[0].radius > 0, plan.w_fine},
+        {plan.w_mid > 0.0f, kSaltMid, std::sqrt(plan.size_mix_scale), plan.corr[1], true, plan.w_mid},
+        {plan.w_coarse > 0.0f, kSaltCoarse, plan.size_mix_scale, plan.corr[2], true, plan.w_coarse},
+    };
+    int last = -1;
+    for (int i = 0; i < 3; ++i)
+        if (scales[i].on) last = i;
+    const float gain = std::isfinite(plan.size_mix_gain) ? plan.size_mix_gain : 0.0f;
+    bool init = true, gained = false;
+    for (int i = 0; i < 3; ++i) {
+        const Scale& sc = scales[i];
+        if (!sc.on) continue;
+        const bool fuse = i == last && sc.apply && sc.corr.radius > 0;
+        hipError_t e = generate_scale(plan, base, dens, c, sc.salt, sc.factor, sc.corr, sc.apply, sc.weight, init, wk,
+                                      out, s, fuse ? gain : 1.0f);
         if (e != hipSuccess) return e;
         init = false;
+        gained = gained || fuse;
     }
-    if (plan.w_mid > 0.0f) {
-        e = generate_scale(plan, base, dens, c, kSaltMid, std::sqrt(plan.size_mix_scale), plan.corr[1], true,
-                           plan.w_mid, init, wk, s);
-        if (e != hipSuccess) return e;
-        init = false;
-    }
-    if (plan.w_coarse > 0.0f) {
-        e = generate_scale(plan, base, dens, c, kSaltCoarse, plan.size_mix_scale, plan.corr[2], true, plan.w_coarse,
-                           init, wk, s);
-        if (e != hipSuccess) return e;
-    }
-    return elt(kEltScale, wk.delta, nullptr, wk.n, s, 0, 0.0f, 0.0f, plan.size_mix_gain);
+    if (gained) return hipSuccess;
+    return elt(kEltScale, out, nullptr, wk.n, s, 0, 0.0f, 0.0f, plan.size_mix_gain);
 }
 
 }  // namespace
@@ -557,7 +573,8 @@ hipError_t run_grain(const GrainPlan& plan, float* const dens[3], int width, int
             if (!coarse || plan.w_coarse > 0.0f) {
                 const GrainKernel& k = coarse ? plan.corr[2] : plan.corr[0];
                 e = generate_scale(plan, plan.base, dens[c], c, coarse ? kSaltCoarse : kSaltFine,
-                                   coarse ? plan.size_mix_scale : 1.0f, k, !raw && k.radius > 0, 1.0f, true, wk, stream);
+                                   coarse ? plan.size_mix_scale : 1.0f, k, !raw && k.radius > 0, 1.0f, true, wk,
+                                   wk.delta, stream);
             } else {
                 e = elt(kEltClear, wk.delta, nullptr, wk.n, stream);
             }
@@ -575,37 +592,34 @@ hipError_t run_grain(const GrainPlan& plan, float* const dens[3], int width, int
         GrainDev sg = plan.base;
         sg.seed_base ^= kSaltShared;
         sg.seed_base_next ^= kSaltShared;
-        e = generate_mix(plan, sg, dens[1], 1, wk, stream);
-        if (e != hipSuccess) return e;
-        e = elt(kEltCopy, wk.shared, wk.delta, wk.n, stream);
+        e = generate_mix(plan, sg, dens[1], 1, wk, wk.shared, stream);
         if (e != hipSuccess) return e;
     }
     for (int c = 0; c < 3; ++c) {
-        if (plan.ind_w > 0.0f || !shared)
-            e = generate_mix(plan, plan.base, dens[c], c, wk, stream);
-        else
-            e = elt(kEltClear, wk.delta, nullptr, wk.n, stream);
-        if (e != hipSuccess) return e;
-        if (shared) {
-            GrainEltArgs m;
-            m.op = kEltMixShared;
-            m.a = wk.delta;
-            m.shared = wk.shared;
-            m.n = wk.n;
-            m.use_ind = plan.ind_w > 0.0f;
-            m.sw = plan.shared_w;
-            m.iw = plan.ind_w;
-            m.amp = plan.amplitude;
-            e = launch_grain_elt(m, stream);
+        const bool independent = plan.ind_w > 0.0f || !shared;
+        if (independent) {
+            e = generate_mix(plan, plan.base, dens[c], c, wk, wk.delta, stream);
             if (e != hipSuccess) return e;
         }
+        // With the shared part: the mix and the reconstruction in one pass
+        // (the independent delta is read only when it is used).
         GrainEltArgs r;
-        r.op = kEltReconstruct;
         r.a = dens[c];
         r.b = wk.delta;
         r.n = wk.n;
-        r.s = shared ? 1.0f : plan.amplitude;
         r.add_mean = plan.debug_view == 0;
+        if (shared) {
+            r.op = kEltMixReconstruct;
+            r.shared = wk.shared;
+            r.use_ind = plan.ind_w > 0.0f;
+            r.sw = plan.shared_w;
+            r.iw = plan.ind_w;
+            r.amp = plan.amplitude;
+            r.s = 1.0f;
+        } else {
+            r.op = kEltReconstruct;
+            r.s = plan.amplitude;
+        }
         e = launch_grain_elt(r, stream);
         if (e != hipSuccess) return e;
     }

@@ -38,6 +38,7 @@ constexpr int kHalMaxGauss = 5;
 constexpr int kHalRowThreads = 256;  // chunks per row
 constexpr int kHalColWidth = 16;     // columns per column-kernel block
 constexpr int kHalColChunks = 64;    // chunks per column
+constexpr int kHalColRegRows = 36;   // chunk rows a column thread holds in registers (H <= 2304)
 constexpr int kHalMaxLine = 8192;    // longest row the row kernel holds in LDS
 
 enum HalKind : int { kHalIdentity = 0, kHalFir = 1, kHalRecursive = 2 };
@@ -71,6 +72,7 @@ struct HalArgs {
     int width, height;
     int row_len;                       // Lr = ceil(width / kHalRowThreads)
     int col_len;                       // Lc = ceil(height / kHalColChunks)
+    int column_fir;                    // some Gaussian of the stage is an FIR
 };
 
 // Row + column launches; any Gaussian kinds; `out` may alias `in`.

@@ -129,6 +129,59 @@ __device__ __forceinline__ void chunk_scan(float* buf, int line, const float e[3
     __syncthreads();
 }
 
+// chunk_scan for a pass that processes chunks in reverse thread order: the
+// thread of virtual position r' = (blockDim.x / kLines - 1) - r takes the
+// place of position r' in chunk_scan, so a chunk gets the same arithmetic as
+// there. Every thread of the block must call it.
+template <int kLines>
+__device__ __forceinline__ void chunk_scan_rev(float* buf, int line, const float e[3], const float* tab,
+                                               float prev[3]) {
+    constexpr int kPer = 64 / kLines;
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockDim.x >> 6) - 1 - (threadIdx.x >> 6);
+    const int rw = kPer - 1 - lane / kLines;
+    float E0 = e[0], E1 = e[1], E2 = e[2];
+#pragma unroll
+    for (int d = 1; d < kPer; d <<= 1) {
+        const float o0 = __shfl_down(E0, d * kLines, 64);
+        const float o1 = __shfl_down(E1, d * kLines, 64);
+        const float o2 = __shfl_down(E2, d * kLines, 64);
+        if (rw >= d) {
+            const float* P = tab + d * 9;
+            E0 += fmaf(P[0], o0, fmaf(P[1], o1, P[2] * o2));
+            E1 += fmaf(P[3], o0, fmaf(P[4], o1, P[5] * o2));
+            E2 += fmaf(P[6], o0, fmaf(P[7], o1, P[8] * o2));
+        }
+    }
+    float x0 = __shfl_down(E0, kLines, 64);
+    float x1 = __shfl_down(E1, kLines, 64);
+    float x2 = __shfl_down(E2, kLines, 64);
+    if (rw == 0) x0 = x1 = x2 = 0.0f;
+    if (rw == kPer - 1) {
+        float* me = buf + (wave * kLines + line) * 3;
+        me[0] = E0;
+        me[1] = E1;
+        me[2] = E2;
+    }
+    __syncthreads();
+    float T0 = 0.0, T1 = 0.0, T2 = 0.0f;
+    const float* Mw = tab + kPer * 9;
+    for (int w = 0; w < wave; ++w) {
+        const float* o = buf + (w * kLines + line) * 3;
+        const float n0 = fmaf(Mw[0], T0, fmaf(Mw[1], T1, fmaf(Mw[2], T2, o[0])));
+        const float n1 = fmaf(Mw[3], T0, fmaf(Mw[4], T1, fmaf(Mw[5], T2, o[1])));
+        const float n2 = fmaf(Mw[6], T0, fmaf(Mw[7], T1, fmaf(Mw[8], T2, o[2])));
+        T0 = n0;
+        T1 = n1;
+        T2 = n2;
+    }
+    const float* P = tab + rw * 9;
+    prev[0] = fmaf(P[0], T0, fmaf(P[1], T1, fmaf(P[2], T2, x0)));
+    prev[1] = fmaf(P[3], T0, fmaf(P[4], T1, fmaf(P[5], T2, x1)));
+    prev[2] = fmaf(P[6], T0, fmaf(P[7], T1, fmaf(P[8], T2, x2)));
+    __syncthreads();
+}
+
 // FIR weights zero-padded to 19 taps centred at index 9.
 __device__ __forceinline__ void padded_taps(const HalGaussDev* g, float w[kFirTaps]) {
     const int r = g->radius;
@@ -331,6 +384,150 @@ __global__ __launch_bounds__(kHalColWidth * kHalColChunks) void hal_cols_kernel(
     }
 }
 
+// Column pass + combination for columns of at most 64 * kHalColRegRows rows:
+// the layout of hal_cols_kernel, but thread j owns chunk j in both
+// directions and holds it in registers (v: the Gaussian's rows, acc: the
+// weighted sum), so Q_k is read once and never written back. Slot m of a
+// thread holds row hi - (R - 1) + m; the chunk's rows are the slots
+// m >= m0 = R - len. Slots below m0 load some row of the plane (clamped) and
+// their results are discarded, so no load or step needs a branch: the upward
+// pass (slots R-1 .. 0) takes its end state and its last output at slot m0,
+// the downward pass (0 .. R-1) resets its state at slot m0. The downward
+// scan is chunk_scan_rev, which gives every chunk the arithmetic of
+// hal_cols_kernel's downward pass, so the two kernels agree bit for bit.
+// Stages with FIR column filters run hal_cols_kernel (their windows would not
+// fit beside the two register arrays).
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));  // a fresh value per pass: per-slot addresses and masks are not shared
+    return v;
+}
+
+__global__ __launch_bounds__(kHalColWidth * kHalColChunks) void hal_cols_reg_kernel(HalArgs a) {
+    constexpr int R = kHalColRegRows;
+    const int ch = blockIdx.y;
+    const HalStageDev* st = a.stage + ch;
+    const int count = st->count;
+    if (count == 0) return;
+    __shared__ float scan[(kHalColChunks * kHalColWidth / 64) * kHalColWidth * 3];
+    __shared__ float first_row[kHalColWidth];
+    const int cx = threadIdx.x % kHalColWidth;
+    const int j = threadIdx.x / kHalColWidth;
+    const int W = a.width, H = a.height, L = a.col_len;
+    const int x = blockIdx.x * kHalColWidth + cx;
+    const bool live = x < W;
+    const int xc = live ? x : W - 1;
+    const int hi = H - 1 - j * L;
+    const int lo = max(0, H - (j + 1) * L);
+    const int len = max(0, hi - lo + 1);
+    const int m0 = R - len;
+    const int top = (H + L - 1) / L - 1;
+    auto elem = [&](auto* plane, int r0, int m) {
+        const int r = min(max(r0 + m, 0), H - 1);
+        return plane + static_cast<size_t>(r) * W + xc;
+    };
+    const float* Xin = a.in[ch];
+    float* Xout = a.out[ch];
+    const float c0 = st->c0;
+    float acc[R];
+    {
+        const int r0 = opaque(hi) - (R - 1);
+#pragma unroll
+        for (int m = 0; m < R; ++m) acc[m] = c0 * *elem(Xin, r0, m);
+    }
+    for (int k = 0; k < count; ++k) {
+        const HalGaussDev* g = st->g + k;
+        const float om = st->omega[k];
+        const int kind = g->kind;
+        if (kind == kHalIdentity) {
+            const int r0 = opaque(hi) - (R - 1);
+#pragma unroll
+            for (int m = 0; m < R; ++m) acc[m] = fmaf(om, *elem(Xin, r0, m), acc[m]);
+        } else {
+            const Rec rec = load_rec(g);
+            const float* tab = &g->col_pow[0][0];
+            const float* Q = a.q[ch][k];
+            float v[R];
+            {
+                const int r0 = opaque(hi) - (R - 1);
+#pragma unroll
+                for (int m = 0; m < R; ++m) v[m] = *elem(Q, r0, m);
+            }
+            float s[3], e[3], prev[3];
+            // Bottom to top, seeded with the bottom row (slot R - 1 of chunk 0).
+            const float bottom = v[R - 1];
+            if (j == 0) hal_seed(rec, bottom, s);
+            else hal_zero(s);
+            e[0] = s[0], e[1] = s[1], e[2] = s[2];
+            {
+                const int mb = opaque(m0);
+#pragma unroll
+                for (int m = R - 1; m >= 0; --m) {
+                    hal_step(rec, v[m], s);
+                    const bool at = m == mb;
+                    e[0] = at ? s[0] : e[0];
+                    e[1] = at ? s[1] : e[1];
+                    e[2] = at ? s[2] : e[2];
+                }
+            }
+            chunk_scan<kHalColWidth>(scan, cx, e, tab, prev);
+            if (j == 0) hal_seed(rec, bottom, s);
+            else s[0] = prev[0], s[1] = prev[1], s[2] = prev[2];
+            float last = 0.0f;
+            {
+                const int mb = opaque(m0);
+#pragma unroll
+                for (int m = R - 1; m >= 0; --m) {
+                    v[m] = hal_step(rec, v[m], s);
+                    last = m == mb ? v[m] : last;
+                }
+            }
+            // Row 0 of the upward result: the last output of the top chunk.
+            if (j == top) first_row[cx] = last;
+            __syncthreads();
+            const float first = first_row[cx];
+            // Top to bottom, seeded with row 0 of the upward result.
+            float init[3];
+            if (j == top) hal_seed(rec, first, init);
+            else hal_zero(init);
+            s[0] = init[0], s[1] = init[1], s[2] = init[2];
+            {
+                const int mb = opaque(m0);
+#pragma unroll
+                for (int m = 0; m < R; ++m) {
+                    const bool at = m == mb;
+                    s[0] = at ? init[0] : s[0];
+                    s[1] = at ? init[1] : s[1];
+                    s[2] = at ? init[2] : s[2];
+                    hal_step(rec, v[m], s);
+                }
+            }
+            if (len == 0) hal_zero(s);  // an empty chunk passes its incoming state on unchanged
+            chunk_scan_rev<kHalColWidth>(scan, cx, s, tab, prev);
+            if (j == top) hal_seed(rec, first, init);
+            else init[0] = prev[0], init[1] = prev[1], init[2] = prev[2];
+            s[0] = init[0], s[1] = init[1], s[2] = init[2];
+            {
+                const int mb = opaque(m0);
+#pragma unroll
+                for (int m = 0; m < R; ++m) {
+                    const bool at = m == mb;
+                    s[0] = at ? init[0] : s[0];
+                    s[1] = at ? init[1] : s[1];
+                    s[2] = at ? init[2] : s[2];
+                    acc[m] = fmaf(om, hal_step(rec, v[m], s), acc[m]);
+                }
+            }
+        }
+    }
+    if (live) {
+        const int r0 = opaque(hi) - (R - 1);
+        const int mb = opaque(m0);
+#pragma unroll
+        for (int m = 0; m < R; ++m)
+            if (m >= mb) *elem(Xout, r0, m) = acc[m];
+    }
+}
+
 // FIR-only stage: grid (ceil(W/64), ceil(H/32), 3), 256 threads. The tile and
 // a 9-pixel reflected halo sit in LDS; for every Gaussian the horizontal taps
 // fill a 50 x 64 intermediate and the vertical taps accumulate the 64 x 32
@@ -430,8 +627,11 @@ hipError_t launch_halation_stage(const HalArgs& a, hipStream_t stream) {
                        stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(hal_cols_kernel, dim3((a.width + kHalColWidth - 1) / kHalColWidth, 3),
-                       dim3(kHalColWidth * kHalColChunks), 0, stream, a);
+    const dim3 grid((a.width + kHalColWidth - 1) / kHalColWidth, 3);
+    if (a.col_len <= kHalColRegRows && !a.column_fir)
+        hipLaunchKernelGGL(hal_cols_reg_kernel, grid, dim3(kHalColWidth * kHalColChunks), 0, stream, a);
+    else
+        hipLaunchKernelGGL(hal_cols_kernel, grid, dim3(kHalColWidth * kHalColChunks), 0, stream, a);
     return hipGetLastError();
 }
 

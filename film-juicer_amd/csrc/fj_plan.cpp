@@ -1007,6 +1007,10 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
                     ha.in[c] = cur[c];
                     ha.out[c] = cur[c];
                 }
+                ha.column_fir = 0;
+                for (int c = 0; c < 3; ++c)
+                    for (int k = 0; k < host[c].count; ++k)
+                        if (host[c].g[k].kind == kHalFir) ha.column_fir = 1;
                 return launch_halation_stage(ha, stream);
             };
             ha.width = pp.io.width;

@@ -61,6 +61,8 @@ CASES = {
     "sublayer-count-3": dict(sublayers=False, sublayer_count=3, chroma_shared_weight=0.0,
                              chroma_independent_weight=1.0),
     "coarse-pitch": dict(px=40.0),
+    # few particles per pixel: most samples take the product loop (lambda < 10)
+    "sparse-particles": dict(particle_area_um2=40.0),
 }
 
 

@@ -56,7 +56,12 @@ def kinds(p):
     return (c.core.kind, tuple(c.tail[i].kind for i in range(3)), tuple(c.bounce[i].kind for i in range(d.bounce_count)))
 
 
-@pytest.mark.parametrize("shape", [(257, 131), (300, 70), (64, 48), (17, 65), (1, 7), (5, 1)])
+# Column kernels: heights up to 64 x 36 rows keep each chunk in registers
+# (2160: 34-row chunks and a short top chunk; 2304: the largest such height;
+# 131 and 7: short top chunks and empty chunks); 2305 takes the streaming
+# column kernel. Widths that are not a multiple of 16 leave idle columns.
+@pytest.mark.parametrize("shape", [(257, 131), (300, 70), (64, 48), (17, 65), (1, 7), (5, 1), (40, 2160),
+                                   (96, 2304), (33, 2305)])
 def test_halation_shapes(shape):
     w, h = shape
     src = synth.frame(w, h, seed=w * 7 + h)
