@@ -729,15 +729,24 @@ __global__ __launch_bounds__(256) void transpose_kernel(TransposeArgs t) {
     float* out = t.out[plane];
     const int bx = blockIdx.x * 64, by = blockIdx.y * 64;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
-    for (int j = ty; j < 64; j += 4) {
+    // All 16 loads of a thread are issued before the first is used.
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int j = ty + 4 * u;
         const int x = bx + tx, y = by + j;
+        v[u] = 0.0f;  // outside the plane: never read back
         if (x < t.width && y < t.height) {
             const int sy = t.src_rows > 0 ? reflect101(y - t.row_pad, t.src_rows) : y;
-            tile[j][tx] = in[static_cast<long long>(sy) * t.in_stride + x];
+            v[u] = in[static_cast<long long>(sy) * t.in_stride + x];
         }
     }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) tile[ty + 4 * u][tx] = v[u];
     __syncthreads();
-    for (int j = ty; j < 64; j += 4) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int j = ty + 4 * u;
         const int x = by + tx, y = bx + j;  // out is width x height transposed
         if (x < t.height && y < t.width) out[static_cast<long long>(y) * t.out_stride + x] = tile[tx][j];
     }
