@@ -21,6 +21,7 @@
 // applied as IDCT(T ⊙ DCT(x)) with T the kernel's frequency response.
 #include <hip/hip_runtime.h>
 
+#include "fj_batch.h"
 #include "fj_fft.h"
 
 namespace fjx {
@@ -429,27 +430,6 @@ __device__ __forceinline__ int reflect101(int i, int n) {
 
 // Makhoul permutation index: position of x[i] in v.
 __device__ __forceinline__ int makhoul_pos(int i, int n) { return (i & 1) ? n - 1 - (i >> 1) : (i >> 1); }
-
-// The strided loop i = first, first + step, ... < end in groups of kU: a
-// group's global loads (load(i) -> T) are all issued before any of them is
-// used (use(i, T)), so a thread waits for memory once per group rather than
-// once per element.
-template <int kU, typename T, typename Load, typename Use>
-__device__ __forceinline__ void batched(int first, int end, int step, Load load, Use use) {
-    for (int i0 = first; i0 < end; i0 += kU * step) {
-        T v[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int i = i0 + u * step;
-            if (i < end) v[u] = load(i);
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int i = i0 + u * step;
-            if (i < end) use(i, v[u]);
-        }
-    }
-}
 
 // DCT-II of two real lines packed as (re, im) of one complex line held in
 // LDS (Makhoul order). On return out_a / out_b (global) hold X_a[k], X_b[k].

@@ -11,6 +11,7 @@
 // JuicerCudaFilmPipeline.cu:2456-2577.
 #include <hip/hip_runtime.h>
 
+#include "fj_batch.h"
 #include "fj_detmath.h"
 #include "fj_grain.h"
 
@@ -443,12 +444,15 @@ __global__ __launch_bounds__(256) void grain_blur_v_kernel(GrainBlurArgs a) {
     const int x = blockIdx.x * 64 + tx;
     const int y0 = blockIdx.y * 16;
     const int rows = 16 + 2 * r;
-    for (int i = ty; i < rows; i += 4) {
-        long long yi = static_cast<long long>(y0) + i - r;
-        while (yi < 0 || yi >= a.height) yi = yi < 0 ? -yi - 1 : 2LL * a.height - yi - 1;
-        if (a.height <= 1) yi = 0;
-        tile[i * 64 + tx] = x < a.width ? a.in[yi * a.width + x] : 0.0f;
-    }
+    batched<8, float>(
+        ty, rows, 4,
+        [&](int i) {
+            long long yi = static_cast<long long>(y0) + i - r;
+            while (yi < 0 || yi >= a.height) yi = yi < 0 ? -yi - 1 : 2LL * a.height - yi - 1;
+            if (a.height <= 1) yi = 0;
+            return x < a.width ? a.in[yi * a.width + x] : 0.0f;
+        },
+        [&](int i, float v) { tile[i * 64 + tx] = v; });
     __syncthreads();
     if (x >= a.width) return;
     const float wf = isfinite(a.weight) ? a.weight : 0.0f;

@@ -11,6 +11,7 @@
 // only of FIR taps run as one tiled launch with both passes in LDS.
 #include <hip/hip_runtime.h>
 
+#include "fj_batch.h"
 #include "fj_halation.h"
 
 namespace fjx {
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(kHalRowThreads) void hal_rows_kernel(HalArgs a) {
     float* ys = hal_lds + W;
     const int t = threadIdx.x;
     const float* X = a.in[ch] + static_cast<size_t>(y) * W;
-    for (int i = t; i < W; i += kHalRowThreads) xs[i] = X[i];
+    batched<8, float>(t, W, kHalRowThreads, [&](int i) { return X[i]; }, [&](int i, float v) { xs[i] = v; });
     __syncthreads();
     const int L = a.row_len;
     const int lo = t * L;
@@ -550,10 +551,16 @@ __global__ __launch_bounds__(256) void hal_fir_tile_kernel(HalArgs a) {
     __shared__ float tin[kInH][kInW + 1];
     __shared__ float th[kInH][kTileW + 1];
     const int t = threadIdx.x;
-    for (int i = t; i < kInH * kInW; i += 256) {
-        const int r = i / kInW, c = i - r * kInW;
-        tin[r][c] = X[static_cast<size_t>(hal_reflect(y0 - kHalo + r, H)) * W + hal_reflect(x0 - kHalo + c, W)];
-    }
+    batched<8, float>(
+        t, kInH * kInW, 256,
+        [&](int i) {
+            const int r = i / kInW, c = i - r * kInW;
+            return X[static_cast<size_t>(hal_reflect(y0 - kHalo + r, H)) * W + hal_reflect(x0 - kHalo + c, W)];
+        },
+        [&](int i, float v) {
+            const int r = i / kInW, c = i - r * kInW;
+            tin[r][c] = v;
+        });
     __syncthreads();
     // Vertical-pass ownership: column vc, rows vr0 .. vr0+7.
     const int vc = t % kTileW;

@@ -7,6 +7,7 @@
 // borders, then the output CCTF, a [0,1] clamp and the source alpha.
 #include <hip/hip_runtime.h>
 
+#include "fj_batch.h"
 #include "fj_device.h"
 #include "fj_pixel_math.h"
 
@@ -43,11 +44,14 @@ __global__ __launch_bounds__(256) void post_kernel(PostParams p) {
         const float* src = p.planes + c * plane;
         if (r > 0) {
             __syncthreads();
-            for (int i = threadIdx.x; i < in_w * in_h; i += blockDim.x) {
-                const int ly = i / in_w, lx = i - ly * in_w;
-                const int gy = reflect_half(y0 + ly - r, H), gx = reflect_half(x0 + lx - r, W);
-                s_in[i] = src[static_cast<long long>(gy) * W + gx];
-            }
+            batched<8, float>(
+                threadIdx.x, in_w * in_h, blockDim.x,
+                [&](int i) {
+                    const int ly = i / in_w, lx = i - ly * in_w;
+                    const int gy = reflect_half(y0 + ly - r, H), gx = reflect_half(x0 + lx - r, W);
+                    return src[static_cast<long long>(gy) * W + gx];
+                },
+                [&](int i, float v) { s_in[i] = v; });
             __syncthreads();
             for (int i = threadIdx.x; i < kTileW * in_h; i += blockDim.x) {
                 const int ly = i / kTileW, lx = i - ly * kTileW;
