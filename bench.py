@@ -35,8 +35,8 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the diffusion column filter (dct_filter_rows_kernel<true>)
 # at this workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
 # scaled by the factors a plain copy kernel of known size reads (x2.0 fetch, x1.0 write).
-FILTER_FETCH_KIB = 254183.3
-FILTER_WRITE_KIB = 142383.8
+FILTER_FETCH_KIB = 253524.7
+FILTER_WRITE_KIB = 142384.3
 
 
 def smooth_length(n: int) -> int:
