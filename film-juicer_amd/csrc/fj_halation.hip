@@ -540,6 +540,48 @@ constexpr int kInW = kTileW + 2 * kHalo;  // 82
 constexpr int kInH = kTileH + 2 * kHalo;  // 50
 constexpr int kStrip = 8;                 // outputs per thread and pass
 
+// One FIR Gaussian of radius kR (2 kR + 1 taps, w = g->w) on a tile: the
+// horizontal taps over the rows the vertical taps need (kTileH + 2 kR) into
+// th, then the vertical taps into acc. Only the real taps are evaluated; the
+// sums run in the order of the zero-padded 19-tap form, whose extra terms
+// add nothing.
+template <int kR>
+__device__ __forceinline__ void fir_gaussian(const HalGaussDev* g, float om, const float (*tin)[kInW + 1],
+                                             float (*th)[kTileW + 1], int vc, int vr0, float acc[kStrip]) {
+    constexpr int kTaps = 2 * kR + 1;
+    constexpr int kRows = kTileH + 2 * kR;
+    constexpr int kRow0 = kHalo - kR;  // first tin / th row the vertical taps read
+    float w[kTaps];
+#pragma unroll
+    for (int i = 0; i < kTaps; ++i) w[i] = g->w[i];
+    for (int item = threadIdx.x; item < kRows * (kTileW / kStrip); item += 256) {
+        const int r = kRow0 + item / (kTileW / kStrip);
+        const int c = (item % (kTileW / kStrip)) * kStrip;
+        float win[kStrip + kTaps - 1];
+#pragma unroll
+        for (int i = 0; i < kStrip + kTaps - 1; ++i) win[i] = tin[r][c + kHalo - kR + i];
+#pragma unroll
+        for (int i = 0; i < kStrip; ++i) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int o = 0; o < kTaps; ++o) sum = fmaf(win[i + o], w[o], sum);
+            th[r][c + i] = sum;
+        }
+    }
+    __syncthreads();
+    float win[kStrip + kTaps - 1];
+#pragma unroll
+    for (int i = 0; i < kStrip + kTaps - 1; ++i) win[i] = th[kRow0 + vr0 + i][vc];
+#pragma unroll
+    for (int i = 0; i < kStrip; ++i) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int o = 0; o < kTaps; ++o) sum = fmaf(win[i + o], w[o], sum);
+        acc[i] = fmaf(om, sum, acc[i]);
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void hal_fir_tile_kernel(HalArgs a) {
     const int ch = blockIdx.z;
     const HalStageDev* st = a.stage + ch;
@@ -577,37 +619,18 @@ __global__ __launch_bounds__(256) void hal_fir_tile_kernel(HalArgs a) {
             for (int i = 0; i < kStrip; ++i) acc[i] = fmaf(om, tin[kHalo + vr0 + i][kHalo + vc], acc[i]);
             continue;
         }
-        float w[kFirTaps];
-        padded_taps(g, w);
-        // Horizontal: 50 rows x 8 strips of 8 columns.
-        for (int item = t; item < kInH * (kTileW / kStrip); item += 256) {
-            const int r = item / (kTileW / kStrip);
-            const int c = (item - r * (kTileW / kStrip)) * kStrip;
-            float win[kStrip + kFirTaps - 1];
-#pragma unroll
-            for (int i = 0; i < kStrip + kFirTaps - 1; ++i) win[i] = tin[r][c + i];
-#pragma unroll
-            for (int i = 0; i < kStrip; ++i) {
-                float sum = 0.0f;
-#pragma unroll
-                for (int o = 0; o < kFirTaps; ++o) sum = fmaf(win[i + o], w[o], sum);
-                th[r][c + i] = sum;
-            }
+        switch (g->radius) {
+            case 0: fir_gaussian<0>(g, om, tin, th, vc, vr0, acc); break;
+            case 1: fir_gaussian<1>(g, om, tin, th, vc, vr0, acc); break;
+            case 2: fir_gaussian<2>(g, om, tin, th, vc, vr0, acc); break;
+            case 3: fir_gaussian<3>(g, om, tin, th, vc, vr0, acc); break;
+            case 4: fir_gaussian<4>(g, om, tin, th, vc, vr0, acc); break;
+            case 5: fir_gaussian<5>(g, om, tin, th, vc, vr0, acc); break;
+            case 6: fir_gaussian<6>(g, om, tin, th, vc, vr0, acc); break;
+            case 7: fir_gaussian<7>(g, om, tin, th, vc, vr0, acc); break;
+            case 8: fir_gaussian<8>(g, om, tin, th, vc, vr0, acc); break;
+            default: fir_gaussian<9>(g, om, tin, th, vc, vr0, acc); break;
         }
-        __syncthreads();
-        {
-            float win[kStrip + kFirTaps - 1];
-#pragma unroll
-            for (int i = 0; i < kStrip + kFirTaps - 1; ++i) win[i] = th[vr0 + i][vc];
-#pragma unroll
-            for (int i = 0; i < kStrip; ++i) {
-                float sum = 0.0f;
-#pragma unroll
-                for (int o = 0; o < kFirTaps; ++o) sum = fmaf(win[i + o], w[o], sum);
-                acc[i] = fmaf(om, sum, acc[i]);
-            }
-        }
-        __syncthreads();
     }
     const int x = x0 + vc;
     if (x < W) {

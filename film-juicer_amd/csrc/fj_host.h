@@ -32,7 +32,9 @@ struct ScannerLut {
 bool build_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& diagnostic);
 
 // kernels (fj_pixel.hip)
-hipError_t launch_pixel_stage(const PixelParams& p, int mode, int grid_limit, hipStream_t stream);
+// Persistent launch: each kernel gets the blocks that are resident at once on
+// compute_units CUs (at most one per 256-pixel tile).
+hipError_t launch_pixel_stage(const PixelParams& p, int mode, int compute_units, hipStream_t stream);
 hipError_t launch_post(const PostParams& p, hipStream_t stream);
 hipError_t launch_clear_word(unsigned int* word, hipStream_t stream);
 size_t fused_lds_bytes(const PixelParams& p);

@@ -869,7 +869,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         if (name && stage < DeviceState::kMaxStages) dev.stage_names[stage] = name;
         ++stage;
     };
-    const int grid_limit = dev.compute_units * 8;
+    const int cus = dev.compute_units;
     const bool spatial_dir = prepared.spatial_dir.hash != 0;
     const bool post = post_active(prepared.scanner_post);
     const bool halation = (prepared.optics.flags & FJ_OPTICS_SCATTER_HALATION) != 0;
@@ -964,7 +964,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     if (carrier) {
         for (int c = 0; c < 3; ++c) pp.carrier[c] = ws + carrier_off + c * plane;
         mark("camera_carrier");
-        e = launch_pixel_stage(pp, kCarrier, grid_limit, stream);
+        e = launch_pixel_stage(pp, kCarrier, cus, stream);
         if (e != hipSuccess) return outcome_hip(e, "camera carrier stage", error);
         // Descriptor channels are R, G, B; carrier planes are B/G/R-sensitive.
         float* cur[3];
@@ -1042,7 +1042,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     if (spatial_dir) {
         for (int c = 0; c < 6; ++c) pp.dir_planes[c] = ws + dir_off + c * plane;
         mark("dir_source");
-        e = launch_pixel_stage(pp, kDirSource, grid_limit, stream);
+        e = launch_pixel_stage(pp, kDirSource, cus, stream);
         if (e != hipSuccess) return outcome_hip(e, "DIR source stage", error);
         DctFilterJob job;
         job.width = pp.io.width;
@@ -1074,11 +1074,11 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         e = run_dct_filter(dev, job, stream);
         if (e != hipSuccess) return outcome_hip(e, "DIR spatial filter", error);
         mark("pixel_chain");
-        e = launch_pixel_stage(pp, kDirFinal, grid_limit, stream);
+        e = launch_pixel_stage(pp, kDirFinal, cus, stream);
         if (e != hipSuccess) return outcome_hip(e, "DIR final stage", error);
     } else {
         mark("pixel_chain");
-        e = launch_pixel_stage(pp, kFused, grid_limit, stream);
+        e = launch_pixel_stage(pp, kFused, cus, stream);
         if (e != hipSuccess) return outcome_hip(e, "fused pixel kernel", error);
     }
     if (grain) {
@@ -1088,7 +1088,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         e = run_grain(gp, pp.dens, pp.io.width, pp.io.height, ws + grain_off, stream);
         if (e != hipSuccess) return outcome_hip(e, "print visual grain launch", error);
         mark(enlarger_diffusion ? "print_linear" : "print_scan");
-        e = launch_pixel_stage(pp, kDensityFinal, grid_limit, stream);
+        e = launch_pixel_stage(pp, kDensityFinal, cus, stream);
         if (e != hipSuccess) return outcome_hip(e, "density final stage", error);
     }
     if (enlarger_diffusion) {
@@ -1100,7 +1100,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         });
         if (e != hipSuccess) return outcome_hip(e, "enlarger diffusion stage", error);
         mark("print_scan");
-        e = launch_pixel_stage(pp, kEnlargerFinal, grid_limit, stream);
+        e = launch_pixel_stage(pp, kEnlargerFinal, cus, stream);
         if (e != hipSuccess) return outcome_hip(e, "enlarger print stage", error);
     }
     if (post) {
