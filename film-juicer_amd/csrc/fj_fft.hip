@@ -605,9 +605,10 @@ __device__ __forceinline__ void dct_filter_rows_body(const DctRowsArgs& args, co
     const float* full = tf.full[plane];
     float fa[kMaxDctComponents], fb[kMaxDctComponents];
     if (!kFullTable) {
-        for (int c = 0; c < tf.count; ++c) {
-            fa[c] = tf.weight[c] * tf.t_rows[c][row0];
-            fb[c] = has1 ? tf.weight[c] * tf.t_rows[c][row0 + 1] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < kMaxDctComponents; ++c) {
+            fa[c] = c < tf.count ? tf.weight[c] * tf.t_rows[c][row0] : 0.0f;
+            fb[c] = c < tf.count && has1 ? tf.weight[c] * tf.t_rows[c][row0 + 1] : 0.0f;
         }
     }
     const int row1 = has1 ? row0 + 1 : row0;
@@ -619,17 +620,39 @@ __device__ __forceinline__ void dct_filter_rows_body(const DctRowsArgs& args, co
             float2 w1, w2;
             float ta1, ta2, tb1, tb2;
         };
-        batched<4, Coef>(
+        batched<kFullTable ? 4 : 2, Coef>(
             threadIdx.x, n / 2 + 1, blockDim.x,
             [&](int p) {
                 const int k1 = p, k2 = p == 0 ? 0 : n - p;
                 Coef c;
                 c.w1 = args.half_tw[k1];
                 c.w2 = args.half_tw[k2];
-                c.ta1 = transfer_at<kFullTable>(tf, full, fa, row0, n, k1);
-                c.ta2 = transfer_at<kFullTable>(tf, full, fa, row0, n, k2);
-                c.tb1 = has1 ? transfer_at<kFullTable>(tf, full, fb, row1, n, k1) : 0.0f;
-                c.tb2 = has1 ? transfer_at<kFullTable>(tf, full, fb, row1, n, k2) : 0.0f;
+                if constexpr (kFullTable) {
+                    c.ta1 = transfer_at<true>(tf, full, fa, row0, n, k1);
+                    c.ta2 = transfer_at<true>(tf, full, fa, row0, n, k2);
+                    c.tb1 = has1 ? transfer_at<true>(tf, full, fb, row1, n, k1) : 0.0f;
+                    c.tb2 = has1 ? transfer_at<true>(tf, full, fb, row1, n, k2) : 0.0f;
+                } else {
+                    // transfer_at's sums, with each column factor loaded once
+                    // (both rows use it) and all of them before the first sum.
+                    float v1[kMaxDctComponents], v2[kMaxDctComponents];
+#pragma unroll
+                    for (int q = 0; q < kMaxDctComponents; ++q) {
+                        v1[q] = q < tf.count ? tf.t_cols[q][k1] : 0.0f;
+                        v2[q] = q < tf.count ? tf.t_cols[q][k2] : 0.0f;
+                    }
+                    c.ta1 = c.ta2 = c.tb1 = c.tb2 = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < kMaxDctComponents; ++q) {
+                        if (q < tf.count) {
+                            c.ta1 = fmaf(fa[q], v1[q], c.ta1);
+                            c.ta2 = fmaf(fa[q], v2[q], c.ta2);
+                            c.tb1 = fmaf(fb[q], v1[q], c.tb1);
+                            c.tb2 = fmaf(fb[q], v2[q], c.tb2);
+                        }
+                    }
+                    if (!has1) c.tb1 = c.tb2 = 0.0f;
+                }
                 return c;
             },
             [&](int p, const Coef& c) {
