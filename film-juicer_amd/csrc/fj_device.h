@@ -24,6 +24,10 @@ struct ExposureParams {
     float scale;            // exp2(EV) * route correction (validated > 0)
     float camera_scale;     // exp2(EV) alone: the camera carrier's scale
     float route_scale;      // route correction applied when a carrier is developed
+    // Auto exposure: the metered scale (device scalar) replaces `scale` by
+    // (ae * camera_scale) [* route_scale when with_route], validated > 0.
+    const float* ae;
+    int with_route;
     // Mallett 2019 reconstruction: k pairs {basis_r*illum, basis_g*illum,
     // basis_b*illum, 0}, {sens_B, sens_G, sens_R, 0}; rgb_to_srgb maps the
     // input to linear sRGB; green mid-gray scale applied to the result.

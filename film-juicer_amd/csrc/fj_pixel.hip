@@ -300,6 +300,7 @@ hipError_t launch_pixel_stage(const PixelParams& p, int mode, int compute_units,
             PixelParams q = p;
             q.print = 0;
             q.ex.scale = p.ex.camera_scale;
+            q.ex.with_route = 0;
             const size_t lq = fused_lds_bytes(q);
             if (use_lds)
                 launch_resident(pixel_stage_kernel<false, true, kCarrier>, q, cus, tiles, lq, stream);

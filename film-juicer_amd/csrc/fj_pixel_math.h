@@ -79,16 +79,17 @@ __device__ __forceinline__ void mallett_exposure(const ExposureParams& ex, const
         eg = fmaf(spd, sens.y, eg);
         er = fmaf(spd, sens.z, er);
     }
-    E[0] = fmaxf(0.0f, fmaxf(0.0f, (isfinite(eb) ? fmaxf(0.0f, eb) : 0.0f) * ex.mallett_scale) * ex.scale);
-    E[1] = fmaxf(0.0f, fmaxf(0.0f, (isfinite(eg) ? fmaxf(0.0f, eg) : 0.0f) * ex.mallett_scale) * ex.scale);
-    E[2] = fmaxf(0.0f, fmaxf(0.0f, (isfinite(er) ? fmaxf(0.0f, er) : 0.0f) * ex.mallett_scale) * ex.scale);
+    E[0] = fmaxf(0.0f, (isfinite(eb) ? fmaxf(0.0f, eb) : 0.0f) * ex.mallett_scale);
+    E[1] = fmaxf(0.0f, (isfinite(eg) ? fmaxf(0.0f, eg) : 0.0f) * ex.mallett_scale);
+    E[2] = fmaxf(0.0f, (isfinite(er) ? fmaxf(0.0f, er) : 0.0f) * ex.mallett_scale);
 }
 
-// Camera film linear exposure in B/G/R (JuicerCudaDeviceHelpers.cuh:509-617,
-// :709-763): XYZ -> chromaticity -> tri2quad -> bicubic TC LUT -> x sum ->
-// x exposure scale; Mallett inputs take the spectral path above.
-__device__ __forceinline__ void camera_exposure(const ExposureParams& ex, const float4* mallett_table,
-                                                const float in[3], float E[3]) {
+// Reconstructed film raw exposure in B/G/R before any exposure scale
+// (reconstruct_film_raw_device, JuicerCudaDeviceHelpers.cuh:509-617,
+// :670-708): XYZ -> chromaticity -> tri2quad -> bicubic TC LUT -> x sum;
+// Mallett inputs take the spectral path above.
+__device__ __forceinline__ void camera_raw(const ExposureParams& ex, const float4* mallett_table, const float in[3],
+                                           float E[3]) {
     float rgb[3];
     if (ex.decode_cctf) {
         rgb[0] = decode_channel(ex.decode_cctf, in[0]);
@@ -140,9 +141,32 @@ __device__ __forceinline__ void camera_exposure(const ExposureParams& ex, const 
     }
     const float winv = wsum != 0.0f ? 1.0f / wsum : 0.0f;
     const float lr = fin0(acc0 * winv), lg = fin0(acc1 * winv), lb = fin0(acc2 * winv);
-    E[0] = fmaxf(0.0f, fin0(sum * lb) * ex.scale);
-    E[1] = fmaxf(0.0f, fin0(sum * lg) * ex.scale);
-    E[2] = fmaxf(0.0f, fin0(sum * lr) * ex.scale);
+    E[0] = fin0(sum * lb);
+    E[1] = fin0(sum * lg);
+    E[2] = fin0(sum * lr);
+}
+
+// The exposure scale of a launch (compute_film_linear_exposure_device,
+// JuicerCudaDeviceHelpers.cuh:709-744): with auto exposure, the metered scale
+// times the manual scale (times the route correction when the launch applies
+// it), else the launch's precomputed scale.
+__device__ __forceinline__ float exposure_scale(const ExposureParams& ex) {
+    if (!ex.ae) return ex.scale;
+    float a = *ex.ae;
+    if (!(isfinite(a) && a > 0.0f)) a = 1.0f;
+    float s = a * ex.camera_scale;
+    if (ex.with_route) s *= ex.route_scale;
+    return isfinite(s) && s > 0.0f ? s : 1.0f;
+}
+
+// Camera film linear exposure in B/G/R: the raw exposure times the scale.
+__device__ __forceinline__ void camera_exposure(const ExposureParams& ex, const float4* mallett_table,
+                                                const float in[3], float E[3]) {
+    camera_raw(ex, mallett_table, in, E);
+    const float s = exposure_scale(ex);
+    E[0] = fmaxf(0.0f, E[0] * s);
+    E[1] = fmaxf(0.0f, E[1] * s);
+    E[2] = fmaxf(0.0f, E[2] * s);
 }
 
 // Clamped linear lookup on an ascending axis (JuicerCudaDeviceHelpers.cuh:34-89).

@@ -1,6 +1,7 @@
 // fj_plan.cpp — validation of one render call, packing of prepared tables into
 // device layouts, and the frame schedule on the caller's stream.
 #include "fj_plan.h"
+#include "fj_autoexposure.h"
 #include "fj_diffusion_host.h"
 #include "fj_grain_host.h"
 #include "fj_halation_host.h"
@@ -356,7 +357,22 @@ bool validate_frame(const FjFrame& f, Diag& d) {
 
 // Families this build renders; everything else is rejected loudly.
 bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
-    if (p.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE) return d.unsupported("auto exposure metering");
+    if (p.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE) {
+        // meter_camera_exposure (JuicerCudaExecutor.cpp:1013-1066) and the
+        // launcher's checks (JuicerCudaAutoExposure.cu:700-740, :3145-3147).
+        const FjAutoExposure& a = p.auto_exposure;
+        const FjRect& sb = f.source.bounds;
+        if (a.preview_width <= 0 || a.preview_height <= 0)
+            return d.unsupported("MissingRequiredResource phase=4C field=auto_exposure_buffers");
+        if (a.preview_width > 256 || a.preview_height > 256 || a.meter_bounds.x2 <= a.meter_bounds.x1 ||
+            a.meter_bounds.y2 <= a.meter_bounds.y1)
+            return d.malformed("auto_exposure.preview");
+        if (a.source_bounds.x1 != sb.x1 || a.source_bounds.y1 != sb.y1 || a.source_bounds.x2 != sb.x2 ||
+            a.source_bounds.y2 != sb.y2)
+            return d.unsupported("auto exposure source bounds differ from the source image");
+        const bool print_route = p.route == FJ_ROUTE_NEGATIVE_PRINT || p.route == FJ_ROUTE_POSITIVE_PRINT;
+        if (!print_route && a.method == FJ_METER_MEDIAN) return d.unsupported("QuantizedMedianNotAcceptedForPhase3");
+    }
     if (p.scanner_post.flags & FJ_POST_GLARE) return d.unsupported("scanner glare");
     if (p.scanner_post.hash != 0 && p.scanner_post.lens_blur_sigma_px > 0.0f && p.scanner_post.lens_blur_radius > 0)
         return d.unsupported("scanner lens blur");
@@ -940,6 +956,9 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     long long post_off = -1;
     if (post) post_off = spatial_dir ? dir_off + 9 * plane : (q_off >= 0 ? q_off : take(3 * plane));
     const long long pad_off = padded_floats ? take(padded_floats) : -1;
+    const bool auto_exposure = (prepared.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE) != 0;
+    const long long ae_off =
+        auto_exposure ? take(static_cast<long long>((sizeof(AeScratch) + sizeof(float) - 1) / sizeof(float))) : -1;
     float* ws = nullptr;
     if (ws_floats) {
         e = dev.ensure_workspace(static_cast<size_t>(ws_floats) * sizeof(float));
@@ -951,6 +970,26 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     if (post) {
         pp.linear_out = 1;
         pp.linear_planes = ws + post_off;
+    }
+    if (auto_exposure) {
+        // Metered before every exposure launch of the frame, into this
+        // frame's workspace (concurrent frames meter into their own slots).
+        AeScratch* ae = reinterpret_cast<AeScratch*>(ws + ae_off);
+        AeArgs aa{};
+        aa.src = reinterpret_cast<const unsigned char*>(s.address);
+        aa.row_bytes = s.row_bytes;
+        aa.comps = comps;
+        aa.desc = prepared.auto_exposure;
+        aa.ex = pp.ex;
+        aa.partials = ae->partials;
+        aa.histogram = ae->histogram;
+        aa.max_y = &ae->max_y;
+        aa.scale = &ae->scale;
+        mark("auto_exposure");
+        e = launch_auto_exposure(aa, stream);
+        if (e != hipSuccess) return outcome_hip(e, "auto exposure", error);
+        pp.ex.ae = &ae->scale;
+        pp.ex.with_route = 1;
     }
     // Transfer tables first (built once per PSF and padded size).
     DiffusionStage diff_stage[2];

@@ -45,6 +45,8 @@ FJ_RAW_MALLETT_2019 = 1
 FJ_FILM_DECODE_CCTF = 1
 FJ_FILM_ADAPT_INPUT = 2
 FJ_FILM_AUTO_EXPOSURE = 4
+FJ_METER_CENTER_WEIGHTED, FJ_METER_AVERAGE, FJ_METER_MEDIAN, FJ_METER_PARTIAL = 0, 1, 2, 3
+FJ_METER_MATRIX, FJ_METER_MULTI_ZONE, FJ_METER_HIGHLIGHT_WEIGHTED = 4, 5, 6
 FJ_DIR_INACTIVE = 0
 FJ_DIR_NEGATIVE_DONOR_LANGMUIR = 1
 FJ_DIR_POSITIVE_RECEIVER_LANGMUIR = 2

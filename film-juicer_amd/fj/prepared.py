@@ -226,6 +226,42 @@ def simple_view(p: Prepared, *, encode: bool = True, gamut: bool = True, unsharp
     return p
 
 
+def auto_exposure_descriptor(source, meter, method: int) -> "abi.FjAutoExposure":
+    """The auto-exposure preview descriptor of a meter window (the meter
+    extent scaled to a long edge of at most 256, and its identity hash;
+    make_auto_exposure_preview_descriptor, src/Cuda/JuicerCudaExecutor.cpp:
+    774-827). source, meter: (x1, y1, x2, y2)."""
+    import struct
+    from . import descriptors
+    w, h = max(0, meter[2] - meter[0]), max(0, meter[3] - meter[1])
+    pw, ph = w, h
+    if max(w, h) > 256:
+        if w >= h:
+            pw, ph = 256, max(1, (h * 256 + w // 2) // w)
+        else:
+            ph, pw = 256, max(1, (w * 256 + h // 2) // h)
+    words = struct.pack("<10i", *source, *meter, pw, ph) + bytes([int(method) & 0xFF])
+    hh = descriptors._fnv(descriptors.FNV_OFFSET, words)
+    d = abi.FjAutoExposure()
+    d.source_bounds = abi.FjRect(*source)
+    d.meter_bounds = abi.FjRect(*meter)
+    d.preview_width, d.preview_height = pw, ph
+    d.method = int(method)
+    d.hash = hh or 1
+    return d
+
+
+def with_auto_exposure(p: Prepared, width: int, height: int, method: int = 0, meter=None) -> Prepared:
+    """Turns on camera auto-exposure metering for a full width x height source
+    with the given meter window (x1, y1, x2, y2; default the whole frame) and
+    method (FJ_METER_*)."""
+    source = (0, 0, width, height)
+    r = p.record
+    r.auto_exposure = auto_exposure_descriptor(source, tuple(meter) if meter else source, method)
+    r.film_exposure.flags |= abi.FJ_FILM_AUTO_EXPOSURE
+    return p
+
+
 def with_halation(p: Prepared, width: int, height: int, pixel_size_um: float, *, legacy: bool = True,
                   **recipe) -> Prepared:
     """Turns on scatter/halation for a full width x height frame with the

@@ -387,6 +387,7 @@ typedef struct Frame {
     int dens_out;       /* the back end stores density instead of printing */
     double* linear;   /* [h][w][3] */
     uint32_t errors;
+    double ae;          /* metered auto-exposure scale, 1 when off */
     pthread_mutex_t lock;
 } Frame;
 
@@ -433,23 +434,159 @@ static void px_raw(const FjoPlan* pl, const float* px, double E[3]) {
     }
 }
 
-/* Film raw exposure with the manual and route scales
- * (compute_film_raw_device, JuicerCudaDeviceHelpers.cuh:709-763). */
-static void px_exposure(const FjoPlan* pl, const float* px, double E[3]) {
+/* Film raw exposure with the auto-exposure (1 when off), manual and route
+ * scales (compute_film_raw_device, JuicerCudaDeviceHelpers.cuh:709-763). */
+static void px_exposure(const FjoPlan* pl, double ae, const float* px, double E[3]) {
     px_raw(pl, px, E);
-    double scale = pl->manual_scale * (double)pl->p.film_exposure.route_correction_scale;
+    double scale = ae * pl->manual_scale * (double)pl->p.film_exposure.route_correction_scale;
     if (!(isfinite(scale) && scale > 0.0)) scale = 1.0;
     for (int c = 0; c < 3; ++c) E[c] = fmax(0.0, E[c] * scale);
 }
 
-/* Camera carrier exposure: manual scale only
+/* Camera carrier exposure: auto-exposure and manual scales
  * (compute_camera_film_linear_exposure_device, JuicerCudaDeviceHelpers.cuh:
  * 709-744). */
-static void px_camera(const FjoPlan* pl, const float* px, double E[3]) {
+static void px_camera(const FjoPlan* pl, double ae, const float* px, double E[3]) {
     px_raw(pl, px, E);
-    double scale = pl->manual_scale;
+    double scale = ae * pl->manual_scale;
     if (!(isfinite(scale) && scale > 0.0)) scale = 1.0;
     for (int c = 0; c < 3; ++c) E[c] = fmax(0.0, E[c] * scale);
+}
+
+/* 2^(-log2 Y) with the reference's guards, rounded to the device float
+ * (JuicerCudaAutoExposure.cu:500-521). */
+static double ae_scale_from(double y) {
+    double ev = 0.0;
+    if (y > 0.0 && isfinite(y)) {
+        const double c = -log(y) / log(2.0);
+        if (isfinite(c)) ev = c;
+    }
+    const double s = exp2(ev);
+    return (isfinite(s) && s > 0.0) ? (double)(float)s : 1.0;
+}
+
+/* Preview index -> source coordinate (JuicerCudaAutoExposure.cu:75-85). */
+static int ae_coord(int i, int preview, int source) {
+    if (preview <= 0 || source <= 0) return 0;
+    const long long c = ((2LL * i + 1LL) * (long long)source) / (2LL * preview);
+    return (int)(c < 0 ? 0 : (c > source - 1 ? source - 1 : c));
+}
+
+double fjo_auto_exposure(const FjoPlan* pl, const float* src, int width, int height, int comps, ptrdiff_t stride) {
+    const FjAutoExposure* d = &pl->p.auto_exposure;
+    if (!(pl->p.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE)) return 1.0;
+    const int w = d->preview_width, h = d->preview_height;
+    const FjRect sb = d->source_bounds, mb = d->meter_bounds;
+    if (w <= 0 || h <= 0 || sb.x1 != 0 || sb.y1 != 0 || sb.x2 != width || sb.y2 != height) return -1.0;
+    const int method = (int)d->method;
+    double sy[4] = {0, 0, 0, 0}, sw[4] = {0, 0, 0, 0};
+    float* means = (float*)malloc(sizeof(float) * (size_t)w * h);
+    unsigned char* valid = (unsigned char*)calloc((size_t)w * h, 1);
+    float max_y = 0.0f;
+    const int maxd = w > h ? w : h;
+    for (int y = 0; y < h; ++y) {
+        for (int x = 0; x < w; ++x) {
+            const int px = mb.x1 + ae_coord(x, w, mb.x2 - mb.x1), py = mb.y1 + ae_coord(y, h, mb.y2 - mb.y1);
+            if (px < sb.x1 || px >= sb.x2 || py < sb.y1 || py >= sb.y2) continue;
+            double E[3];
+            px_raw(pl, src + (ptrdiff_t)(py - sb.y1) * stride + (ptrdiff_t)(px - sb.x1) * comps, E);
+            const double mean = (E[0] + E[1] + E[2]) / 3.0;
+            if (!isfinite(mean)) continue;
+            means[(size_t)y * w + x] = (float)mean;
+            valid[(size_t)y * w + x] = 1;
+            if ((float)mean > max_y) max_y = (float)mean;
+            /* metering_weights (JuicerCudaAutoExposure.cu:87-153) */
+            double wt[4] = {0, 0, 0, 0};
+            const double nx = ((double)x / w - 0.5) * w / maxd, ny = ((double)y / h - 0.5) * h / maxd;
+            /* Ring and spot membership are decisions on the reference's FP32
+             * radius (unfused single-precision operations, as the device meter
+             * evaluates them). */
+            const float inv_max = 1.0f / (float)maxd;
+            const float fx = (((float)x / (float)w) - 0.5f) * (float)w * inv_max;
+            const float fy = (((float)y / (float)h) - 0.5f) * (float)h * inv_max;
+            const float fxx = fx * fx, fyy = fy * fy;
+            const float radius = sqrtf(fxx + fyy);
+            if (method == FJ_METER_AVERAGE || method == FJ_METER_MEDIAN) {
+                wt[0] = 1.0;
+            } else if (method == FJ_METER_CENTER_WEIGHTED) {
+                wt[0] = exp(-(nx * nx) / (2.0 * 0.04)) * exp(-(ny * ny) / (2.0 * 0.04));
+            } else if (method == FJ_METER_PARTIAL) {
+                wt[0] = radius < 0.15f ? 1.0 : 0.0;
+                wt[3] = 1.0;
+            } else if (method == FJ_METER_MATRIX) {
+                const int ch = h / 5, cw = w / 5;
+                if (ch > 0 && cw > 0 && y / ch < 5 && x / cw < 5) {
+                    const double dy = (y / ch - 2.0) / 2.0, dx = (x / cw - 2.0) / 2.0;
+                    wt[0] = 0.5 * (1.0 + cos(3.14159265358979323846 * sqrt(dx * dx + dy * dy) / sqrt(2.0)));
+                }
+            } else if (method == FJ_METER_MULTI_ZONE) {
+                if (radius < 0.05f) wt[0] = 1.0;
+                else if (radius < 0.25f) wt[1] = 1.0;
+                else if (radius < 0.50f) wt[2] = 1.0;
+                wt[3] = 1.0;
+            } else if (method == FJ_METER_HIGHLIGHT_WEIGHTED) {
+                wt[0] = mean * mean;
+                wt[3] = 1.0;
+            }
+            for (int l = 0; l < 4; ++l) {
+                const double v = (isfinite(wt[l]) && wt[l] > 0.0) ? wt[l] : 0.0;
+                sy[l] += mean * v;
+                sw[l] += v;
+            }
+        }
+    }
+    double result = 1.0;
+    if (method == FJ_METER_MEDIAN) {
+        /* 2048-bin histogram median (:524-660) */
+        if (max_y > 0.0f && isfinite(max_y)) {
+            unsigned long long hist[2048] = {0}, total = 0;
+            for (size_t i = 0; i < (size_t)w * h; ++i) {
+                if (!valid[i]) continue;
+                const float m = means[i] < 0.0f ? 0.0f : means[i];
+                const float norm = fminf(1.0f, m / max_y);
+                hist[(int)(norm * 2047.0f)]++;
+                total++;
+            }
+            if (total) {
+                const unsigned long long target = total / 2ull;
+                unsigned long long cum = 0, in_bin = 0;
+                int bin = 2047;
+                for (int i = 0; i < 2048; ++i) {
+                    if (cum + hist[i] > target) {
+                        bin = i;
+                        in_bin = hist[i];
+                        break;
+                    }
+                    cum += hist[i];
+                }
+                const double frac = in_bin ? (double)(target - cum) / (double)in_bin : 0.0;
+                result = ae_scale_from(((double)bin + frac) * ((double)max_y / 2048.0));
+            }
+        }
+    } else {
+        double y = 0.0;
+        int has = 0;
+        if (method == FJ_METER_PARTIAL) {
+            const int l = sw[0] > 0.0 ? 0 : 3;
+            if (sw[l] > 0.0) { y = sy[l] / sw[l]; has = 1; }
+        } else if (method == FJ_METER_MULTI_ZONE) {
+            const double ring[3] = {0.50, 0.30, 0.20};
+            double sum = 0.0, tot = 0.0;
+            for (int l = 0; l < 3; ++l)
+                if (sw[l] > 0.0) { sum += ring[l] * sy[l] / sw[l]; tot += ring[l]; }
+            if (tot > 0.0) { y = sum / tot; has = 1; }
+            else if (sw[3] > 0.0) { y = sy[3] / sw[3]; has = 1; }
+        } else if (method == FJ_METER_HIGHLIGHT_WEIGHTED && sw[0] < 1e-12) {
+            if (sw[3] > 0.0) { y = sy[3] / sw[3]; has = 1; }
+        } else if (sw[0] > 0.0) {
+            y = sy[0] / sw[0];
+            has = 1;
+        }
+        result = has ? ae_scale_from(y) : 1.0;
+    }
+    free(means);
+    free(valid);
+    return result;
 }
 
 /* Route correction applied when a carrier is developed
@@ -1370,7 +1507,7 @@ static void rows_chain(Frame* f, int y0, int y1) {
             if (f->carrier[0])
                 carrier_exposure(pl, f->carrier, (size_t)y * f->w + x, E);
             else
-                px_exposure(pl, f->src + y * f->src_stride + (ptrdiff_t)x * f->comps, E);
+                px_exposure(pl, f->ae, f->src + y * f->src_stride + (ptrdiff_t)x * f->comps, E);
             errs |= px_develop(pl, E, D);
             errs |= px_back(pl, f, (size_t)y * f->w + x, D);
         }
@@ -1383,7 +1520,7 @@ static void rows_carrier(Frame* f, int y0, int y1) {
     for (int y = y0; y < y1; ++y)
         for (int x = 0; x < f->w; ++x) {
             double E[3];
-            px_camera(f->plan, f->src + y * f->src_stride + (ptrdiff_t)x * f->comps, E);
+            px_camera(f->plan, f->ae, f->src + y * f->src_stride + (ptrdiff_t)x * f->comps, E);
             const size_t idx = (size_t)y * f->w + x;
             for (int c = 0; c < 3; ++c) f->carrier[c][idx] = E[c];
         }
@@ -1414,7 +1551,7 @@ static void rows_dir_source(Frame* f, int y0, int y1) {
             if (f->carrier[0])
                 carrier_exposure(pl, f->carrier, idx, E);
             else
-                px_exposure(pl, f->src + y * f->src_stride + (ptrdiff_t)x * f->comps, E);
+                px_exposure(pl, f->ae, f->src + y * f->src_stride + (ptrdiff_t)x * f->comps, E);
             uint32_t fail = 0;
             for (int b = 0; b < 3; ++b) {
                 const double raw = log10(fmax(E[b], 0.0) + 1e-10);
@@ -1511,6 +1648,8 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     f.src_stride = src_stride;
     f.dst = dst;
     f.dst_stride = dst_stride;
+    f.ae = fjo_auto_exposure(plan, src, width, height, comps, src_stride);
+    if (f.ae < 0.0) return 5; /* auto exposure over a source other than the frame */
     f.linear = (double*)malloc(sizeof(double) * N * 3);
     pthread_mutex_init(&f.lock, NULL);
     int rc = 0;

@@ -98,6 +98,12 @@ int fjo_plan_scanner_lut(const FjoPlan* plan, int* resolution, const float** log
  *        after gamut compression and post effects, before output encoding.
  *   pixel_size_um / seed inputs come from the prepared record.
  * Returns 0 on success; *error_flags receives the scan/DIR error bits. */
+/* Camera auto-exposure scale of a frame (JuicerCudaAutoExposure.cu): 1 when
+ * the record's auto exposure is off, -1 when its source bounds are not the
+ * rendered frame (0, 0, width, height). */
+double fjo_auto_exposure(const FjoPlan* plan, const float* src, int width, int height, int components,
+                         ptrdiff_t src_stride_floats);
+
 int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int components,
                ptrdiff_t src_stride_floats, float* dst, ptrdiff_t dst_stride_floats,
                double* linear_rgb, int threads, uint32_t* error_flags);

@@ -54,6 +54,8 @@ def lib():
             getattr(L, name).argtypes = [C.c_float]
         L.fjo_render.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t, fp, C.c_ssize_t,
                                  C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_uint32)]
+        L.fjo_auto_exposure.restype = C.c_double
+        L.fjo_auto_exposure.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t]
         _lib = L
     return _lib
 
@@ -188,6 +190,12 @@ class Plan:
         r = res.value
         sizes = [r ** 3 * 3] * 4 + [(r - 1) ** 3 * 3] * 2
         return r, [np.ctypeslib.as_array(p, shape=(n,)).copy() for p, n in zip(ptrs, sizes)]
+
+    def auto_exposure(self, src: np.ndarray) -> float:
+        """The metered auto-exposure scale of an HxWxC frame (1 when off)."""
+        src = np.ascontiguousarray(src, dtype=np.float32)
+        h, w, c = src.shape
+        return lib().fjo_auto_exposure(self.handle, _fp(src), w, h, c, w * c)
 
     def render(self, src: np.ndarray, threads: int = 1, want_dst: bool = True):
         """Returns (encoded float32 HxWxC or None, linear float64 HxWx3, error flags)."""
