@@ -9,7 +9,9 @@
  * development (+DIR) -> [print exposure -> print development] -> scanner LUT
  * -> scanner colour -> output gamut -> [post effects] -> output encoding.
  * Every stage works on whole double planes so spatial operators can sit
- * between stages exactly where the reference schedule puts them.
+ * between stages exactly where the reference schedule puts them. Camera auto
+ * exposure (src/Cuda/JuicerCudaAutoExposure.cu) meters the frame first
+ * (fjo_auto_exposure).
  */
 #include "fj_oracle.h"
 
