@@ -2,6 +2,7 @@
 // device layouts, and the frame schedule on the caller's stream.
 #include "fj_plan.h"
 #include "fj_autoexposure.h"
+#include "fj_highlight.h"
 #include "fj_diffusion_host.h"
 #include "fj_grain_host.h"
 #include "fj_halation_host.h"
@@ -383,7 +384,16 @@ bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
         std::string field;
         if (!diffusion_decode(p.optics, is_print_route(p.route), field)) return d.malformed(field.c_str());
     }
-    if (p.optics.flags & FJ_OPTICS_HIGHLIGHT_BOOST) return d.unsupported("highlight boost");
+    {
+        // Highlight boost runs when its recipe is bound
+        // (launch_highlight_boost, JuicerCudaHighlightBoost.cu:195-205).
+        const FjHighlightBoost& hb = p.optics.highlight_boost;
+        const bool flag = (p.optics.flags & FJ_OPTICS_HIGHLIGHT_BOOST) != 0;
+        if (flag != (hb.recipe_hash != 0)) return d.malformed("optics.highlight_boost");
+        if (flag && (!std::isfinite(hb.protection) || !std::isfinite(hb.sharpness) || !std::isfinite(hb.peak_gain) ||
+                     hb.protection < 0.0 || hb.sharpness < 0.0 || hb.peak_gain < 1.0))
+            return d.malformed("optics.highlight_boost");
+    }
     if (p.optics.flags & FJ_OPTICS_SCATTER_HALATION) {
         std::string field;
         if (!halation_validate(p.optics.scatter_halation, field)) return d.malformed(field.c_str());
@@ -891,7 +901,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     const bool halation = (prepared.optics.flags & FJ_OPTICS_SCATTER_HALATION) != 0;
     const bool camera_diffusion = (prepared.optics.flags & FJ_OPTICS_CAMERA_DIFFUSION) != 0;
     const bool enlarger_diffusion = pp.print && (prepared.optics.flags & FJ_OPTICS_ENLARGER_DIFFUSION) != 0;
-    const bool carrier = halation || camera_diffusion;
+    const bool boost = (prepared.optics.flags & FJ_OPTICS_HIGHLIGHT_BOOST) != 0;
+    const bool carrier = halation || camera_diffusion || boost;
     const long long plane = static_cast<long long>(pp.io.width) * pp.io.height;
     if (halation && pp.io.width > dev.halation_max_line)
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput halation row above the LDS line limit", error);
@@ -956,6 +967,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     long long post_off = -1;
     if (post) post_off = spatial_dir ? dir_off + 9 * plane : (q_off >= 0 ? q_off : take(3 * plane));
     const long long pad_off = padded_floats ? take(padded_floats) : -1;
+    const long long hb_off =
+        boost ? take(static_cast<long long>((sizeof(HbState) + sizeof(float) - 1) / sizeof(float))) : -1;
     const bool auto_exposure = (prepared.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE) != 0;
     const long long ae_off =
         auto_exposure ? take(static_cast<long long>((sizeof(AeScratch) + sizeof(float) - 1) / sizeof(float))) : -1;
@@ -1005,6 +1018,23 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         mark("camera_carrier");
         e = launch_pixel_stage(pp, kCarrier, cus, stream);
         if (e != hipSuccess) return outcome_hip(e, "camera carrier stage", error);
+        if (boost) {
+            // expose_print_camera / expose_direct_camera
+            // (JuicerCudaExecutor.cpp:1886-1960): boosted right after the
+            // exposure, before camera diffusion and halation.
+            mark("highlight_boost");
+            const FjHighlightBoost& hb = prepared.optics.highlight_boost;
+            HbArgs ha{};
+            for (int c = 0; c < 3; ++c) ha.planes[c] = pp.carrier[c];
+            ha.n = static_cast<long long>(pp.io.width) * pp.io.height;
+            ha.protection = hb.protection;
+            ha.sharpness = hb.sharpness;
+            ha.peak_gain = hb.peak_gain;
+            ha.state = reinterpret_cast<HbState*>(ws + hb_off);
+            ha.error_word = pp.error_word;
+            e = launch_highlight_boost(ha, stream);
+            if (e != hipSuccess) return outcome_hip(e, "highlight boost", error);
+        }
         // Descriptor channels are R, G, B; carrier planes are B/G/R-sensitive.
         float* cur[3];
         float* alt[3];
@@ -1172,6 +1202,19 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     e = hipEventRecord(dev.error_events[slot], stream);
     if (e != hipSuccess) return outcome_hip(e, "error event", error);
     dev.error_pending[slot] = true;
+    if (boost) {
+        // The boost's disposition is checked before the frame returns
+        // (check_highlight_boost, ProcessRoot.cpp:4063-4096).
+        e = hipEventSynchronize(dev.error_events[slot]);
+        if (e != hipSuccess) return outcome_hip(e, "highlight boost status", error);
+        dev.error_pending[slot] = false;
+        const unsigned int bits = dev.error_host[slot];
+        earlier |= bits;
+        if (bits & kHbInvalidCarrierBit)
+            return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "HighlightBoostInvalidCarrier", error);
+        if (bits & kHbPeakOverflowBit)
+            return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "HighlightBoostPeakOverflow", error);
+    }
 
     FjRenderOutcome out{};
     out.status = status_of(FJ_STATUS_SUCCESS);

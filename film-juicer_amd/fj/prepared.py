@@ -262,6 +262,23 @@ def with_auto_exposure(p: Prepared, width: int, height: int, method: int = 0, me
     return p
 
 
+def with_highlight_boost(p: Prepared, width: int, height: int, protection: float, sharpness: float,
+                         peak_gain: float) -> Prepared:
+    """Turns on the camera-carrier highlight boost for a full width x height
+    frame (FjHighlightBoost; JuicerCudaHighlightBoost.cu)."""
+    import struct
+    from . import descriptors
+    o = p.record.optics
+    o.flags |= abi.FJ_OPTICS_HIGHLIGHT_BOOST
+    o.highlight_boost.protection = float(protection)
+    o.highlight_boost.sharpness = float(sharpness)
+    o.highlight_boost.peak_gain = float(peak_gain)
+    o.highlight_boost.recipe_hash = descriptors._fnv(descriptors.FNV_OFFSET,
+                                                     struct.pack("<3d", protection, sharpness, peak_gain)) or 1
+    o.full_frame = abi.FjExtent(0, 0, width, height)
+    return p
+
+
 def with_halation(p: Prepared, width: int, height: int, pixel_size_um: float, *, legacy: bool = True,
                   **recipe) -> Prepared:
     """Turns on scatter/halation for a full width x height frame with the

@@ -455,6 +455,39 @@ static void px_camera(const FjoPlan* pl, double ae, const float* px, double E[3]
     for (int c = 0; c < 3; ++c) E[c] = fmax(0.0, E[c] * scale);
 }
 
+/* (e^t - t - 1) / t^2 (normalized_exponential, JuicerCudaHighlightBoost.cu:62-70). */
+static double hb_norm(double t) {
+    if (fabs(t) >= 1e-4) return (expm1(t) - t) / (t * t);
+    return 0.5 + t * (1.0 / 6.0 + t * (1.0 / 24.0 + t / 120.0));
+}
+
+int fjo_highlight_boost(const FjHighlightBoost* hb, double* const planes[3], size_t n) {
+    /* JuicerCudaHighlightBoost.cu:40-186: maximum m over the three planes
+     * (any non-finite or negative sample: invalid carrier), bypass when
+     * m <= protection, overflow when m * gain > FLT_MAX; samples s > p become
+     * s + m (g - 1) f^2 N(k f) / N(k), f = (s - p) / (m - p), k = k0 (m - p) / m. */
+    double m = 0.0;
+    for (int c = 0; c < 3; ++c)
+        for (size_t i = 0; i < n; ++i) {
+            const double v = planes[c][i];
+            if (!(isfinite(v) && v >= 0.0)) return 6;
+            if (v > m) m = v;
+        }
+    const double p = hb->protection;
+    if (m <= p) return 0;
+    if (m * hb->peak_gain > (double)FLT_MAX) return 7;
+    const double range = m - p, shape = hb->sharpness * range / m;
+    const double inv = 1.0 / hb_norm(shape), delta = m * (hb->peak_gain - 1.0);
+    for (int c = 0; c < 3; ++c)
+        for (size_t i = 0; i < n; ++i) {
+            const double v = planes[c][i];
+            if (!(v > p)) continue;
+            const double f = (v - p) / range;
+            planes[c][i] = v + delta * f * f * hb_norm(shape * f) * inv;
+        }
+    return 0;
+}
+
 /* 2^(-log2 Y) with the reference's guards, rounded to the device float
  * (JuicerCudaAutoExposure.cu:500-521). */
 static double ae_scale_from(double y) {
@@ -1655,13 +1688,16 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     f.linear = (double*)malloc(sizeof(double) * N * 3);
     pthread_mutex_init(&f.lock, NULL);
     int rc = 0;
-    if (halation || camera_diffusion) {
+    const int boost = (p->optics.flags & FJ_OPTICS_HIGHLIGHT_BOOST) != 0;
+    if (halation || camera_diffusion || boost) {
         /* expose_print_camera (JuicerCudaExecutor.cpp:1874-1989): carrier,
-         * camera diffusion, then scatter/halation in place on it. */
+         * highlight boost, camera diffusion, then scatter/halation in place
+         * on it. */
         for (int c = 0; c < 3; ++c) f.carrier[c] = (double*)malloc(sizeof(double) * N);
         run_rows(&f, rows_carrier, threads);
         double* const rgb_planes[3] = {f.carrier[2], f.carrier[1], f.carrier[0]};
-        if (camera_diffusion) rc = fjo_diffusion(&p->optics.camera, rgb_planes, width, height);
+        if (boost) rc = fjo_highlight_boost(&p->optics.highlight_boost, rgb_planes, N);
+        if (rc == 0 && camera_diffusion) rc = fjo_diffusion(&p->optics.camera, rgb_planes, width, height);
         if (rc == 0 && halation) rc = fjo_halation(&p->optics.scatter_halation, rgb_planes, width, height);
     }
     if (enlarger_diffusion)

@@ -98,6 +98,11 @@ int fjo_plan_scanner_lut(const FjoPlan* plan, int* resolution, const float** log
  *        after gamut compression and post effects, before output encoding.
  *   pixel_size_um / seed inputs come from the prepared record.
  * Returns 0 on success; *error_flags receives the scan/DIR error bits. */
+/* Highlight boost of the camera carrier (three planes of n doubles, in
+ * place; JuicerCudaHighlightBoost.cu). Returns 0 (applied or bypassed),
+ * 6 for an invalid carrier, 7 when the peak exceeds FLT_MAX. */
+int fjo_highlight_boost(const FjHighlightBoost* boost, double* const planes[3], size_t n);
+
 /* Camera auto-exposure scale of a frame (JuicerCudaAutoExposure.cu): 1 when
  * the record's auto exposure is off, -1 when its source bounds are not the
  * rendered frame (0, 0, width, height). */

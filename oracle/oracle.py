@@ -54,6 +54,8 @@ def lib():
             getattr(L, name).argtypes = [C.c_float]
         L.fjo_render.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t, fp, C.c_ssize_t,
                                  C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_uint32)]
+        L.fjo_highlight_boost.restype = C.c_int
+        L.fjo_highlight_boost.argtypes = [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.c_size_t]
         L.fjo_auto_exposure.restype = C.c_double
         L.fjo_auto_exposure.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t]
         _lib = L
@@ -209,6 +211,15 @@ class Plan:
         if rc != 0:
             raise RuntimeError(f"oracle render failed rc={rc}")
         return dst, lin, flags.value
+
+
+def highlight_boost(boost, planes):
+    """Highlight boost (an FjHighlightBoost) of three carrier planes (float64,
+    any shape); returns (rc, boosted planes)."""
+    arr = [np.ascontiguousarray(q, dtype=np.float64).copy() for q in planes]
+    ptrs = (C.POINTER(C.c_double) * 3)(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arr])
+    rc = lib().fjo_highlight_boost(C.byref(boost), ptrs, arr[0].size)
+    return rc, arr
 
 
 def grain_apply(record, density) -> np.ndarray:
