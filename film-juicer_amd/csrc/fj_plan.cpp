@@ -4,6 +4,7 @@
 #include "fj_autoexposure.h"
 #include "fj_highlight.h"
 #include "fj_diffusion_host.h"
+#include "fj_grain.h"
 #include "fj_grain_host.h"
 #include "fj_halation_host.h"
 
@@ -375,8 +376,8 @@ bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
         if (!print_route && a.method == FJ_METER_MEDIAN) return d.unsupported("QuantizedMedianNotAcceptedForPhase3");
     }
     if (p.scanner_post.flags & FJ_POST_GLARE) return d.unsupported("scanner glare");
-    if (p.scanner_post.hash != 0 && p.scanner_post.lens_blur_sigma_px > 0.0f && p.scanner_post.lens_blur_radius > 0)
-        return d.unsupported("scanner lens blur");
+    if (p.scanner_post.hash != 0 && p.scanner_post.lens_blur_sigma_px > 0.0f && p.scanner_post.lens_blur_radius > 100)
+        return d.unsupported("scanner lens blur radius above 100");
     if (p.optics.flags & ~(FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_SCATTER_HALATION |
                            FJ_OPTICS_HIGHLIGHT_BOOST))
         return d.malformed("optics.flags");
@@ -460,9 +461,17 @@ bool check_grain_frame(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
     return true;
 }
 
-bool post_active(const FjScannerPostEffects& sp) {
+bool unsharp_active(const FjScannerPostEffects& sp) {
     return sp.hash != 0 && sp.unsharp_radius > 0 && std::isfinite(sp.unsharp_amount) && sp.unsharp_amount > 0.0f;
 }
+
+bool lens_active(const FjScannerPostEffects& sp) {
+    return sp.hash != 0 && sp.lens_blur_sigma_px > 0.0f && sp.lens_blur_radius > 0;
+}
+
+// Post effects on the scanner-linear planes: lens blur, then the unsharp
+// mask (launch_focused_scanner_post_output, JuicerCudaScanPipeline.cu:2733-2870).
+bool post_active(const FjScannerPostEffects& sp) { return unsharp_active(sp) || lens_active(sp); }
 
 // ---------------------------------------------------------------------------
 // packing
@@ -966,6 +975,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     const long long grain_off = grain ? take(5 * plane) : -1;
     long long post_off = -1;
     if (post) post_off = spatial_dir ? dir_off + 9 * plane : (q_off >= 0 ? q_off : take(3 * plane));
+    const long long lens_off = (post && lens_active(prepared.scanner_post)) ? take(plane) : -1;
     const long long pad_off = padded_floats ? take(padded_floats) : -1;
     const long long hb_off =
         boost ? take(static_cast<long long>((sizeof(HbState) + sizeof(float) - 1) / sizeof(float))) : -1;
@@ -1178,15 +1188,44 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         po.co = pp.co;
         po.planes = pp.linear_planes;
         const FjScannerPostEffects& sp = prepared.scanner_post;
-        std::vector<float> w = gaussian_weights(sp.unsharp_radius, sp.unsharp_sigma_px);
+        if (lens_active(sp)) {
+            std::vector<float> lw = gaussian_weights(sp.lens_blur_radius, sp.lens_blur_sigma_px);
+            if (lw.empty())
+                return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput lens blur radius/sigma", error);
+            const void* lptr = nullptr;
+            e = dev.table(hash_bytes(lw.data(), lw.size() * 4, 0x4C42ull), lw.data(), lw.size() * 4, &lptr);
+            if (e != hipSuccess) return outcome_hip(e, "lens blur weights", error);
+            mark("post_lens_blur");
+            const long long np = static_cast<long long>(pp.io.width) * pp.io.height;
+            for (int c = 0; c < 3 && e == hipSuccess; ++c) {
+                GrainBlurArgs h{};
+                h.in = pp.linear_planes + c * np;
+                h.out = ws + lens_off;
+                h.weights = static_cast<const float*>(lptr);
+                h.radius = sp.lens_blur_radius;
+                h.width = pp.io.width;
+                h.height = pp.io.height;
+                e = launch_grain_blur_h(h, stream);
+                if (e != hipSuccess) break;
+                GrainBlurArgs v = h;
+                v.in = ws + lens_off;
+                v.out = pp.linear_planes + c * np;
+                v.initialize = 1;
+                e = launch_grain_blur_v(v, stream);
+            }
+            if (e != hipSuccess) return outcome_hip(e, "lens blur", error);
+        }
+        static const float kIdentityTap = 1.0f;
+        std::vector<float> w = unsharp_active(sp) ? gaussian_weights(sp.unsharp_radius, sp.unsharp_sigma_px)
+                                                  : std::vector<float>{kIdentityTap};
         if (w.empty() || sp.unsharp_radius > 16)
             return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput unsharp radius/sigma", error);
         const void* wptr = nullptr;
         e = dev.table(hash_bytes(w.data(), w.size() * 4, 0x5057ull), w.data(), w.size() * 4, &wptr);
         if (e != hipSuccess) return outcome_hip(e, "unsharp weights", error);
         po.weights = static_cast<const float*>(wptr);
-        po.radius = sp.unsharp_radius;
-        po.amount = sp.unsharp_amount;
+        po.radius = unsharp_active(sp) ? sp.unsharp_radius : 0;
+        po.amount = unsharp_active(sp) ? sp.unsharp_amount : 0.0f;
         mark("post_unsharp_encode");
         e = launch_post(po, stream);
         if (e != hipSuccess) return outcome_hip(e, "post stage", error);

@@ -262,6 +262,17 @@ def with_auto_exposure(p: Prepared, width: int, height: int, method: int = 0, me
     return p
 
 
+def with_lens_blur(p: Prepared, sigma_px: float, radius: int | None = None) -> Prepared:
+    """Turns on the scanner lens blur (a Gaussian of sigma_px, radius
+    int(3 sigma + 0.5) by default) before the unsharp mask."""
+    sp = p.record.scanner_post
+    sp.lens_blur_sigma_px = float(sigma_px)
+    sp.lens_blur_radius = int(radius if radius is not None else int(3.0 * sigma_px + 0.5))
+    if sp.hash == 0:
+        sp.hash = 0x4C454E53
+    return p
+
+
 def with_highlight_boost(p: Prepared, width: int, height: int, protection: float, sharpness: float,
                          peak_gain: float) -> Prepared:
     """Turns on the camera-carrier highlight boost for a full width x height

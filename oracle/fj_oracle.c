@@ -1412,6 +1412,26 @@ static int reflect_half(long i, int n) {
     return (int)i;
 }
 
+/* Lens blur of one plane: the separable half-sample-reflected Gaussian of
+ * blur_plane_in_place (JuicerCudaScanPipeline.cu:2686-2731, kernels :317-430). */
+static void lens_plane(double* v, int W, int H, const float* w, int r) {
+    const size_t N = (size_t)W * H;
+    double* h = (double*)malloc(sizeof(double) * N);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double a = 0.0;
+            for (int o = -r; o <= r; ++o) a += v[(size_t)y * W + reflect_half(x + o, W)] * (double)w[o + r];
+            h[(size_t)y * W + x] = isfinite(a) ? a : 0.0;
+        }
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double a = 0.0;
+            for (int o = -r; o <= r; ++o) a += h[(size_t)reflect_half(y + o, H) * W + x] * (double)w[o + r];
+            v[(size_t)y * W + x] = isfinite(a) ? a : 0.0;
+        }
+    free(h);
+}
+
 static void unsharp_plane(double* v, int W, int H, const float* w, int r, double amount) {
     const size_t N = (size_t)W * H;
     double* h = (double*)malloc(sizeof(double) * N);
@@ -1727,6 +1747,22 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
         if (rc == 0) run_rows(&f, rows_enlarged, threads);
     }
     const FjScannerPostEffects* sp = &p->scanner_post;
+    if (rc == 0 && sp->hash != 0 && sp->lens_blur_sigma_px > 0.0f && sp->lens_blur_radius > 0) {
+        /* lens blur before the unsharp mask (JuicerCudaScanPipeline.cu:2795-2809) */
+        float* w = (float*)malloc(sizeof(float) * (2 * (size_t)sp->lens_blur_radius + 1));
+        if (sp->lens_blur_radius > 4096 || fjo_gaussian_weights(sp->lens_blur_radius, sp->lens_blur_sigma_px, w)) {
+            rc = 4;
+        } else {
+            double* pl3 = (double*)malloc(sizeof(double) * N);
+            for (int c = 0; c < 3; ++c) {
+                for (size_t i = 0; i < N; ++i) pl3[i] = f.linear[i * 3 + c];
+                lens_plane(pl3, width, height, w, sp->lens_blur_radius);
+                for (size_t i = 0; i < N; ++i) f.linear[i * 3 + c] = pl3[i];
+            }
+            free(pl3);
+        }
+        free(w);
+    }
     if (rc == 0 && sp->hash != 0 && sp->unsharp_radius > 0 && isfinite(sp->unsharp_amount) && sp->unsharp_amount > 0.0f) {
         float w[8193];
         if (sp->unsharp_radius > 4096 || fjo_gaussian_weights(sp->unsharp_radius, sp->unsharp_sigma_px, w)) {
