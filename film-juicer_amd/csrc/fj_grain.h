@@ -79,6 +79,9 @@ struct GrainBlurArgs {
     const float* sub = nullptr;
     float bias = 0.0f;
     float gain = 1.0f;               // 1: none
+    // vertical pass, unsharp combine instead of the epilogue above:
+    // out = x + weight * (x - blur), with x the previous out
+    int unsharp = 0;
 };
 
 // Elementwise passes over n samples.

@@ -463,6 +463,13 @@ __global__ __launch_bounds__(256) void grain_blur_v_kernel(GrainBlurArgs a) {
         for (int j = 0; j < taps; ++j) acc = fmaf(tile[(yy + j) * 64 + tx], w[j], acc);
         const float blurred = san(acc);
         const long long i = static_cast<long long>(y) * a.width + x;
+        if (a.unsharp) {
+            // Scanner unsharp mask (optics_unsharp_vertical_combine_kernel,
+            // JuicerCudaScanPipeline.cu:586-660).
+            const float v0 = a.out[i];
+            a.out[i] = san(fmaf(a.weight, v0 - blurred, v0));
+            continue;
+        }
         const float prev = a.initialize ? 0.0f : a.out[i];
         float v = a.weighted ? prev + wf * blurred : prev + blurred;
         v = san(v);

@@ -6,6 +6,8 @@
 namespace fjx {
 
 constexpr int kHbMaxBlocks = 1024;
+// Largest curve sharpness admitted: the recipe's 28^(1 - range).
+constexpr double kHighlightMaxSharpness = 28.0;
 // Frame error-word bits of the failing dispositions.
 constexpr unsigned int kHbInvalidCarrierBit = 1u << 9;
 constexpr unsigned int kHbPeakOverflowBit = 1u << 10;

@@ -36,6 +36,8 @@ bool build_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& 
 // compute_units CUs (at most one per 256-pixel tile).
 hipError_t launch_pixel_stage(const PixelParams& p, int mode, int compute_units, hipStream_t stream);
 hipError_t launch_post(const PostParams& p, hipStream_t stream);
+// Largest unsharp radius the fused post kernel holds in its LDS tile.
+int max_post_radius();
 hipError_t launch_clear_word(unsigned int* word, hipStream_t stream);
 size_t fused_lds_bytes(const PixelParams& p);
 

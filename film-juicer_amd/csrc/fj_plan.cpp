@@ -216,12 +216,19 @@ namespace {
 
 struct Diag {
     std::string text;
+    uint32_t category = FJ_STATUS_UNSUPPORTED_INPUT;
     bool malformed(const char* field) {
         text = std::string("MalformedPreparedHostData field=") + field;
         return false;
     }
     bool unsupported(const char* what) {
         text = std::string("UnsupportedInput ") + what;
+        return false;
+    }
+    // A descriptor the reference admits but cannot prepare.
+    bool preparation(const char* what) {
+        text = what;
+        category = FJ_STATUS_PREPARATION_FAILURE;
         return false;
     }
 };
@@ -357,6 +364,42 @@ bool validate_frame(const FjFrame& f, Diag& d) {
     return true;
 }
 
+// Scanner Gaussian kernels: a bound post descriptor prepares its lens-blur
+// and unsharp kernels whenever their sigma is positive, and refuses a
+// non-finite sigma or a radius outside 1..75 (prepare_scanner_post_effects,
+// src/ProcessRoot.cpp:3121-3158; ensure_gaussian_kernel,
+// src/Cuda/JuicerCudaResources.cpp:6436-6448).
+constexpr int kMaxGaussianRadius = 75;
+
+bool gaussian_valid(float sigma, int radius) {
+    return std::isfinite(sigma) && sigma > 0.0f && radius >= 1 && radius <= kMaxGaussianRadius;
+}
+
+bool post_kernels_valid(const FjScannerPostEffects& sp) {
+    if (sp.hash == 0) return true;
+    if (sp.lens_blur_sigma_px > 0.0f && !gaussian_valid(sp.lens_blur_sigma_px, sp.lens_blur_radius)) return false;
+    if (sp.unsharp_sigma_px > 0.0f && !gaussian_valid(sp.unsharp_sigma_px, sp.unsharp_radius)) return false;
+    return true;
+}
+
+// Launch conditions (launch_focused_scanner_post_output,
+// JuicerCudaScanPipeline.cu:2793-2813): lens blur with a prepared kernel;
+// the unsharp mask with a prepared kernel and a finite positive amount.
+bool lens_active(const FjScannerPostEffects& sp) { return sp.hash != 0 && sp.lens_blur_sigma_px > 0.0f; }
+
+bool unsharp_active(const FjScannerPostEffects& sp) {
+    return sp.hash != 0 && sp.unsharp_sigma_px > 0.0f && sp.unsharp_radius > 0 && std::isfinite(sp.unsharp_amount) &&
+           sp.unsharp_amount > 0.0f;
+}
+
+// Post effects on the scanner-linear planes: lens blur, then the unsharp
+// mask.
+bool post_active(const FjScannerPostEffects& sp) { return unsharp_active(sp) || lens_active(sp); }
+
+// Unsharp masks wider than the fused post kernel's LDS tile run as a
+// separate blur + combine before the encode.
+bool unsharp_separate(const FjScannerPostEffects& sp) { return unsharp_active(sp) && sp.unsharp_radius > max_post_radius(); }
+
 // Families this build renders; everything else is rejected loudly.
 bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
     if (p.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE) {
@@ -376,8 +419,7 @@ bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
         if (!print_route && a.method == FJ_METER_MEDIAN) return d.unsupported("QuantizedMedianNotAcceptedForPhase3");
     }
     if (p.scanner_post.flags & FJ_POST_GLARE) return d.unsupported("scanner glare");
-    if (p.scanner_post.hash != 0 && p.scanner_post.lens_blur_sigma_px > 0.0f && p.scanner_post.lens_blur_radius > 100)
-        return d.unsupported("scanner lens blur radius above 100");
+    if (!post_kernels_valid(p.scanner_post)) return d.preparation("gaussian descriptor is invalid");
     if (p.optics.flags & ~(FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_SCATTER_HALATION |
                            FJ_OPTICS_HIGHLIGHT_BOOST))
         return d.malformed("optics.flags");
@@ -391,8 +433,12 @@ bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
         const FjHighlightBoost& hb = p.optics.highlight_boost;
         const bool flag = (p.optics.flags & FJ_OPTICS_HIGHLIGHT_BOOST) != 0;
         if (flag != (hb.recipe_hash != 0)) return d.malformed("optics.highlight_boost");
+        // The recipe's sharpness is 28^(1 - range), range in [0, 1]
+        // (rust optics/highlight_boost.rs:56, :84); larger shapes overflow
+        // the curve's normalization.
         if (flag && (!std::isfinite(hb.protection) || !std::isfinite(hb.sharpness) || !std::isfinite(hb.peak_gain) ||
-                     hb.protection < 0.0 || hb.sharpness < 0.0 || hb.peak_gain < 1.0))
+                     hb.protection < 0.0 || hb.sharpness < 0.0 || hb.sharpness > kHighlightMaxSharpness ||
+                     hb.peak_gain < 1.0))
             return d.malformed("optics.highlight_boost");
     }
     if (p.optics.flags & FJ_OPTICS_SCATTER_HALATION) {
@@ -461,17 +507,6 @@ bool check_grain_frame(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
     return true;
 }
 
-bool unsharp_active(const FjScannerPostEffects& sp) {
-    return sp.hash != 0 && sp.unsharp_radius > 0 && std::isfinite(sp.unsharp_amount) && sp.unsharp_amount > 0.0f;
-}
-
-bool lens_active(const FjScannerPostEffects& sp) {
-    return sp.hash != 0 && sp.lens_blur_sigma_px > 0.0f && sp.lens_blur_radius > 0;
-}
-
-// Post effects on the scanner-linear planes: lens blur, then the unsharp
-// mask (launch_focused_scanner_post_output, JuicerCudaScanPipeline.cu:2733-2870).
-bool post_active(const FjScannerPostEffects& sp) { return unsharp_active(sp) || lens_active(sp); }
 
 // ---------------------------------------------------------------------------
 // packing
@@ -850,7 +885,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     if (!validate_frame(frame, d) || !check_supported(prepared, frame, d) ||
         !check_spatial_dir(prepared, frame, dev.max_line, d) || !check_optics_frame(prepared, frame, d) ||
         !check_grain_frame(prepared, frame, d))
-        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
+        return outcome_diag(d.category, d.text, error);
     ++dev.frame_serial;
     bool invalid = false;
     if (abort_requested(abort_callback, invalid))
@@ -975,7 +1010,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     const long long grain_off = grain ? take(5 * plane) : -1;
     long long post_off = -1;
     if (post) post_off = spatial_dir ? dir_off + 9 * plane : (q_off >= 0 ? q_off : take(3 * plane));
-    const long long lens_off = (post && lens_active(prepared.scanner_post)) ? take(plane) : -1;
+    const long long lens_off =
+        (post && (lens_active(prepared.scanner_post) || unsharp_separate(prepared.scanner_post))) ? take(plane) : -1;
     const long long pad_off = padded_floats ? take(padded_floats) : -1;
     const long long hb_off =
         boost ? take(static_cast<long long>((sizeof(HbState) + sizeof(float) - 1) / sizeof(float))) : -1;
@@ -983,12 +1019,27 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     const long long ae_off =
         auto_exposure ? take(static_cast<long long>((sizeof(AeScratch) + sizeof(float) - 1) / sizeof(float))) : -1;
     float* ws = nullptr;
+    // Every return after the slot is acquired records the slot's done event
+    // on this frame's stream, so that the slot's next user waits for the
+    // launches this frame already queued into it.
+    struct SlotUse {
+        DeviceState& dev;
+        hipStream_t stream;
+        bool held = false;
+        hipError_t release() {
+            if (!held) return hipSuccess;
+            held = false;
+            return dev.release_workspace(stream);
+        }
+        ~SlotUse() { (void)release(); }
+    } slot_use{dev, stream};
     if (ws_floats) {
         e = dev.ensure_workspace(static_cast<size_t>(ws_floats) * sizeof(float));
         if (e != hipSuccess) return outcome_hip(e, "workspace", error);
         ws = static_cast<float*>(dev.workspace());
         e = dev.acquire_workspace(stream);
         if (e != hipSuccess) return outcome_hip(e, "workspace order", error);
+        slot_use.held = true;
     }
     if (post) {
         pp.linear_out = 1;
@@ -1188,15 +1239,14 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         po.co = pp.co;
         po.planes = pp.linear_planes;
         const FjScannerPostEffects& sp = prepared.scanner_post;
+        const long long np = static_cast<long long>(pp.io.width) * pp.io.height;
         if (lens_active(sp)) {
             std::vector<float> lw = gaussian_weights(sp.lens_blur_radius, sp.lens_blur_sigma_px);
-            if (lw.empty())
-                return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput lens blur radius/sigma", error);
+            if (lw.empty()) return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "gaussian descriptor is invalid", error);
             const void* lptr = nullptr;
             e = dev.table(hash_bytes(lw.data(), lw.size() * 4, 0x4C42ull), lw.data(), lw.size() * 4, &lptr);
             if (e != hipSuccess) return outcome_hip(e, "lens blur weights", error);
             mark("post_lens_blur");
-            const long long np = static_cast<long long>(pp.io.width) * pp.io.height;
             for (int c = 0; c < 3 && e == hipSuccess; ++c) {
                 GrainBlurArgs h{};
                 h.in = pp.linear_planes + c * np;
@@ -1218,24 +1268,45 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         static const float kIdentityTap = 1.0f;
         std::vector<float> w = unsharp_active(sp) ? gaussian_weights(sp.unsharp_radius, sp.unsharp_sigma_px)
                                                   : std::vector<float>{kIdentityTap};
-        if (w.empty() || sp.unsharp_radius > 16)
-            return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput unsharp radius/sigma", error);
+        if (w.empty()) return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "gaussian descriptor is invalid", error);
         const void* wptr = nullptr;
         e = dev.table(hash_bytes(w.data(), w.size() * 4, 0x5057ull), w.data(), w.size() * 4, &wptr);
         if (e != hipSuccess) return outcome_hip(e, "unsharp weights", error);
+        const bool fused_unsharp = unsharp_active(sp) && !unsharp_separate(sp);
+        if (unsharp_separate(sp)) {
+            // x + a (x - G * x) per plane: horizontal blur into the scratch
+            // plane, then the vertical blur with the unsharp combine in place.
+            mark("post_unsharp_wide");
+            for (int c = 0; c < 3 && e == hipSuccess; ++c) {
+                GrainBlurArgs h{};
+                h.in = pp.linear_planes + c * np;
+                h.out = ws + lens_off;
+                h.weights = static_cast<const float*>(wptr);
+                h.radius = sp.unsharp_radius;
+                h.width = pp.io.width;
+                h.height = pp.io.height;
+                e = launch_grain_blur_h(h, stream);
+                if (e != hipSuccess) break;
+                GrainBlurArgs v = h;
+                v.in = ws + lens_off;
+                v.out = pp.linear_planes + c * np;
+                v.unsharp = 1;
+                v.weight = sp.unsharp_amount;
+                e = launch_grain_blur_v(v, stream);
+            }
+            if (e != hipSuccess) return outcome_hip(e, "unsharp mask", error);
+        }
         po.weights = static_cast<const float*>(wptr);
-        po.radius = unsharp_active(sp) ? sp.unsharp_radius : 0;
-        po.amount = unsharp_active(sp) ? sp.unsharp_amount : 0.0f;
+        po.radius = fused_unsharp ? sp.unsharp_radius : 0;
+        po.amount = fused_unsharp ? sp.unsharp_amount : 0.0f;
         mark("post_unsharp_encode");
         e = launch_post(po, stream);
         if (e != hipSuccess) return outcome_hip(e, "post stage", error);
     }
     mark(nullptr);
     dev.stage_count = trace ? std::min(stage - 1, static_cast<int>(DeviceState::kMaxStages)) : dev.stage_count;
-    if (ws_floats) {
-        e = dev.release_workspace(stream);
-        if (e != hipSuccess) return outcome_hip(e, "workspace order", error);
-    }
+    e = slot_use.release();
+    if (e != hipSuccess) return outcome_hip(e, "workspace order", error);
     e = hipMemcpyAsync(dev.error_host + slot, pp.error_word, sizeof(unsigned int), hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) return outcome_hip(e, "error readback", error);
     e = hipEventRecord(dev.error_events[slot], stream);

@@ -1747,10 +1747,19 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
         if (rc == 0) run_rows(&f, rows_enlarged, threads);
     }
     const FjScannerPostEffects* sp = &p->scanner_post;
-    if (rc == 0 && sp->hash != 0 && sp->lens_blur_sigma_px > 0.0f && sp->lens_blur_radius > 0) {
+    /* A bound post descriptor prepares each Gaussian whose sigma is positive
+     * and refuses a non-finite sigma or a radius outside 1..75
+     * (ProcessRoot.cpp:3121-3158, JuicerCudaResources.cpp:6436-6448). */
+    if (rc == 0 && sp->hash != 0 &&
+        ((sp->lens_blur_sigma_px > 0.0f &&
+          (!isfinite(sp->lens_blur_sigma_px) || sp->lens_blur_radius < 1 || sp->lens_blur_radius > 75)) ||
+         (sp->unsharp_sigma_px > 0.0f &&
+          (!isfinite(sp->unsharp_sigma_px) || sp->unsharp_radius < 1 || sp->unsharp_radius > 75))))
+        rc = 4;
+    if (rc == 0 && sp->hash != 0 && sp->lens_blur_sigma_px > 0.0f) {
         /* lens blur before the unsharp mask (JuicerCudaScanPipeline.cu:2795-2809) */
         float* w = (float*)malloc(sizeof(float) * (2 * (size_t)sp->lens_blur_radius + 1));
-        if (sp->lens_blur_radius > 4096 || fjo_gaussian_weights(sp->lens_blur_radius, sp->lens_blur_sigma_px, w)) {
+        if (fjo_gaussian_weights(sp->lens_blur_radius, sp->lens_blur_sigma_px, w)) {
             rc = 4;
         } else {
             double* pl3 = (double*)malloc(sizeof(double) * N);
@@ -1763,9 +1772,11 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
         }
         free(w);
     }
-    if (rc == 0 && sp->hash != 0 && sp->unsharp_radius > 0 && isfinite(sp->unsharp_amount) && sp->unsharp_amount > 0.0f) {
-        float w[8193];
-        if (sp->unsharp_radius > 4096 || fjo_gaussian_weights(sp->unsharp_radius, sp->unsharp_sigma_px, w)) {
+    if (rc == 0 && sp->hash != 0 && sp->unsharp_sigma_px > 0.0f && sp->unsharp_radius > 0 &&
+        isfinite(sp->unsharp_amount) && sp->unsharp_amount > 0.0f) {
+        /* JuicerCudaScanPipeline.cu:2811-2866 */
+        float w[151];
+        if (fjo_gaussian_weights(sp->unsharp_radius, sp->unsharp_sigma_px, w)) {
             rc = 4;
         } else {
             double* pl3 = (double*)malloc(sizeof(double) * N);

@@ -68,3 +68,14 @@ def test_bypass_when_protection_exceeds_the_maximum():
 def test_peak_overflow_refused():
     _, out, text = render_gpu(boosted("PrintPreflash", 0.0, gain=1e38), bright_frame(34))
     assert out.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT and "HighlightBoostPeakOverflow" in text
+
+
+def test_sharpness_at_the_recipe_bound():
+    """28 = 28^(1 - range) at range 0, the recipe's largest sharpness
+    (rust optics/highlight_boost.rs:84)."""
+    check(boosted("PrintPreflash", 0.05, sharpness=28.0, gain=4.0), bright_frame(35))
+
+
+def test_sharpness_above_the_bound_refused():
+    _, out, text = render_gpu(boosted("PrintPreflash", 0.05, sharpness=28.5), bright_frame(36))
+    assert out.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT and "highlight_boost" in text

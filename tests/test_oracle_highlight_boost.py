@@ -25,7 +25,7 @@ def planes(values):
     return [v, v * 0.5, v * 0.25]
 
 
-@pytest.mark.parametrize("k", [0.0, 0.3, 2.0, 9.0])
+@pytest.mark.parametrize("k", [0.0, 0.3, 2.0, 9.0, 28.0])  # 28: the recipe's largest sharpness
 def test_endpoints_monotone_and_slope(k):
     p, g = 0.8, 3.0
     s = np.linspace(0.0, 4.0, 4001)
