@@ -14,6 +14,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
 #include <list>
 #include <map>
@@ -36,6 +37,93 @@ namespace fjx {
 
 std::atomic<FjCuda*> g_owner{nullptr};
 thread_local bool t_in_call = false;
+
+namespace {
+// Registration: publishing and clearing the owner, and counting entrants.
+std::mutex g_registry;
+std::condition_variable g_registry_cv;
+}  // namespace
+
+OwnerEntry::OwnerEntry(FjCuda* cuda) {
+    if (!cuda) return;
+    std::lock_guard<std::mutex> lock(g_registry);
+    if (g_owner.load(std::memory_order_acquire) != cuda) return;
+    ++cuda->entrants;
+    cuda_ = cuda;
+}
+
+OwnerEntry::~OwnerEntry() {
+    if (!cuda_) return;
+    std::lock_guard<std::mutex> lock(g_registry);
+    if (--cuda_->entrants == 0) g_registry_cv.notify_all();
+}
+
+NativeCall::NativeCall(FjCuda* cuda) : entry(cuda) {
+    if (!cuda || !guard.ok) {
+        status = make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        text = cuda ? "runtime re-entered from its own call" : "invalid CUDA call";
+        return;
+    }
+    if (!entry.ok()) {
+        status = make_status(FJ_STATUS_PREPARATION_FAILURE);
+        text = "CUDA call requires the registered owner (unknown runtime handle)";
+        return;
+    }
+    lock = std::unique_lock<std::mutex>(cuda->mu);
+    if (!cuda->accepting()) {
+        lock.unlock();
+        status = make_status(FJ_STATUS_PREPARATION_FAILURE);
+        text = "CUDA owner admission is closed (runtime is shut down)";
+        return;
+    }
+    status = make_status(FJ_STATUS_SUCCESS);
+}
+
+FjStatus owner_shutdown(FjCuda* cuda) {
+    Lifecycle state = cuda->lifecycle.load(std::memory_order_acquire);
+    for (;;) {
+        if (state == Lifecycle::Closed) return make_status(FJ_STATUS_SUCCESS);
+        if (state == Lifecycle::Blocked || state == Lifecycle::Retained) return cuda->terminal;
+        if (state == Lifecycle::Closing) {
+            cuda->lifecycle.wait(state, std::memory_order_acquire);
+        } else if (cuda->lifecycle.compare_exchange_weak(state, Lifecycle::Closing, std::memory_order_acq_rel)) {
+            break;
+        }
+        state = cuda->lifecycle.load(std::memory_order_acquire);
+    }
+    auto publish = [&](Lifecycle next, FjStatus st, const std::string& text) {
+        if (next != Lifecycle::Closed) {
+            cuda->terminal = st;
+            cuda->terminal_text = text;
+        }
+        cuda->lifecycle.store(next, std::memory_order_release);
+        cuda->lifecycle.notify_all();
+        return st;
+    };
+    try {
+        // Closing refuses new admissions; the call in flight (if any) ends first.
+        std::lock_guard<std::mutex> lock(cuda->mu);
+        hipError_t first = hipSuccess;
+        for (auto& kv : cuda->devices) {
+            if (!kv.second) continue;
+            const hipError_t e = kv.second->release();
+            if (e != hipSuccess && first == hipSuccess) first = e;
+        }
+        cuda->devices.clear();
+        if (first != hipSuccess) {
+            (void)hipGetLastError();
+            return publish(Lifecycle::Blocked, make_status(FJ_STATUS_CUDA_FAILURE, FJ_API_CUDA_RUNTIME, first),
+                           std::string("HIP failure in shutdown: ") + hipGetErrorString(first) +
+                               "; device state released with errors");
+        }
+        return publish(Lifecycle::Closed, make_status(FJ_STATUS_SUCCESS), "");
+    } catch (const std::bad_alloc&) {
+        return publish(Lifecycle::Blocked, make_status(FJ_STATUS_ALLOCATION_FAILURE), "CUDA shutdown allocation failed");
+    } catch (...) {
+        return publish(Lifecycle::Blocked, make_status(FJ_STATUS_INTERNAL_FAILURE),
+                       "CUDA shutdown failed with unknown exception");
+    }
+}
 
 FjStatus make_status(uint32_t category, uint32_t api, int32_t code) {
     FjStatus s;
@@ -103,8 +191,8 @@ FjRenderOutcome render_locked(FjCuda* cuda, const FjCudaContext* context, const 
                               const FjSubmission* submission, const FjPreparedHostData* prepared,
                               FjAbortCallback abort_callback, FjErrorBuffer* error) {
     FjRenderOutcome outcome{};
-    if (!cuda->open) {
-        outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+    if (!cuda->accepting()) {
+        outcome.status = fail(error, FJ_STATUS_PREPARATION_FAILURE, "CUDA owner admission is closed (runtime is shut down)");
         return outcome;
     }
     if (!context || !frame || !submission || !prepared) {
@@ -153,41 +241,123 @@ bool retire_device_locked(FjCuda* cuda, int32_t device_id) {
 
 }  // namespace fjx
 
+namespace {
+
+// invalid_render_arguments (native/juicer_cuda_api.cpp:141-148).
+bool invalid_render_arguments(const FjCudaContext* context, const FjFrame* frame, const FjSubmission* submission,
+                              const FjPreparedHostData* prepared) {
+    return !context || !frame || !submission || !prepared || context->device_id < 0 || context->context == 0 ||
+           submission->instance_token == 0 || submission->submission_id == 0 || !std::isfinite(frame->time_frames) ||
+           frame->time_frames < -0x1p63 || frame->time_frames >= 0x1p63 || !std::isfinite(frame->frame_rate) ||
+           frame->frame_rate < 0 || !std::isfinite(frame->pixel_size_um) || frame->pixel_size_um < 0;
+}
+
+// A terminal call that cannot complete keeps the owner registered and
+// blocked (terminal_admission_failure, native/juicer_cuda_api.cpp:163-177).
+FjStatus terminal_failure(FjCuda* cuda, FjStatus result, const std::string& text, bool consuming, FjErrorBuffer* error) {
+    if (cuda != g_owner.load(std::memory_order_acquire)) return fail(error, result.category, text);
+    Lifecycle state = cuda->lifecycle.load(std::memory_order_acquire);
+    for (;;) {
+        if (state == Lifecycle::Blocked || state == Lifecycle::Retained) break;
+        if (state == Lifecycle::Closing) {
+            cuda->lifecycle.wait(state, std::memory_order_acquire);
+            state = cuda->lifecycle.load(std::memory_order_acquire);
+            continue;
+        }
+        if (cuda->lifecycle.compare_exchange_weak(state, Lifecycle::Closing, std::memory_order_acq_rel)) {
+            cuda->terminal = result;
+            cuda->terminal_text = text;
+            cuda->lifecycle.store(Lifecycle::Blocked, std::memory_order_release);
+            cuda->lifecycle.notify_all();
+            break;
+        }
+    }
+    if (consuming) cuda->lifecycle.store(Lifecycle::Retained, std::memory_order_release);
+    write_error(error, cuda->terminal_text);
+    return cuda->terminal;
+}
+
+DeviceState* device_state(FjCuda* cuda, int device_id, FjStatus& st, FjErrorBuffer* error) {
+    int count = 0;
+    hipError_t he = hipGetDeviceCount(&count);
+    if (he != hipSuccess) {
+        st = hip_fail(error, he, "hipGetDeviceCount");
+        return nullptr;
+    }
+    if (device_id < 0 || device_id >= count) {
+        st = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "device id out of range");
+        return nullptr;
+    }
+    he = hipSetDevice(device_id);
+    if (he != hipSuccess) {
+        st = hip_fail(error, he, "hipSetDevice");
+        return nullptr;
+    }
+    auto& slot = cuda->devices[device_id];
+    if (!slot) {
+        slot = std::make_unique<DeviceState>();
+        he = slot->init(device_id);
+        if (he != hipSuccess) {
+            slot.reset();
+            st = hip_fail(error, he, "device state init");
+            return nullptr;
+        }
+    }
+    st = make_status(FJ_STATUS_SUCCESS);
+    return slot.get();
+}
+
+}  // namespace
+
 extern "C" {
 
 FJ_EXPORT FjStatus fj_cuda_create(FjCuda** out_cuda, FjErrorBuffer* error) {
+    if (out_cuda) *out_cuda = nullptr;
+    if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
     try {
-        if (!out_cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "out_cuda is null");
-        *out_cuda = nullptr;
-        auto owner = std::make_unique<FjCuda>();
-        FjCuda* expected = nullptr;
-        if (!g_owner.compare_exchange_strong(expected, owner.get()))
-            return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "a film render runtime is already active");
-        *out_cuda = owner.release();
+        if (t_in_call || !out_cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "invalid CUDA owner creation input");
+        auto candidate = std::make_unique<FjCuda>();
+        bool published = false;
+        {
+            std::lock_guard<std::mutex> lock(g_registry);
+            if (!g_owner.load(std::memory_order_acquire)) {
+                *out_cuda = candidate.release();
+                g_owner.store(*out_cuda, std::memory_order_release);
+                published = true;
+            }
+        }
+        if (!published) return fail(error, FJ_STATUS_PREPARATION_FAILURE, "a film render runtime is already active");
         write_error(error, "");
         return make_status(FJ_STATUS_SUCCESS);
+    } catch (const std::bad_alloc&) {
+        return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "CUDA owner allocation failed");
     } catch (...) {
         return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fj_cuda_create");
     }
 }
 
 FJ_EXPORT FjStatus fj_cuda_check_terminal(FjCuda* cuda, FjErrorBuffer* error) {
-    if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
-    if (t_in_call) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
-    std::lock_guard<std::mutex> lock(cuda->mu);
-    if (!cuda->open) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
-    write_error(error, "");
-    return make_status(FJ_STATUS_SUCCESS);
+    // Registration and re-entry precheck only: no drain, no lifetime exclusion.
+    if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+    try {
+        if (!cuda || t_in_call)
+            return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "invalid or reentrant terminal precheck (runtime re-entered from its own call)");
+        if (cuda != g_owner.load(std::memory_order_acquire))
+            return fail(error, FJ_STATUS_PREPARATION_FAILURE, "terminal precheck requires the registered owner (unknown runtime handle)");
+        write_error(error, "");
+        return make_status(FJ_STATUS_SUCCESS);
+    } catch (...) {
+        return make_status(FJ_STATUS_INTERNAL_FAILURE);
+    }
 }
 
 FJ_EXPORT FjStatus fj_cuda_inspect(FjCuda* cuda, const FjFrame* frame, FjCudaContext* out_context, FjErrorBuffer* error) {
     try {
-        if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
-        CallGuard guard;
-        if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
-        std::lock_guard<std::mutex> lock(cuda->mu);
-        if (!cuda->open) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
-        if (!out_context) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "out_context is null");
+        if (!cuda || !frame || !out_context || bad_error_buffer(error))
+            return fail(bad_error_buffer(error) ? nullptr : error, FJ_STATUS_UNSUPPORTED_INPUT,
+                        "invalid CUDA inspection arguments");
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
         FjCudaContext ctx{};
         const FjStatus s = inspect_frame(frame, &ctx, error);
         if (s.category == FJ_STATUS_SUCCESS) {
@@ -195,6 +365,8 @@ FJ_EXPORT FjStatus fj_cuda_inspect(FjCuda* cuda, const FjFrame* frame, FjCudaCon
             write_error(error, "");
         }
         return s;
+    } catch (const std::bad_alloc&) {
+        return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "CUDA inspection allocation failed");
     } catch (...) {
         return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fj_cuda_inspect");
     }
@@ -205,16 +377,16 @@ FJ_EXPORT FjRenderOutcome fj_cuda_render(FjCuda* cuda, const FjCudaContext* cont
                                FjAbortCallback abort_callback, FjErrorBuffer* error) {
     FjRenderOutcome outcome{};
     try {
-        if (!cuda || g_owner.load() != cuda) {
-            outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
+        if (!cuda || invalid_render_arguments(context, frame, submission, prepared) || bad_error_buffer(error)) {
+            outcome.status = fail(bad_error_buffer(error) ? nullptr : error, FJ_STATUS_UNSUPPORTED_INPUT,
+                                  "invalid CUDA render arguments (null render argument)");
             return outcome;
         }
-        CallGuard guard;
-        if (!guard.ok) {
-            outcome.status = fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
+        NativeCall call(cuda);
+        if (!call.ok()) {
+            outcome.status = call.refuse(error);
             return outcome;
         }
-        std::lock_guard<std::mutex> lock(cuda->mu);
         return render_locked(cuda, context, frame, submission, prepared, abort_callback, error);
     } catch (const std::bad_alloc&) {
         outcome.status = fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
@@ -226,83 +398,107 @@ FJ_EXPORT FjRenderOutcome fj_cuda_render(FjCuda* cuda, const FjCudaContext* cont
 }
 
 FJ_EXPORT FjStatus fj_cuda_retire_instance(FjCuda* cuda, uint64_t instance_token, FjErrorBuffer* error) {
-    if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
-    if (instance_token == 0) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "instance token is zero");
-    CallGuard guard;
-    if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
-    std::lock_guard<std::mutex> lock(cuda->mu);
-    // Static grain state is keyed by table content, not by instance: nothing
-    // instance-owned remains after a render returns.
-    write_error(error, "");
-    return make_status(FJ_STATUS_SUCCESS);
+    try {
+        if (!cuda || instance_token == 0 || bad_error_buffer(error))
+            return fail(bad_error_buffer(error) ? nullptr : error, FJ_STATUS_UNSUPPORTED_INPUT,
+                        "invalid CUDA instance retirement arguments (instance token is zero)");
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
+        // Static grain state is keyed by table content, not by instance: nothing
+        // instance-owned remains after a render returns.
+        write_error(error, "");
+        return make_status(FJ_STATUS_SUCCESS);
+    } catch (const std::bad_alloc&) {
+        return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "CUDA instance retirement allocation failed");
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "CUDA instance retirement failed with unknown exception");
+    }
 }
 
 FJ_EXPORT FjStatus fj_cuda_shutdown(FjCuda* cuda, FjErrorBuffer* error) {
-    if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
-    CallGuard guard;
-    if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
-    std::lock_guard<std::mutex> lock(cuda->mu);
-    FjStatus result = make_status(FJ_STATUS_SUCCESS);
-    for (auto& kv : cuda->devices) {
-        if (!kv.second) continue;
-        const hipError_t e = kv.second->release();
-        if (e != hipSuccess && result.category == FJ_STATUS_SUCCESS) result = hip_fail(error, e, "shutdown");
+    if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+    if (!cuda || t_in_call)
+        return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "invalid or reentrant CUDA shutdown (runtime re-entered from its own call)");
+    try {
+        OwnerEntry entry(cuda);
+        if (!entry.ok()) return fail(error, FJ_STATUS_PREPARATION_FAILURE, "CUDA shutdown requires the registered owner");
+        const FjStatus result = owner_shutdown(cuda);
+        write_error(error, result.category == FJ_STATUS_SUCCESS ? std::string() : cuda->terminal_text);
+        return result;
+    } catch (const std::bad_alloc&) {
+        return terminal_failure(cuda, make_status(FJ_STATUS_ALLOCATION_FAILURE), "CUDA shutdown admission allocation failed",
+                                false, error);
+    } catch (...) {
+        return terminal_failure(cuda, make_status(FJ_STATUS_INTERNAL_FAILURE), "CUDA shutdown admission failed", false,
+                                error);
     }
-    cuda->devices.clear();
-    cuda->open = false;
-    if (result.category == FJ_STATUS_SUCCESS) write_error(error, "");
-    return result;
 }
 
 FJ_EXPORT FjStatus fj_cuda_destroy(FjCuda* cuda, FjErrorBuffer* error) {
-    if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
-    if (t_in_call) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
-    FjStatus s = make_status(FJ_STATUS_SUCCESS);
-    {
-        std::lock_guard<std::mutex> lock(cuda->mu);
-        for (auto& kv : cuda->devices) {
-            if (kv.second) {
-                const hipError_t e = kv.second->release();
-                if (e != hipSuccess && s.category == FJ_STATUS_SUCCESS) s = hip_fail(error, e, "destroy");
+    const bool diagnostic_valid = !bad_error_buffer(error);
+    FjErrorBuffer* out = diagnostic_valid ? error : nullptr;
+    if (!cuda) return fail(out, FJ_STATUS_UNSUPPORTED_INPUT, "CUDA destroy requires an owning handle");
+    if (cuda != g_owner.load(std::memory_order_acquire))
+        return fail(out, FJ_STATUS_PREPARATION_FAILURE, "CUDA destroy requires the registered owner (unknown runtime handle)");
+    if (t_in_call)
+        return terminal_failure(cuda, make_status(FJ_STATUS_UNSUPPORTED_INPUT),
+                                "reentrant CUDA destroy retained the owner (runtime re-entered from its own call)", true, out);
+    FjStatus result = make_status(FJ_STATUS_SUCCESS);
+    bool closed = false;
+    try {
+        std::unique_lock<std::mutex> lock(g_registry);
+        if (cuda != g_owner.load(std::memory_order_acquire))
+            return fail(out, FJ_STATUS_PREPARATION_FAILURE, "CUDA destroy requires the registered owner (unknown runtime handle)");
+        ++cuda->entrants;  // this call, until it decides the owner's fate
+        lock.unlock();
+        // Shutdown takes the call mutex; entrants not yet past admission then
+        // see a closed owner.
+        result = owner_shutdown(cuda);
+        lock.lock();
+        if (result.category == FJ_STATUS_SUCCESS && g_owner.load(std::memory_order_acquire) == cuda) {
+            g_owner.store(nullptr, std::memory_order_release);
+            // Every other entrant that passed the registration check leaves
+            // before the owner's memory goes.
+            g_registry_cv.wait(lock, [&] { return cuda->entrants == 1; });
+            cuda->entrants = 0;
+            closed = true;
+        } else {
+            if (--cuda->entrants == 0) g_registry_cv.notify_all();
+            if (result.category == FJ_STATUS_SUCCESS) {
+                // A concurrent destroy consumed the owner first.
+                return fail(out, FJ_STATUS_PREPARATION_FAILURE, "CUDA destroy requires the registered owner");
             }
+            cuda->lifecycle.store(Lifecycle::Retained, std::memory_order_release);
+            write_error(out, cuda->terminal_text);
         }
-        cuda->devices.clear();
-        cuda->open = false;
+    } catch (const std::bad_alloc&) {
+        result = terminal_failure(cuda, make_status(FJ_STATUS_ALLOCATION_FAILURE), "CUDA destroy admission allocation failed",
+                                  true, out);
+    } catch (...) {
+        result = terminal_failure(cuda, make_status(FJ_STATUS_INTERNAL_FAILURE), "CUDA destroy admission failed", true, out);
     }
-    FjCuda* expected = cuda;
-    g_owner.compare_exchange_strong(expected, nullptr);
-    delete cuda;
-    if (s.category == FJ_STATUS_SUCCESS) write_error(error, "");
-    return s;
+    if (closed) {
+        delete cuda;
+        if (!diagnostic_valid) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        write_error(out, "");
+        return make_status(FJ_STATUS_SUCCESS);
+    }
+    return result;
 }
 
 FJ_EXPORT FjStatus fjx_grain_apply(FjCuda* cuda, int32_t device_id, const FjPreparedHostData* prepared,
                                    float* density_c, float* density_m, float* density_y, int32_t width, int32_t height,
                                    uint64_t stream, FjErrorBuffer* error) {
     try {
-        if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
-        CallGuard guard;
-        if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
-        std::lock_guard<std::mutex> lock(cuda->mu);
-        if (!cuda->open) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+        if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
         if (!prepared) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null prepared record");
-        int count = 0;
-        hipError_t he = hipGetDeviceCount(&count);
-        if (he != hipSuccess) return hip_fail(error, he, "hipGetDeviceCount");
-        if (device_id < 0 || device_id >= count) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "device id out of range");
-        he = hipSetDevice(device_id);
-        if (he != hipSuccess) return hip_fail(error, he, "hipSetDevice");
-        auto& slot = cuda->devices[device_id];
-        if (!slot) {
-            slot = std::make_unique<DeviceState>();
-            he = slot->init(device_id);
-            if (he != hipSuccess) {
-                slot.reset();
-                return hip_fail(error, he, "device state init");
-            }
-        }
+        FjStatus st{};
+        DeviceState* dev = device_state(cuda, device_id, st, error);
+        if (!dev) return st;
         float* const planes[3] = {density_c, density_m, density_y};
-        return grain_apply_planes(*slot, *prepared, planes, width, height, reinterpret_cast<hipStream_t>(stream), error);
+        return grain_apply_planes(*dev, *prepared, planes, width, height, reinterpret_cast<hipStream_t>(stream), error);
     } catch (const std::bad_alloc&) {
         return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
     } catch (...) {
@@ -314,29 +510,15 @@ FJ_EXPORT FjStatus fjx_diffusion_apply(FjCuda* cuda, int32_t device_id, const Fj
                                        float* green, float* blue, int32_t width, int32_t height, uint64_t stream,
                                        FjErrorBuffer* error) {
     try {
-        if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
-        CallGuard guard;
-        if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
-        std::lock_guard<std::mutex> lock(cuda->mu);
-        if (!cuda->open) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+        if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
         if (!stage) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null diffusion stage");
-        int count = 0;
-        hipError_t he = hipGetDeviceCount(&count);
-        if (he != hipSuccess) return hip_fail(error, he, "hipGetDeviceCount");
-        if (device_id < 0 || device_id >= count) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "device id out of range");
-        he = hipSetDevice(device_id);
-        if (he != hipSuccess) return hip_fail(error, he, "hipSetDevice");
-        auto& slot = cuda->devices[device_id];
-        if (!slot) {
-            slot = std::make_unique<DeviceState>();
-            he = slot->init(device_id);
-            if (he != hipSuccess) {
-                slot.reset();
-                return hip_fail(error, he, "device state init");
-            }
-        }
+        FjStatus st{};
+        DeviceState* dev = device_state(cuda, device_id, st, error);
+        if (!dev) return st;
         float* const planes[3] = {red, green, blue};
-        return diffusion_apply_planes(*slot, *stage, planes, width, height, reinterpret_cast<hipStream_t>(stream), error);
+        return diffusion_apply_planes(*dev, *stage, planes, width, height, reinterpret_cast<hipStream_t>(stream), error);
     } catch (const std::bad_alloc&) {
         return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
     } catch (...) {
@@ -349,11 +531,9 @@ FJ_EXPORT FjStatus fjx_render_batch(FjCuda* cuda, const FjCudaContext* context, 
                                     int32_t streams, FjAbortCallback abort_callback, FjRenderOutcome* outcomes,
                                     FjErrorBuffer* error) {
     try {
-        if (!cuda || g_owner.load() != cuda) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
-        CallGuard guard;
-        if (!guard.ok) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
-        std::lock_guard<std::mutex> lock(cuda->mu);
-        if (!cuda->open) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+        if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
         if (!context || count < 0 || (count > 0 && (!frames || !submissions || !prepared || !outcomes)))
             return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null batch argument");
         if (count == 0) {
@@ -373,22 +553,14 @@ FJ_EXPORT FjStatus fjx_render_batch(FjCuda* cuda, const FjCudaContext* context, 
             if (hs != has_stream || (hs && frames[i].stream != frames[0].stream))
                 return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "batch frames name different streams");
         }
-        hipError_t he = hipSetDevice(context->device_id);
-        if (he != hipSuccess) return hip_fail(error, he, "hipSetDevice");
-        auto& slot = cuda->devices[context->device_id];
-        if (!slot) {
-            slot = std::make_unique<DeviceState>();
-            he = slot->init(context->device_id);
-            if (he != hipSuccess) {
-                slot.reset();
-                return hip_fail(error, he, "device state init");
-            }
-        }
+        FjStatus dst{};
+        DeviceState* dev = device_state(cuda, context->device_id, dst, error);
+        if (!dev) return dst;
         hipStream_t caller = has_stream ? reinterpret_cast<hipStream_t>(frames[0].stream) : nullptr;
         const FjStatus st =
-            render_batch(*slot, count, frames, submissions, prepared, streams, caller, abort_callback, outcomes, error);
+            render_batch(*dev, count, frames, submissions, prepared, streams, caller, abort_callback, outcomes, error);
         if (!has_stream) {
-            he = hipStreamSynchronize(caller);
+            const hipError_t he = hipStreamSynchronize(caller);
             if (he != hipSuccess && st.category == FJ_STATUS_SUCCESS) return hip_fail(error, he, "batch synchronize");
         }
         return st;
@@ -401,8 +573,9 @@ FJ_EXPORT FjStatus fjx_render_batch(FjCuda* cuda, const FjCudaContext* context, 
 
 FJ_EXPORT int32_t fjx_stage_timings(FjCuda* cuda, int32_t device_id, FjxStageTiming* out, int32_t capacity) {
     try {
-        if (!cuda || g_owner.load() != cuda || capacity < 0 || (capacity > 0 && !out)) return -1;
-        std::lock_guard<std::mutex> lock(cuda->mu);
+        if (capacity < 0 || (capacity > 0 && !out)) return -1;
+        NativeCall call(cuda);
+        if (!call.ok()) return -1;
         auto it = cuda->devices.find(device_id);
         if (it == cuda->devices.end() || !it->second) return -1;
         DeviceState& dev = *it->second;

@@ -102,25 +102,24 @@ private:
         return fail(err, category, text);
     }
 
-    bool take_admission() {
+    // Admission holds a NativeCall (re-entry guard, owner entry, call mutex)
+    // until release or the scope's end.
+    bool take_admission(FjStatus* refused = nullptr, std::string* text = nullptr) {
         if (admitted_) return true;
-        if (t_in_call || g_owner.load() != owner_) return false;
-        t_in_call = true;
-        guard_set_ = true;
-        lock_ = std::unique_lock<std::mutex>(owner_->mu);
+        call_ = std::make_unique<NativeCall>(owner_);
+        if (!call_->ok()) {
+            if (refused) *refused = call_->status;
+            if (text) *text = call_->text;
+            call_.reset();
+            return false;
+        }
         admitted_ = true;
         return true;
     }
 
     void drop_admission() {
-        if (admitted_) {
-            lock_.unlock();
-            admitted_ = false;
-        }
-        if (guard_set_) {
-            t_in_call = false;
-            guard_set_ = false;
-        }
+        call_.reset();
+        admitted_ = false;
     }
 
     static FjStatus precheck(void* user, FjErrorBuffer* err) {
@@ -128,12 +127,15 @@ private:
         if (!s || !s->owner_ || s->completed_ || s->admission_attempted_)
             return fail(err, FJ_STATUS_UNSUPPORTED_INPUT, kInvalidScope);
         try {
-            if (g_owner.load() != s->owner_) return s->error(err, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
+            // check_native_call_admission (native/juicer_cuda_api.cpp:648-655),
+            // plus the lifecycle the admission itself checks.
             if (t_in_call) return s->error(err, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
-            {
-                std::lock_guard<std::mutex> lock(s->owner_->mu);
-                if (!s->owner_->open) return s->error(err, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
-            }
+            OwnerEntry entry(s->owner_);
+            if (!entry.ok())
+                return s->error(err, FJ_STATUS_PREPARATION_FAILURE,
+                                "CUDA call requires the registered owner (unknown runtime handle)");
+            if (!s->owner_->accepting())
+                return s->error(err, FJ_STATUS_PREPARATION_FAILURE, "CUDA owner admission is closed (runtime is shut down)");
             s->diagnostic_.clear();
             write_error(err, "");
             return make_status(FJ_STATUS_SUCCESS);
@@ -149,13 +151,9 @@ private:
             return s->error(err, FJ_STATUS_UNSUPPORTED_INPUT, kInvalidScope);
         s->admission_attempted_ = true;
         try {
-            if (g_owner.load() != s->owner_) return s->error(err, FJ_STATUS_UNSUPPORTED_INPUT, "unknown runtime handle");
-            if (t_in_call) return s->error(err, FJ_STATUS_UNSUPPORTED_INPUT, "runtime re-entered from its own call");
-            s->take_admission();
-            if (!s->owner_->open) {
-                s->drop_admission();
-                return s->error(err, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
-            }
+            FjStatus refused{};
+            std::string text;
+            if (!s->take_admission(&refused, &text)) return s->error(err, refused.category, text);
             s->diagnostic_.clear();
             write_error(err, "");
             return make_status(FJ_STATUS_SUCCESS);
@@ -182,7 +180,8 @@ private:
             s->diagnostic_.clear();
             s->has_context_ = false;
             if (!context) return s->error(err, FJ_STATUS_UNSUPPORTED_INPUT, "out_context is null");
-            if (!s->owner_->open) return s->error(err, FJ_STATUS_UNSUPPORTED_INPUT, "runtime is shut down");
+            if (!s->owner_->accepting())
+                return s->error(err, FJ_STATUS_PREPARATION_FAILURE, "CUDA owner admission is closed (runtime is shut down)");
             char text[2048];
             FjErrorBuffer local{text, sizeof(text), 0};
             FjCudaContext ctx{};
@@ -318,8 +317,7 @@ private:
     uint32_t route_;
     FjCudaCompletionContinuation completion_;
     void* user_;
-    std::unique_lock<std::mutex> lock_;
-    bool guard_set_ = false;
+    std::unique_ptr<NativeCall> call_;
     bool admitted_ = false;
     bool admission_attempted_ = false;
     bool released_ = false;

@@ -866,13 +866,21 @@ FjRenderOutcome outcome_diag(uint32_t category, const std::string& text, FjError
     return o;
 }
 
-bool abort_requested(const FjAbortCallback& cb, bool& invalid) {
-    invalid = false;
+// check_abort (JuicerCudaExecutor.cpp:2544-2559), polled before the frame's
+// preparation, after it, and after its launches are queued (:2816, :2871,
+// :3125). True when the frame ends here with `out`.
+bool aborted(const FjAbortCallback& cb, FjRenderOutcome& out, FjErrorBuffer* error) {
     if (!cb.query) return false;
-    const uint32_t r = cb.query(cb.user);
-    if (r == FJ_ABORT_CONTINUE) return false;
-    if (r != FJ_ABORT_REQUESTED) invalid = true;
-    return true;
+    switch (cb.query(cb.user)) {
+        case FJ_ABORT_CONTINUE:
+            return false;
+        case FJ_ABORT_REQUESTED:
+            out = outcome_diag(FJ_STATUS_CANCELLED, "CUDA render cancelled", error);
+            return true;
+        default:
+            out = outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "invalid CUDA abort query result", error);
+            return true;
+    }
 }
 
 }  // namespace
@@ -887,10 +895,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         !check_grain_frame(prepared, frame, d))
         return outcome_diag(d.category, d.text, error);
     ++dev.frame_serial;
-    bool invalid = false;
-    if (abort_requested(abort_callback, invalid))
-        return outcome_diag(invalid ? FJ_STATUS_INTERNAL_FAILURE : FJ_STATUS_CANCELLED,
-                            invalid ? "abort query returned an invalid value" : "render aborted", error);
+    FjRenderOutcome stopped{};
+    if (aborted(abort_callback, stopped, error)) return stopped;
 
     PixelParams pp{};
     bool ok = false;
@@ -1041,6 +1047,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         if (e != hipSuccess) return outcome_hip(e, "workspace order", error);
         slot_use.held = true;
     }
+    // Preparation done: tables resident, workspace ordered.
+    if (aborted(abort_callback, stopped, error)) return stopped;
     if (post) {
         pp.linear_out = 1;
         pp.linear_planes = ws + post_off;
@@ -1305,6 +1313,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     }
     mark(nullptr);
     dev.stage_count = trace ? std::min(stage - 1, static_cast<int>(DeviceState::kMaxStages)) : dev.stage_count;
+    // Every launch is queued; a cancellation now still lets them run.
+    if (aborted(abort_callback, stopped, error)) return stopped;
     e = slot_use.release();
     if (e != hipSuccess) return outcome_hip(e, "workspace order", error);
     e = hipMemcpyAsync(dev.error_host + slot, pp.error_word, sizeof(unsigned int), hipMemcpyDeviceToHost, stream);

@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "native", "libfj_call_harness.so")
 
 PRECHECK, ADMIT, INSPECT, DESCRIPTORS, PREVIEW, RENDER, RELEASE, COMPLETE = 1, 2, 3, 4, 5, 6, 7, 8
-PUBLIC_INSPECT, TRACE, RENDER_OTHER_CONTEXT = 9, 10, 11
+PUBLIC_INSPECT, TRACE, RENDER_OTHER_CONTEXT, PUBLIC_DESTROY, PUBLIC_SHUTDOWN = 9, 10, 11, 12, 13
 MAX_OPS, TEXT = 16, 256
 
 

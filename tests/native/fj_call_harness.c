@@ -22,6 +22,8 @@ enum {
     FJH_PUBLIC_INSPECT = 9, /* fj_cuda_inspect from inside the scope (re-entry) */
     FJH_TRACE = 10,
     FJH_RENDER_OTHER_CONTEXT = 11,
+    FJH_PUBLIC_DESTROY = 12,  /* fj_cuda_destroy from inside the scope (re-entry) */
+    FJH_PUBLIC_SHUTDOWN = 13, /* fj_cuda_shutdown from inside the scope (re-entry) */
 };
 
 #define FJH_MAX_OPS 16
@@ -146,6 +148,8 @@ static FjCallCompletionResult fjh_continuation(void* user, FjCudaCallAccess a) {
                 r = fj_cuda_inspect(st->cuda, s->frame, &ctx, &err);
                 break;
             }
+            case FJH_PUBLIC_DESTROY: r = fj_cuda_destroy(st->cuda, &err); break;
+            case FJH_PUBLIC_SHUTDOWN: r = fj_cuda_shutdown(st->cuda, &err); break;
             case FJH_TRACE:
                 if (a.trace_submission) a.trace_submission(a.user, s->submission, 0);
                 break;

@@ -41,7 +41,7 @@ def test_protocol_in_order(owner):
 
 
 def test_admitted_scope_holds_the_reentry_guard(owner):
-    s = H.run(owner.handle, [H.ADMIT, H.PUBLIC_INSPECT, H.RELEASE, H.COMPLETE])
+    s = H.run(owner.handle, [H.ADMIT, H.PUBLIC_INSPECT, H.RELEASE, H.COMPLETE], frame=abi.FjFrame())
     assert s.op_status(1) == BAD and "re-entered" in s.op_text(1)
     assert s.op_status(3) == OK and s.completion_calls == 1
 
@@ -79,8 +79,9 @@ def test_scope_left_admitted_is_released_on_return(owner):
 def test_shut_down_owner_is_refused(owner):
     assert owner.shutdown().category == OK
     s = H.run(owner.handle, [H.PRECHECK, H.ADMIT])
-    assert s.op_status(0) == BAD and "shut down" in s.op_text(0)
-    assert s.op_status(1) == BAD and "shut down" in s.op_text(1)
+    closed = abi.FJ_STATUS_PREPARATION_FAILURE
+    assert s.op_status(0) == closed and "shut down" in s.op_text(0)
+    assert s.op_status(1) == closed and "shut down" in s.op_text(1)
 
 
 def test_failure_diagnostic_reaches_the_completion(owner):

@@ -145,10 +145,24 @@ def test_abort_callback():
     d = torch.zeros_like(s)
     f = runtime.frame_for(s.data_ptr(), d.data_ptr(), 16, 16, 4, stream=torch.cuda.current_stream().cuda_stream)
     for value, category in ((abi.FJ_ABORT_REQUESTED, abi.FJ_STATUS_CANCELLED),
-                            (7, abi.FJ_STATUS_INTERNAL_FAILURE)):
+                            (7, abi.FJ_STATUS_UNSUPPORTED_INPUT)):
         q = abi.FjAbortQuery(lambda user, v=value: v)
         out, _ = runtime.shared_renderer().render_raw(f, p.record, abort=abi.FjAbortCallback(q, None))
         assert out.status.category == category
+    # Polled at bounded checkpoints: before preparation, after it, and after
+    # the launches are queued (JuicerCudaExecutor.cpp:2816, :2871, :3125).
+    for stop_at in (1, 2, 3, 4):
+        polls = []
+
+        def query(user, polls=polls, stop_at=stop_at):
+            polls.append(1)
+            return abi.FJ_ABORT_REQUESTED if len(polls) == stop_at else abi.FJ_ABORT_CONTINUE
+
+        q = abi.FjAbortQuery(query)
+        out, _ = runtime.shared_renderer().render_raw(f, p.record, abort=abi.FjAbortCallback(q, None))
+        torch.cuda.synchronize()
+        want = abi.FJ_STATUS_CANCELLED if stop_at <= 3 else abi.FJ_STATUS_SUCCESS
+        assert out.status.category == want and len(polls) == min(stop_at, 3), (stop_at, len(polls))
 
 
 def test_synchronous_call_without_stream():

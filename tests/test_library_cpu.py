@@ -33,12 +33,17 @@ def test_single_owner_lifecycle():
     try:
         with pytest.raises(runtime.FjError) as e:
             runtime.Renderer.create()
-        assert e.value.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT
+        # A duplicate owner is a preparation failure (native/juicer_cuda_api.cpp:262-269).
+        assert e.value.status.category == abi.FJ_STATUS_PREPARATION_FAILURE
         err = runtime.ErrorBuffer()
         assert runtime.lib().fj_cuda_check_terminal(r.handle, err.ptr()).category == abi.FJ_STATUS_SUCCESS
         st = r.shutdown()
         assert st.category == abi.FJ_STATUS_SUCCESS
-        assert runtime.lib().fj_cuda_check_terminal(r.handle, err.ptr()).category != abi.FJ_STATUS_SUCCESS
+        assert r.shutdown().category == abi.FJ_STATUS_SUCCESS  # Closed: shutdown is idempotent
+        # The terminal precheck checks registration and re-entry only.
+        assert runtime.lib().fj_cuda_check_terminal(r.handle, err.ptr()).category == abi.FJ_STATUS_SUCCESS
+        st = runtime.lib().fj_cuda_retire_instance(r.handle, 5, err.ptr())
+        assert st.category == abi.FJ_STATUS_PREPARATION_FAILURE and "admission is closed" in err.text
     finally:
         assert r.destroy().category == abi.FJ_STATUS_SUCCESS
     r2 = runtime.Renderer.create()
@@ -68,4 +73,5 @@ def test_null_arguments_are_rejected():
 def test_unknown_handle_rejected():
     lib = runtime.lib()
     bogus = C.c_void_p(0x1234)
-    assert lib.fj_cuda_check_terminal(bogus, None).category == abi.FJ_STATUS_UNSUPPORTED_INPUT
+    assert lib.fj_cuda_check_terminal(bogus, None).category == abi.FJ_STATUS_PREPARATION_FAILURE
+    assert lib.fj_cuda_check_terminal(None, None).category == abi.FJ_STATUS_UNSUPPORTED_INPUT
