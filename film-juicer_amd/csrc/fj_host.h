@@ -29,7 +29,10 @@ struct ScannerLut {
     std::vector<float> log2_xyz, slope_c, slope_m, slope_y, cell_min, cell_max;
 };
 
-bool build_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& diagnostic);
+// Builds the PCHIP scanner LUT; a non-finite node reports its flat node index
+// ((c * r + m) * r + y) and channel.
+bool build_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& diagnostic,
+                       uint64_t* fail_node = nullptr, uint32_t* fail_channel = nullptr);
 
 // kernels (fj_pixel.hip)
 // Persistent launch: each kernel gets the blocks that are resident at once on

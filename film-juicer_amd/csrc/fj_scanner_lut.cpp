@@ -40,7 +40,8 @@ void line_slopes(const double* v, int n, double* out) {
 
 }  // namespace
 
-bool build_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& diagnostic) {
+bool build_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& diagnostic, uint64_t* fail_node,
+                       uint32_t* fail_channel) {
     const int r = in.resolution;
     if (r < 2 || r > 256) {
         diagnostic = "scanner LUT resolution must be in [2, 256]";
@@ -79,6 +80,8 @@ bool build_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& 
                     const double l10 = k > 0 ? std::log10(pos + 1e-10) : NAN;
                     if (!std::isfinite(l10)) {
                         diagnostic = "scanner LUT build produced a non-finite log2 XYZ node";
+                        if (fail_node) *fail_node = (static_cast<uint64_t>(c) * r + m) * r + y;
+                        if (fail_channel) *fail_channel = static_cast<uint32_t>(ch);
                         return false;
                     }
                     nodes[at(c, m, y, ch)] = l10 * 3.32192809488736234787;

@@ -69,6 +69,23 @@ FjStatus fjx_diffusion_apply(FjCuda* cuda, int32_t device_id, const FjDiffusion*
 FjStatus fjx_frame_descriptors(uint32_t route, const FjFrame* frame, const FjFrameMathInput* input,
                                FjNativeFrameDescriptors* out, FjErrorBuffer* error);
 
+/* The library's own implementations of the two FjHostPreparation callbacks
+ * (juicer_cuda_call_scope.h; the reference's host supplies them from Rust:
+ * native/juicer_host_preparation.cpp:28-109 validates what they return), with
+ * the callbacks' signatures; `user` is unused. fjx_prepare_scanner_lut builds
+ * the PCHIP scanner LUT of rust scanner/lut.rs:186-299 (f32 words equal to the
+ * reference's); fjx_prepare_gaussian the normalized FIR weights of rust
+ * optics/gaussian.rs:18-56 (radius 1..2048, finite positive sigma). On
+ * success the view borrows storage of the calling thread, valid until that
+ * thread's next call of the same function; on failure the view is cleared and
+ * `failure` carries the reason (FJ_SCANNER_LUT_REASON_*; Gaussian: 1 raw
+ * arguments, 2 radius, 3 sigma, 4 allocation, 5 internal). No device is used.
+ * A host without Rust preparation can pass these in an FjHostPreparation. */
+FjStatus fjx_prepare_scanner_lut(void* user, const FjScannerLutInput* input, FjScannerLutView* view,
+                                 FjScannerLutFailure* failure, FjErrorBuffer* error);
+FjStatus fjx_prepare_gaussian(void* user, const FjGaussianKernelInput* input, FjGaussianKernelView* view,
+                              FjGaussianKernelFailure* failure, FjErrorBuffer* error);
+
 #ifdef __cplusplus
 }
 #endif

@@ -393,12 +393,14 @@ typedef struct Frame {
     pthread_mutex_t lock;
 } Frame;
 
-static void tc_exposure(const FjFilmExposure* fe, const double w[3], double E[3]);
+static void tc_exposure(const float* lut, const double w[3], double E[3]);
 
 /* Reconstructed film raw exposure before scaling, B/G/R order
- * (JuicerCudaDeviceHelpers.cuh:509-617, :619-703). */
-static void px_raw(const FjoPlan* pl, const float* px, double E[3]) {
-    const FjFilmExposure* fe = &pl->p.film_exposure;
+ * (JuicerCudaDeviceHelpers.cuh:509-617, :619-703); the working XYZ and, for
+ * Mallett, the clamped linear sRGB the spectral sum consumes are returned
+ * when asked for. */
+static void film_raw(const FjFilmExposure* fe, const float* px, double* xyz_out, double* srgb_out, double* sums_out,
+                     double E[3]) {
     double rgb[3];
     const int decode = (fe->flags & FJ_FILM_DECODE_CCTF) != 0;
     for (int c = 0; c < 3; ++c) rgb[c] = decode ? decode_cctf((int)fe->input_color_space, px[c]) : fin0(px[c]);
@@ -415,6 +417,12 @@ static void px_raw(const FjoPlan* pl, const float* px, double E[3]) {
         double s[3];
         mat3(fe->xyz_to_linear_srgb, w, s);
         const double r = fmax(0.0, fin0(s[0])), g = fmax(0.0, fin0(s[1])), b = fmax(0.0, fin0(s[2]));
+        if (xyz_out) memcpy(xyz_out, w, sizeof(w));
+        if (srgb_out) {
+            srgb_out[0] = r;
+            srgb_out[1] = g;
+            srgb_out[2] = b;
+        }
         const float* basis = fe->mallett_basis.data;
         const float* ill = fe->mallett_illuminant.data;
         const float* sens = fe->sensitivity_rgb.data;
@@ -426,14 +434,24 @@ static void px_raw(const FjoPlan* pl, const float* px, double E[3]) {
             if (isfinite(sens[i * 3 + 1])) eg += spd * sens[i * 3 + 1];
             if (isfinite(sens[i * 3 + 0])) er += spd * sens[i * 3 + 0];
         }
+        if (sums_out) {
+            sums_out[0] = eb;
+            sums_out[1] = eg;
+            sums_out[2] = er;
+        }
         const double ms = fe->mallett_green_midgray_scale;
         E[0] = fmax(0.0, (isfinite(eb) ? fmax(0.0, eb) : 0.0) * ms);
         E[1] = fmax(0.0, (isfinite(eg) ? fmax(0.0, eg) : 0.0) * ms);
         E[2] = fmax(0.0, (isfinite(er) ? fmax(0.0, er) : 0.0) * ms);
     } else {
         for (int c = 0; c < 3; ++c) w[c] = fin0(w[c]);
-        tc_exposure(fe, w, E);
+        if (xyz_out) memcpy(xyz_out, w, sizeof(w));
+        tc_exposure(fe->tc_lut_rgba.data, w, E);
     }
+}
+
+static void px_raw(const FjoPlan* pl, const float* px, double E[3]) {
+    film_raw(&pl->p.film_exposure, px, NULL, NULL, NULL, E);
 }
 
 /* Film raw exposure with the auto-exposure (1 when off), manual and route
@@ -632,15 +650,21 @@ static void carrier_exposure(const FjoPlan* pl, double* const carrier[3], size_t
 }
 
 /* TC-LUT exposure (Hanatos / Arctic) from working XYZ. */
-static void tc_exposure(const FjFilmExposure* fe, const double w[3], double E[3]) {
+static void tc_exposure(const float* lut, const double w[3], double E[3]) {
     const double sum = w[0] + w[1] + w[2];
+    if (!(fabs(sum) <= FLT_MAX)) {
+        /* The FP32 brightness overflows: brightness x sample is not finite,
+         * and a non-finite lane becomes 0 (reconstruction.rs:684-703). */
+        E[0] = E[1] = E[2] = 0.0;
+        return;
+    }
     const double den = fmax(sum, 1e-10);
     const double x = w[0] / den, y = w[1] / den;
     /* tri2quad (JuicerCudaDeviceHelpers.cuh:244-252) */
     const double qx = fmin(1.0, fmax(0.0, (1.0 - x) * (1.0 - x)));
     const double qy = fmin(1.0, fmax(0.0, y / fmax(1.0 - x, 1e-10)));
     double rgbL[3];
-    tc_sample(fe->tc_lut_rgba.data, qx, qy, rgbL);
+    tc_sample(lut, qx, qy, rgbL);
     E[0] = fin0(sum * fin0(rgbL[2]));
     E[1] = fin0(sum * fin0(rgbL[1]));
     E[2] = fin0(sum * fin0(rgbL[0]));
@@ -738,8 +762,7 @@ static uint32_t px_develop(const FjoPlan* pl, const double E[3], double Dcmy[3])
  * balance normalizer, preflash, print exposure, log10, print curves. */
 /* 81-λ print exposure up to the mid-gray normalizer and preflash
  * (print_inner_raw_device, JuicerCudaDeviceHelpers.cuh:866-995). */
-static void print_inner(const FjoPlan* pl, const double D[3], double acc[3]) {
-    const FjPrint* pr = &pl->p.print;
+static void print_integral(const FjPrint* pr, const double D[3], double acc[3]) {
     const float* eps = pr->film_density_cmy.data;
     const float* base = pr->film_base_density.data;
     const float* sens = pr->sensitivity_cmy.data;
@@ -754,6 +777,11 @@ static void print_inner(const FjoPlan* pl, const double D[3], double acc[3]) {
         for (int c = 0; c < 3; ++c)
             if (!isnan(s[c])) acc[c] += e * s[c];
     }
+}
+
+static void print_inner(const FjoPlan* pl, const double D[3], double acc[3]) {
+    const FjPrint* pr = &pl->p.print;
+    print_integral(pr, D, acc);
     const double pf = pr->preflash_exposure;
     const int preflash = (pr->flags & FJ_PRINT_PREFLASH) != 0;
     for (int c = 0; c < 3; ++c) {
@@ -1814,3 +1842,38 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     if (error_flags) *error_flags = f.errors;
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* per-pixel stages as test entry points                                    */
+/* ------------------------------------------------------------------------ */
+
+void fjo_film_raw(const FjFilmExposure* fe, const float rgb[3], double xyz[3], double srgb[3], double sums_bgr[3],
+                  double raw_bgr[3]) {
+    film_raw(fe, rgb, xyz, srgb, sums_bgr, raw_bgr);
+}
+
+void fjo_tc_exposure(const float* tc_lut_rgba, const double xyz[3], double raw_bgr[3]) {
+    const double w[3] = {fin0(xyz[0]), fin0(xyz[1]), fin0(xyz[2])};
+    tc_exposure(tc_lut_rgba, w, raw_bgr);
+}
+
+void fjo_print_integral(const FjPrint* print, const double density_cmy[3], double raw[3]) {
+    print_integral(print, density_cmy, raw);
+}
+
+double fjo_curve_at(const float* x, const float* y, int n, int stride, double log_exposure, double gamma) {
+    const Curve cv = {x, y, n, stride};
+    return curve_at(&cv, log_exposure, gamma);
+}
+
+int fjo_langmuir(double v, double k, double ref, double* out) { return langmuir(v, k, ref, out); }
+
+int fjo_plan_scan_lut(const FjoPlan* plan, const double normalized_cmy[3], double log2_xyz[3]) {
+    return px_scan_lut(plan, normalized_cmy, log2_xyz);
+}
+
+int fjo_plan_scan(const FjoPlan* plan, const double density_cmy[3], double rgb[3]) { return px_scan(plan, density_cmy, rgb); }
+
+void fjo_gamut_compress(const FjOutputColor* color, double rgb[3]) { gamut_compress(color, rgb); }
+
+void fjo_encode(const FjOutputColor* color, double rgb[3]) { px_encode(color, rgb); }

@@ -113,6 +113,34 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
                ptrdiff_t src_stride_floats, float* dst, ptrdiff_t dst_stride_floats,
                double* linear_rgb, int threads, uint32_t* error_flags);
 
+/* Per-pixel stages of the chain, the functions fjo_render composes, exposed
+ * so that tests can drive each with the reference's own fixture words. */
+/* Decode -> RGB->XYZ -> adapt -> Mallett (81-λ sum, mid-gray scale applied)
+ * or TC-LUT exposure; xyz: working XYZ, srgb: clamped linear sRGB (Mallett
+ * only), sums_bgr: the unclamped Mallett sums (Mallett only), raw_bgr: the
+ * B/G/R raw exposure (JuicerCudaDeviceHelpers.cuh:509-703). */
+void fjo_film_raw(const FjFilmExposure* exposure, const float rgb[3], double xyz[3], double srgb[3], double sums_bgr[3],
+                  double raw_bgr[3]);
+/* TC-LUT exposure of a working XYZ (tri2quad + Mitchell bicubic x brightness,
+ * JuicerCudaDeviceHelpers.cuh:233-327, :563-617); raw in B/G/R order. */
+void fjo_tc_exposure(const float* tc_lut_rgba, const double xyz[3], double raw_bgr[3]);
+/* The 81-λ print integral of a C/M/Y density (print_spectral_integrate_device,
+ * JuicerCudaDeviceHelpers.cuh:866-933), before normalizer and preflash. */
+void fjo_print_integral(const FjPrint* print, const double density_cmy[3], double raw[3]);
+/* Development curve lookup (JuicerCudaDeviceHelpers.cuh:34-89). */
+double fjo_curve_at(const float* x, const float* y, int n, int stride, double log_exposure, double gamma);
+/* Guarded Langmuir v (k + ref) / (k + v) (JuicerCudaDeviceHelpers.cuh:1073-1123). */
+int fjo_langmuir(double v, double k, double ref, double* out);
+/* Tricubic PCHIP lookup of the plan's scanner LUT at normalized C/M/Y
+ * (JuicerCudaDeviceHelpers.cuh:407-506); 0 for non-finite input. */
+int fjo_plan_scan_lut(const FjoPlan* plan, const double normalized_cmy[3], double log2_xyz[3]);
+/* Scanner stage of one density: LUT, correction, CAT02, RGB, gamut. */
+int fjo_plan_scan(const FjoPlan* plan, const double density_cmy[3], double rgb[3]);
+/* OkLch output gamut compression (src/GamutCompression.h:178-213). */
+void fjo_gamut_compress(const FjOutputColor* color, double rgb[3]);
+/* Output matrix, CCTF and [0, 1] clamp (JuicerCudaScanPipeline.cu:99-190). */
+void fjo_encode(const FjOutputColor* color, double rgb[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,16 @@ def lib():
         L.fjo_highlight_boost.argtypes = [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.c_size_t]
         L.fjo_auto_exposure.restype = C.c_double
         L.fjo_auto_exposure.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t]
+        L.fjo_film_raw.argtypes = [C.c_void_p, fp, dp, dp, dp, dp]
+        L.fjo_tc_exposure.argtypes = [fp, dp, dp]
+        L.fjo_print_integral.argtypes = [C.c_void_p, dp, dp]
+        L.fjo_curve_at.restype = C.c_double
+        L.fjo_curve_at.argtypes = [fp, fp, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.fjo_langmuir.argtypes = [C.c_double, C.c_double, C.c_double, dp]
+        L.fjo_plan_scan_lut.argtypes = [C.c_void_p, dp, dp]
+        L.fjo_plan_scan.argtypes = [C.c_void_p, dp, dp]
+        L.fjo_gamut_compress.argtypes = [C.c_void_p, dp]
+        L.fjo_encode.argtypes = [C.c_void_p, dp]
         _lib = L
     return _lib
 
@@ -193,6 +203,18 @@ class Plan:
         sizes = [r ** 3 * 3] * 4 + [(r - 1) ** 3 * 3] * 2
         return r, [np.ctypeslib.as_array(p, shape=(n,)).copy() for p, n in zip(ptrs, sizes)]
 
+    def scan_lut(self, normalized_cmy):
+        """Tricubic PCHIP lookup (log2 XYZ) at one normalized C/M/Y point."""
+        n, out = _vec(normalized_cmy), np.zeros(3)
+        ok = lib().fjo_plan_scan_lut(self.handle, _dp(n), _dp(out))
+        return out if ok else None
+
+    def scan(self, density_cmy):
+        """Scanner stage (LUT, correction, CAT02, RGB, gamut) of one density."""
+        d, out = _vec(density_cmy), np.zeros(3)
+        ok = lib().fjo_plan_scan(self.handle, _dp(d), _dp(out))
+        return out if ok else None
+
     def auto_exposure(self, src: np.ndarray) -> float:
         """The metered auto-exposure scale of an HxWxC frame (1 when off)."""
         src = np.ascontiguousarray(src, dtype=np.float32)
@@ -245,3 +267,59 @@ def splitmix64(v: int) -> int:
 def det(name: str, values) -> np.ndarray:
     f = getattr(lib(), f"fjo_det_{name}f")
     return np.array([f(float(v)) for v in np.asarray(values, dtype=np.float32).ravel()], dtype=np.float32)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _vec(v):
+    return np.ascontiguousarray(v, dtype=np.float64).copy()
+
+
+def film_raw(exposure, rgb):
+    """(working XYZ, clamped linear sRGB, unclamped 81-λ sums (both Mallett
+    only), raw B/G/R) of one pixel through an FjFilmExposure record."""
+    px = np.ascontiguousarray(rgb, dtype=np.float32)
+    xyz, srgb, sums, raw = np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(3)
+    lib().fjo_film_raw(C.byref(exposure), _fp(px), _dp(xyz), _dp(srgb), _dp(sums), _dp(raw))
+    return xyz, srgb, sums, raw
+
+
+def tc_exposure(tc_lut_rgba, xyz):
+    """Raw B/G/R exposure of a working XYZ through a 192x192 RGBA TC LUT."""
+    lut = np.ascontiguousarray(tc_lut_rgba, dtype=np.float32)
+    assert lut.size == 192 * 192 * 4
+    x, out = _vec(xyz), np.zeros(3)
+    lib().fjo_tc_exposure(_fp(lut), _dp(x), _dp(out))
+    return out
+
+
+def print_integral(print_record, density_cmy):
+    d, out = _vec(density_cmy), np.zeros(3)
+    lib().fjo_print_integral(C.byref(print_record), _dp(d), _dp(out))
+    return out
+
+
+def curve_at(x, y, log_exposure, gamma=1.0, stride=1):
+    xs = np.ascontiguousarray(x, dtype=np.float32)
+    ys = np.ascontiguousarray(y, dtype=np.float32)
+    return lib().fjo_curve_at(_fp(xs), _fp(ys), xs.size, stride, float(log_exposure), float(gamma))
+
+
+def langmuir(v, k, ref):
+    out = C.c_double(0.0)
+    ok = lib().fjo_langmuir(float(v), float(k), float(ref), C.byref(out))
+    return ok, out.value
+
+
+def gamut_compress(output_color, rgb):
+    v = _vec(rgb)
+    lib().fjo_gamut_compress(C.byref(output_color), _dp(v))
+    return v
+
+
+def encode(output_color, rgb):
+    v = _vec(rgb)
+    lib().fjo_encode(C.byref(output_color), _dp(v))
+    return v
