@@ -526,6 +526,26 @@ FJ_EXPORT FjStatus fjx_diffusion_apply(FjCuda* cuda, int32_t device_id, const Fj
     }
 }
 
+FJ_EXPORT FjStatus fjx_highlight_boost_apply(FjCuda* cuda, int32_t device_id, const FjHighlightBoost* boost, float* red,
+                                             float* green, float* blue, int64_t samples, uint64_t stream,
+                                             FjErrorBuffer* error) {
+    try {
+        if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
+        if (!boost) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null highlight boost descriptor");
+        FjStatus st{};
+        DeviceState* dev = device_state(cuda, device_id, st, error);
+        if (!dev) return st;
+        float* const planes[3] = {red, green, blue};
+        return highlight_boost_planes(*dev, *boost, planes, samples, reinterpret_cast<hipStream_t>(stream), error);
+    } catch (const std::bad_alloc&) {
+        return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fjx_highlight_boost_apply");
+    }
+}
+
 FJ_EXPORT FjStatus fjx_render_batch(FjCuda* cuda, const FjCudaContext* context, int32_t count, const FjFrame* frames,
                                     const FjSubmission* submissions, const FjPreparedHostData* const* prepared,
                                     int32_t streams, FjAbortCallback abort_callback, FjRenderOutcome* outcomes,

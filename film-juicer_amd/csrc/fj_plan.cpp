@@ -400,6 +400,18 @@ bool post_active(const FjScannerPostEffects& sp) { return unsharp_active(sp) || 
 // separate blur + combine before the encode.
 bool unsharp_separate(const FjScannerPostEffects& sp) { return unsharp_active(sp) && sp.unsharp_radius > max_post_radius(); }
 
+}  // namespace
+
+bool highlight_boost_valid(const FjHighlightBoost& hb) {
+    // The recipe's sharpness is 28^(1 - range), range in [0, 1]
+    // (rust optics/highlight_boost.rs:56, :84); larger shapes overflow the
+    // curve's normalization.
+    return std::isfinite(hb.protection) && std::isfinite(hb.sharpness) && std::isfinite(hb.peak_gain) &&
+           hb.protection >= 0.0 && hb.sharpness >= 0.0 && hb.sharpness <= kHighlightMaxSharpness && hb.peak_gain >= 1.0;
+}
+
+namespace {
+
 // Families this build renders; everything else is rejected loudly.
 bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
     if (p.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE) {
@@ -433,13 +445,7 @@ bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
         const FjHighlightBoost& hb = p.optics.highlight_boost;
         const bool flag = (p.optics.flags & FJ_OPTICS_HIGHLIGHT_BOOST) != 0;
         if (flag != (hb.recipe_hash != 0)) return d.malformed("optics.highlight_boost");
-        // The recipe's sharpness is 28^(1 - range), range in [0, 1]
-        // (rust optics/highlight_boost.rs:56, :84); larger shapes overflow
-        // the curve's normalization.
-        if (flag && (!std::isfinite(hb.protection) || !std::isfinite(hb.sharpness) || !std::isfinite(hb.peak_gain) ||
-                     hb.protection < 0.0 || hb.sharpness < 0.0 || hb.sharpness > kHighlightMaxSharpness ||
-                     hb.peak_gain < 1.0))
-            return d.malformed("optics.highlight_boost");
+        if (flag && !highlight_boost_valid(hb)) return d.malformed("optics.highlight_boost");
     }
     if (p.optics.flags & FJ_OPTICS_SCATTER_HALATION) {
         std::string field;
@@ -1427,6 +1433,50 @@ FjStatus grain_apply_planes(DeviceState& dev, const FjPreparedHostData& prepared
     if (e != hipSuccess) return outcome_hip(e, "visual grain launch", error).status;
     e = dev.release_workspace(stream);
     if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
+    put_error(error, "");
+    return status_of(FJ_STATUS_SUCCESS);
+}
+
+FjStatus highlight_boost_planes(DeviceState& dev, const FjHighlightBoost& boost, float* const planes[3],
+                                long long samples, hipStream_t stream, FjErrorBuffer* error) {
+    if (boost.recipe_hash == 0 || !highlight_boost_valid(boost))
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "MalformedPreparedHostData field=optics.highlight_boost", error)
+            .status;
+    if (samples < 0 || !planes[0] || !planes[1] || !planes[2])
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput highlight boost planes", error).status;
+    ++dev.frame_serial;
+    hipError_t e = dev.ensure_workspace(sizeof(HbState));
+    if (e != hipSuccess) return outcome_hip(e, "workspace", error).status;
+    e = dev.acquire_workspace(stream);
+    if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
+    const int slot = dev.error_next;
+    dev.error_next = (dev.error_next + 1) % DeviceState::kRing;
+    if (dev.error_pending[slot]) {
+        e = hipEventSynchronize(dev.error_events[slot]);
+        if (e != hipSuccess) return outcome_hip(e, "error ring", error).status;
+        dev.error_pending[slot] = false;
+    }
+    HbArgs ha{};
+    for (int c = 0; c < 3; ++c) ha.planes[c] = planes[c];
+    ha.n = samples;
+    ha.protection = boost.protection;
+    ha.sharpness = boost.sharpness;
+    ha.peak_gain = boost.peak_gain;
+    ha.state = static_cast<HbState*>(dev.workspace());
+    ha.error_word = dev.error_words + slot;
+    e = launch_clear_word(ha.error_word, stream);
+    if (e == hipSuccess) e = launch_highlight_boost(ha, stream);
+    const hipError_t r = dev.release_workspace(stream);
+    if (e != hipSuccess) return outcome_hip(e, "highlight boost", error).status;
+    if (r != hipSuccess) return outcome_hip(r, "workspace order", error).status;
+    e = hipMemcpyAsync(dev.error_host + slot, ha.error_word, sizeof(unsigned int), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipEventRecord(dev.error_events[slot], stream);
+    if (e == hipSuccess) e = hipEventSynchronize(dev.error_events[slot]);
+    if (e != hipSuccess) return outcome_hip(e, "highlight boost status", error).status;
+    const unsigned int bits = dev.error_host[slot];
+    if (bits & kHbInvalidCarrierBit)
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "HighlightBoostInvalidCarrier", error).status;
+    if (bits & kHbPeakOverflowBit) return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "HighlightBoostPeakOverflow", error).status;
     put_error(error, "");
     return status_of(FJ_STATUS_SUCCESS);
 }

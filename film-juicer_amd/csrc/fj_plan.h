@@ -104,6 +104,14 @@ FjStatus grain_apply_planes(DeviceState& dev, const FjPreparedHostData& prepared
 FjStatus diffusion_apply_planes(DeviceState& dev, const FjDiffusion& stage, float* const planes[3], int width,
                                 int height, hipStream_t stream, FjErrorBuffer* error);
 
+// Highlight boost of three caller-owned carrier planes of `samples` floats
+// each (B/G/R or R/G/B: the operator treats the channels alike), in place on
+// `stream`; waits for the disposition and reports a failing one.
+FjStatus highlight_boost_planes(DeviceState& dev, const FjHighlightBoost& boost, float* const planes[3],
+                                long long samples, hipStream_t stream, FjErrorBuffer* error);
+// The checks a boost descriptor must pass (recipe bound, finite fields).
+bool highlight_boost_valid(const FjHighlightBoost& boost);
+
 uint64_t hash_bytes(const void* data, size_t bytes, uint64_t seed);
 
 }  // namespace fjx

@@ -72,6 +72,9 @@ def lib():
         L.fjx_stage_timings.argtypes = [C.c_void_p, C.c_int32, C.POINTER(FjxStageTiming), C.c_int32]
         L.fj_cuda_destroy.restype = abi.FjStatus
         L.fj_cuda_destroy.argtypes = [C.c_void_p, eb]
+        L.fjx_highlight_boost_apply.restype = abi.FjStatus
+        L.fjx_highlight_boost_apply.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjHighlightBoost), C.c_void_p,
+                                                C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64, eb]
         L.fjx_prepare_scanner_lut.restype = abi.FjStatus
         L.fjx_prepare_scanner_lut.argtypes = [C.c_void_p, C.POINTER(abi.FjScannerLutInput),
                                               C.POINTER(abi.FjScannerLutView), C.POINTER(abi.FjScannerLutFailure), eb]

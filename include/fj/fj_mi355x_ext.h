@@ -60,6 +60,20 @@ FjStatus fjx_render_batch(FjCuda* cuda, const FjCudaContext* context, int32_t co
 FjStatus fjx_diffusion_apply(FjCuda* cuda, int32_t device_id, const FjDiffusion* stage, float* red, float* green,
                              float* blue, int32_t width, int32_t height, uint64_t stream, FjErrorBuffer* error);
 
+/* The camera-carrier highlight boost (launch_highlight_boost,
+ * src/Cuda/Film/JuicerCudaHighlightBoost.cu:195-205, internal in the
+ * reference) applied in place to three caller-owned carrier planes of
+ * `samples` floats each in device memory, on `stream`: the carrier maximum and
+ * validity, the disposition, then the boost of every sample above the
+ * protection. The call waits for the disposition and returns
+ * UnsupportedInput "HighlightBoostInvalidCarrier" (a non-finite or negative
+ * sample) or "HighlightBoostPeakOverflow" (maximum x gain above FLT_MAX) as
+ * check_highlight_boost does (src/ProcessRoot.cpp:4063-4096); the planes
+ * are unchanged then. The descriptor needs a recipe hash, finite fields,
+ * protection >= 0, sharpness in [0, 28] and gain >= 1. */
+FjStatus fjx_highlight_boost_apply(FjCuda* cuda, int32_t device_id, const FjHighlightBoost* boost, float* red,
+                                   float* green, float* blue, int64_t samples, uint64_t stream, FjErrorBuffer* error);
+
 /* The spatial-DIR, visual-grain and film-effects descriptors of one frame,
  * composed from the host's frame math without a device: the composition
  * FjCudaCallAccess.frame_descriptors performs for an inspected frame
