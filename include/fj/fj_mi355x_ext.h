@@ -74,6 +74,20 @@ FjStatus fjx_diffusion_apply(FjCuda* cuda, int32_t device_id, const FjDiffusion*
 FjStatus fjx_highlight_boost_apply(FjCuda* cuda, int32_t device_id, const FjHighlightBoost* boost, float* red,
                                    float* green, float* blue, int64_t samples, uint64_t stream, FjErrorBuffer* error);
 
+/* No counterpart in the reference: fjx_highlight_boost_apply for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, validation,
+ * kernels and results as fjx_highlight_boost_apply; its scratch (reduction
+ * partials, disposition state, error word: 8,704 bytes) lives in a fixed
+ * device array that each GPU holds until the process ends, so the call makes
+ * no hipMalloc/hipFree and uses no device state of the owner. Planes of more
+ * than 2^34 samples are refused. Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU race (this call waits for its stream
+ * before returning, so calls through one owner never overlap). */
+FjStatus fjx_highlight_boost_apply_static(FjCuda* cuda, int32_t device_id, const FjHighlightBoost* boost, float* red,
+                                          float* green, float* blue, int64_t samples, uint64_t stream,
+                                          FjErrorBuffer* error);
+
 /* The spatial-DIR, visual-grain and film-effects descriptors of one frame,
  * composed from the host's frame math without a device: the composition
  * FjCudaCallAccess.frame_descriptors performs for an inspected frame
