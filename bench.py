@@ -29,7 +29,6 @@ sys.path.insert(0, ROOT)
 
 WIDTH, HEIGHT = 3840, 2160
 GRAIN_SEED = 7
-CASE = "PrintPreflash"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (packed); 78.6 unpacked
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the diffusion column filter (dct_filter_rows_kernel<true>)
@@ -52,20 +51,20 @@ def smooth_length(n: int) -> int:
         m += 1
 
 
-def flop_census(k_active: int, print_route: bool) -> int:
-    """Algorithmic FP32 flops per pixel of the fused chain (FMA = 2).
+def flop_census(k_active: int) -> dict:
+    """Algorithmic FP32 flops per pixel of the three timed chain kernels
+    (FMA = 2; transcendentals, compares and selects not counted).
 
-    Counted from the implemented chain: input matrix 15, chromaticity 8,
-    TC coordinates 6, Mitchell weights 54, 16-tap bicubic x3 channels 144,
-    normalize/exposure 10, film development + DIR couplers 51,
-    print exposure 12 per active wavelength + 9 scale + 24 develop,
-    scanner normalize 6 + PCHIP tricubic 276, colour matrix 15,
-    OkLch gamut 140, output encode 9. Transcendentals are not counted.
+    pixel_chain (DIR final development): corrected log exposure from the
+    cached log raw and three filtered corrections, receiver Langmuir and the
+    DIR-compensated curve lookups, 51. print_linear (grained density -> print
+    exposure): 12 per active print wavelength (3 FMA dye density, 1 base, the
+    exp2 argument, 3 FMA accumulations) + 9 for the balance normalizer,
+    preflash and exposure scale. print_scan (diffused print exposure -> scan):
+    log + print curves 24, scanner normalize 6, PCHIP tricubic 276, correction
+    + CAT02 + RGB matrix 15, OkLch gamut 140: 461.
     """
-    base = 15 + 8 + 6 + 54 + 144 + 10 + 51 + 6 + 276 + 15 + 140 + 9
-    if print_route:
-        base += 12 * k_active + 9 + 24
-    return base
+    return {"pixel_chain": 51, "print_linear": 12 * k_active + 9, "print_scan": 461}
 
 
 def active_wavelengths(rec) -> int:
@@ -79,75 +78,37 @@ def active_wavelengths(rec) -> int:
     return int(ok.sum())
 
 
-def cpu_baseline(p, src, budget_s: float = 10.0):
-    """The oracle on bounded samples of the same workload, all host cores.
-
-    Two samples, scaled to whole frames and summed: (1) every stage except the
-    two diffusion convolutions, on bands of the UHD frame grown until one band
-    takes about a second and repeated for about `budget_s`; (2) the diffusion
-    operator (FP64 direct convolution with the stage's true 834 px PSF
-    radius, whose cost per output pixel does not depend on the frame size) on
-    one small frame per host thread, run concurrently.
-    """
-    import ctypes as C
-    import threading
-
-    import numpy as np
-
-    from fj import abi
+def cpu_baseline(p, src):
+    """The oracle on one whole frame of the same workload: every stage in FP64
+    on the host cores (the box's CPU share: at most 16 threads), the two
+    diffusion stages by FFT (oracle.diffusion_fft). Returns the baseline and
+    the oracle's encoded and linear frames, which the parity check reuses."""
     from oracle import oracle
     cores = min(len(os.sched_getaffinity(0)), 16)
-    rec = p.record
-    flags = rec.optics.flags
-    diff_flags = abi.FJ_OPTICS_CAMERA_DIFFUSION | abi.FJ_OPTICS_ENLARGER_DIFFUSION
-    camera = abi.FjDiffusion.from_buffer_copy(rec.optics.camera)
-    enlarger = abi.FjDiffusion.from_buffer_copy(rec.optics.enlarger)
-    dhash = rec.optics.diffusion_hash
-    rec.optics.flags = flags & ~diff_flags
-    rec.optics.camera, rec.optics.enlarger, rec.optics.diffusion_hash = abi.FjDiffusion(), abi.FjDiffusion(), 0
-    try:
-        plan = oracle.Plan(p)
-        rows = 16
-        while True:
-            t0 = time.perf_counter()
-            plan.render(src[:rows], threads=cores, want_dst=True)
-            t = time.perf_counter() - t0
-            if t >= 1.0 or rows >= HEIGHT:
-                break
-            rows = min(HEIGHT, rows * 4)
-        reps = max(1, int(round(budget_s / max(t, 1e-6))))
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            plan.render(src[:rows], threads=cores, want_dst=True)
-        t_band = time.perf_counter() - t0
-    finally:
-        rec.optics.flags = flags
-        rec.optics.camera, rec.optics.enlarger, rec.optics.diffusion_hash = camera, enlarger, dhash
-    per_px_chain = t_band / (reps * rows * WIDTH)
-    # Diffusion: each thread convolves a 4 x 2 frame (three planes) with the
-    # camera stage's PSF; both stages share the same radius.
-    sw, sh = 4, 2
-    planes = [[np.random.default_rng(i).random((sh, sw)) for _ in range(3)] for i in range(cores)]
-
-    def work(i):
-        arr = (C.POINTER(C.c_double) * 3)(*[pl.ctypes.data_as(C.POINTER(C.c_double)) for pl in planes[i]])
-        oracle.lib().fjo_diffusion(C.byref(camera), arr, sw, sh)
-
-    threads = [threading.Thread(target=work, args=(i,)) for i in range(cores)]
+    plan = oracle.Plan(p).use_fft_diffusion()
     t0 = time.perf_counter()
-    for th in threads:
-        th.start()
-    for th in threads:
-        th.join()
-    t_diff = time.perf_counter() - t0
-    per_px_diff = t_diff / (cores * sw * sh)  # one stage, three channels, all threads busy
-    stages = sum(1 for f in (abi.FJ_OPTICS_CAMERA_DIFFUSION, abi.FJ_OPTICS_ENLARGER_DIFFUSION) if flags & f)
-    frame_s = (per_px_chain + stages * per_px_diff) * WIDTH * HEIGHT
-    return {"value": 1.0 / frame_s, "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": f"oracle on {cores} threads: all stages but diffusion on {reps} x the top {rows} rows of "
-                      f"the UHD frame ({t_band:.1f} s), plus the diffusion convolution (radius "
-                      f"{camera.radius_px} px) on {cores} concurrent {sw}x{sh} frames ({t_diff:.1f} s); "
-                      f"per-pixel costs scaled to whole frames ({stages} diffusion stages)"}
+    dst, lin, _ = plan.render(src, threads=cores, want_dst=True)
+    t = time.perf_counter() - t0
+    return {"value": 1.0 / t, "unit": "frames/s", "cores": cores, "kind": "port",
+            "sample": f"one whole {WIDTH}x{HEIGHT} frame of the benchmarked workload (every stage, FP64, "
+                      f"diffusion by FFT) on {cores} threads: {t:.1f} s"}, dst, lin
+
+
+def parity(got_encoded, got_linear, oracle_dst, oracle_lin):
+    """Errors of the benchmarked frame against the oracle: the linear output
+    (a second render of the same record with the output CCTF off) and the
+    encoded output under the reference's pixel gate 2e-4 + 3e-4 |ref|."""
+    import numpy as np
+    lin_err = np.abs(got_linear[..., :3].astype(np.float64) - np.clip(oracle_lin, 0.0, 1.0))
+    enc_ref = oracle_dst[..., :3].astype(np.float64)
+    enc_err = np.abs(got_encoded[..., :3].astype(np.float64) - enc_ref)
+    over = lin_err.max(axis=-1) > 2e-4
+    return {"max_abs_err_linear": float(lin_err.max()), "mean_abs_err_linear": float(lin_err.mean()),
+            "pixels_over_2e-4_linear": int(over.sum()),
+            "max_abs_err_linear_other_pixels": float(lin_err[~over].max()) if (~over).any() else 0.0,
+            "max_abs_err_encoded": float(enc_err.max()),
+            "encoded_outside_pixel_gate": int((enc_err > 2e-4 + 3e-4 * np.abs(enc_ref)).sum()),
+            "alpha_exact": bool(np.array_equal(got_encoded[..., 3], oracle_dst[..., 3]))}
 
 
 def config1(with_cpu: bool):
@@ -298,17 +259,10 @@ def main():
     import numpy as np
     import torch
 
-    from fj import abi, descriptors, prepared, runtime, synth
+    from fj import abi, runtime, synth, workloads
 
     dev = torch.device("cuda:0")
-    p = prepared.simple_view(prepared.load(CASE), unsharp=True)
-    p.record.print.preflash_exposure = 0.01
-    pixel_um = descriptors.pixel_size_for(WIDTH, HEIGHT)
-    descriptors.spatial_dir(p.record, 20.0, 200.0, 0.03, pixel_um, WIDTH, HEIGHT)
-    prepared.with_halation(p, WIDTH, HEIGHT, pixel_um)
-    mist = descriptors.diffusion_recipe(1, 0.5)
-    prepared.with_diffusion(p, WIDTH, HEIGHT, pixel_um, camera=mist, enlarger=mist)
-    prepared.with_grain(p, WIDTH, HEIGHT, pixel_um, session_seed=GRAIN_SEED, noise_shape=(512, 512, 256))
+    p, pixel_um = workloads.uhd_print(WIDTH, HEIGHT)
     src_np = synth.frame(WIDTH, HEIGHT, seed=synth.DEFAULT_SEED)
     src = torch.from_numpy(src_np).to(dev)
     dst = torch.empty_like(src)
@@ -374,9 +328,9 @@ def main():
     # it runs as three kernels (film development, print exposure, print
     # development + scan); the census counts the whole chain.
     k = active_wavelengths(p.record)
-    fpx = flop_census(k, True)
+    census = flop_census(k)
     chain_ms = stage("pixel_chain") + stage("print_linear") + stage("print_scan")
-    achieved_tflops = fpx * pixels / (chain_ms * 1e-3) / 1e12
+    achieved_tflops = sum(census.values()) * pixels / (chain_ms * 1e-3) / 1e12
     dct_ms = stage_ms.get("dir_dct_filter")
     hal_ms = stage("halation_scatter") + stage("halation_bounce")
     diff_ms = stage("camera_diffusion") + stage("enlarger_diffusion")
@@ -437,7 +391,7 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-            "flop_per_px": fpx,
+            "flop_per_px": census,
             "kernel_ms": chain_ms,
         },
         "roofline_dct": None if dct_ms is None else {
@@ -474,7 +428,16 @@ def main():
     if not args.no_batch:
         result["batched"] = batched()
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(p, src_np)
+        lin_rec, _ = workloads.uhd_print(WIDTH, HEIGHT, encode=False)
+        lin_dst = torch.empty_like(src)
+        lin_frame = runtime.frame_for(src.data_ptr(), lin_dst.data_ptr(), WIDTH, HEIGHT, 4,
+                                      stream=stream.cuda_stream, pixel_size_um=pixel_um, session_seed=GRAIN_SEED)
+        r.render(frame, p.record, submission=sub, context=ctx)
+        r.render(lin_frame, lin_rec.record, submission=runtime.submission_for(lin_rec.record), context=ctx)
+        torch.cuda.synchronize()
+        got_enc, got_lin = dst.cpu().numpy(), lin_dst.cpu().numpy()
+        result["cpu_baseline"], o_dst, o_lin = cpu_baseline(p, src_np)
+        result["parity"] = parity(got_enc, got_lin, o_dst, o_lin)
     print(json.dumps(result))
 
 

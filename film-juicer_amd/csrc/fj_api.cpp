@@ -546,6 +546,29 @@ FJ_EXPORT FjStatus fjx_highlight_boost_apply(FjCuda* cuda, int32_t device_id, co
     }
 }
 
+FJ_EXPORT FjStatus fjx_capture_grain_input(FjCuda* cuda, int32_t device_id, float* density_c, float* density_m,
+                                           float* density_y, int64_t samples, FjErrorBuffer* error) {
+    try {
+        if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
+        const bool arm = density_c || density_m || density_y;
+        if (arm && (!density_c || !density_m || !density_y || samples <= 0))
+            return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "grain capture needs three planes and a sample count");
+        FjStatus st{};
+        DeviceState* dev = device_state(cuda, device_id, st, error);
+        if (!dev) return st;
+        dev->grain_capture[0] = density_c;
+        dev->grain_capture[1] = density_m;
+        dev->grain_capture[2] = density_y;
+        dev->grain_capture_samples = arm ? samples : 0;
+        write_error(error, "");
+        return make_status(FJ_STATUS_SUCCESS);
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fjx_capture_grain_input");
+    }
+}
+
 FJ_EXPORT FjStatus fjx_render_batch(FjCuda* cuda, const FjCudaContext* context, int32_t count, const FjFrame* frames,
                                     const FjSubmission* submissions, const FjPreparedHostData* const* prepared,
                                     int32_t streams, FjAbortCallback abort_callback, FjRenderOutcome* outcomes,

@@ -1228,6 +1228,14 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     if (grain) {
         // develop_print_capture (JuicerCudaExecutor.cpp:2198-2280): visual
         // grain on the developed density, then the print exposure.
+        if (dev.grain_capture[0]) {
+            if (plane <= dev.grain_capture_samples)
+                for (int c = 0; c < 3 && e == hipSuccess; ++c)
+                    e = hipMemcpyAsync(dev.grain_capture[c], pp.dens[c], static_cast<size_t>(plane) * sizeof(float),
+                                       hipMemcpyDeviceToDevice, stream);
+            dev.grain_capture[0] = dev.grain_capture[1] = dev.grain_capture[2] = nullptr;
+            if (e != hipSuccess) return outcome_hip(e, "grain input capture", error);
+        }
         mark("visual_grain");
         e = run_grain(gp, pp.dens, pp.io.width, pp.io.height, ws + grain_off, stream);
         if (e != hipSuccess) return outcome_hip(e, "print visual grain launch", error);

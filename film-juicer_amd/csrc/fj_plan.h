@@ -62,6 +62,11 @@ struct DeviceState {
     hipEvent_t batch_events[kWorkspaceSlots + 1] = {};
     void* workspace() const { return slots[active_slot].ptr; }
 
+    // One-shot copy of the densities entering the next frame's grain stage
+    // (armed by fjx_capture_grain_input).
+    float* grain_capture[3] = {};
+    long long grain_capture_samples = 0;
+
     // Stage timing of the last traced frame (FJ_FRAME_TRACE_VERBOSE).
     static constexpr int kMaxStages = 32;
     hipEvent_t stage_events[kMaxStages + 1] = {};

@@ -75,6 +75,9 @@ def lib():
         L.fjx_highlight_boost_apply.restype = abi.FjStatus
         L.fjx_highlight_boost_apply.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjHighlightBoost), C.c_void_p,
                                                 C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64, eb]
+        L.fjx_capture_grain_input.restype = abi.FjStatus
+        L.fjx_capture_grain_input.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                              eb]
         L.fjx_prepare_scanner_lut.restype = abi.FjStatus
         L.fjx_prepare_scanner_lut.argtypes = [C.c_void_p, C.POINTER(abi.FjScannerLutInput),
                                               C.POINTER(abi.FjScannerLutView), C.POINTER(abi.FjScannerLutFailure), eb]

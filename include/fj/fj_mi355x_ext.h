@@ -88,6 +88,16 @@ FjStatus fjx_highlight_boost_apply_static(FjCuda* cuda, int32_t device_id, const
                                           float* green, float* blue, int64_t samples, uint64_t stream,
                                           FjErrorBuffer* error);
 
+/* Diagnostic with no counterpart in the reference: arms a one-shot copy of
+ * the three C/M/Y film-density planes entering the visual-grain stage of the
+ * next frame rendered on `device_id` (the developed density of the grain
+ * window, row-major, width x height floats each) into device buffers of at
+ * least `samples` floats, on that frame's stream. A frame larger than
+ * `samples` disarms the capture without copying; null planes disarm it. Used
+ * to check the grain stage on a frame's own densities. */
+FjStatus fjx_capture_grain_input(FjCuda* cuda, int32_t device_id, float* density_c, float* density_m,
+                                 float* density_y, int64_t samples, FjErrorBuffer* error);
+
 /* The spatial-DIR, visual-grain and film-effects descriptors of one frame,
  * composed from the host's frame math without a device: the composition
  * FjCudaCallAccess.frame_descriptors performs for an inspected frame

@@ -176,6 +176,9 @@ struct FjoPlan {
     float* lut; /* one allocation holding the six LUT arrays */
     float *l2, *sc, *sm, *sy, *cmin, *cmax;
     double manual_scale;
+    FjoDiffusionFn diffusion; /* optional replacement of fjo_diffusion */
+    void* diffusion_user;
+    float* grain_capture[3]; /* optional copy of the densities entering the grain stage */
 };
 
 static void set_message(char* msg, size_t cap, const char* text) {
@@ -1386,6 +1389,21 @@ int fjo_diffusion(const FjDiffusion* d, double* const planes[3], int W, int H) {
     return 0;
 }
 
+void fjo_plan_capture_grain_input(FjoPlan* plan, float* c, float* m, float* y) {
+    plan->grain_capture[0] = c;
+    plan->grain_capture[1] = c ? m : NULL;
+    plan->grain_capture[2] = c ? y : NULL;
+}
+
+void fjo_plan_set_diffusion(FjoPlan* plan, FjoDiffusionFn fn, void* user) {
+    plan->diffusion = fn;
+    plan->diffusion_user = user;
+}
+
+static int plan_diffusion(const FjoPlan* plan, const FjDiffusion* d, double* const planes[3], int W, int H) {
+    return plan->diffusion ? plan->diffusion(d, planes, W, H, plan->diffusion_user) : fjo_diffusion(d, planes, W, H);
+}
+
 int fjo_halation(const FjScatterHalation* d, double* const planes[3], int W, int H) {
     static const double kTailAmplitude[3] = {0.1633f, 0.6496f, 0.1870f};
     if (!d || W <= 0 || H <= 0) return 2;
@@ -1745,7 +1763,7 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
         run_rows(&f, rows_carrier, threads);
         double* const rgb_planes[3] = {f.carrier[2], f.carrier[1], f.carrier[0]};
         if (boost) rc = fjo_highlight_boost(&p->optics.highlight_boost, rgb_planes, N);
-        if (rc == 0 && camera_diffusion) rc = fjo_diffusion(&p->optics.camera, rgb_planes, width, height);
+        if (rc == 0 && camera_diffusion) rc = plan_diffusion(plan, &p->optics.camera, rgb_planes, width, height);
         if (rc == 0 && halation) rc = fjo_halation(&p->optics.scatter_halation, rgb_planes, width, height);
     }
     if (enlarger_diffusion)
@@ -1765,13 +1783,15 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
         run_rows(&f, rows_chain, threads);
     }
     if (rc == 0 && grain) {
+        if (plan->grain_capture[0])
+            for (int c = 0; c < 3; ++c) memcpy(plan->grain_capture[c], f.dens[c], sizeof(float) * N);
         rc = fjo_grain_apply(p, f.dens, width, height);
         f.dens_out = 0;
         if (rc == 0) run_rows(&f, rows_density, threads);
     }
     if (rc == 0 && enlarger_diffusion) {
         /* develop_print_medium (JuicerCudaExecutor.cpp:2282-2382). */
-        rc = fjo_diffusion(&p->optics.enlarger, f.plin, width, height);
+        rc = plan_diffusion(plan, &p->optics.enlarger, f.plin, width, height);
         if (rc == 0) run_rows(&f, rows_enlarged, threads);
     }
     const FjScannerPostEffects* sp = &p->scanner_post;

@@ -80,6 +80,16 @@ int fjo_diffusion(const FjDiffusion* d, double* const planes[3], int W, int H);
 
 typedef struct FjoPlan FjoPlan;
 
+/* A diffusion stage implementation with fjo_diffusion's contract (the same
+ * operator on the same planes); 0 on success. */
+typedef int (*FjoDiffusionFn)(const FjDiffusion* d, double* const planes[3], int W, int H, void* user);
+/* Makes fjo_render use `fn` for its camera and enlarger diffusion stages
+ * (NULL restores the direct convolution). */
+void fjo_plan_set_diffusion(FjoPlan* plan, FjoDiffusionFn fn, void* user);
+/* Makes fjo_render copy the C/M/Y float densities entering its grain stage
+ * into three width x height buffers (NULL disarms). */
+void fjo_plan_capture_grain_input(FjoPlan* plan, float* c, float* m, float* y);
+
 /* Builds host state for one prepared record (scanner LUT, curves). Returns
  * NULL and writes a message on unsupported input. */
 FjoPlan* fjo_plan_create(const FjPreparedHostData* prepared, char* message, size_t capacity);

@@ -58,6 +58,8 @@ def lib():
         L.fjo_highlight_boost.argtypes = [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.c_size_t]
         L.fjo_auto_exposure.restype = C.c_double
         L.fjo_auto_exposure.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t]
+        L.fjo_plan_set_diffusion.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fjo_plan_capture_grain_input.argtypes = [C.c_void_p, fp, fp, fp]
         L.fjo_film_raw.argtypes = [C.c_void_p, fp, dp, dp, dp, dp]
         L.fjo_tc_exposure.argtypes = [fp, dp, dp]
         L.fjo_print_integral.argtypes = [C.c_void_p, dp, dp]
@@ -161,6 +163,16 @@ def diffusion(descriptor, planes) -> list:
     return arrs
 
 
+def diffusion_by_fft(descriptor, planes) -> list:
+    """The same stage as diffusion(), by FFT (diffusion_fft)."""
+    arrs = [np.ascontiguousarray(p, dtype=np.float64).copy() for p in planes]
+    h, w = arrs[0].shape
+    if w < 2 or h < 2 or descriptor.radius_px <= 0:
+        raise RuntimeError("oracle diffusion rc=2")
+    diffusion_fft(C.addressof(descriptor), int(descriptor.radius_px), float(descriptor.scatter_fraction), arrs, w, h)
+    return arrs
+
+
 def diffusion_components(descriptor):
     lam = np.zeros(9)
     wts = np.zeros((3, 9))
@@ -178,6 +190,67 @@ def diffusion_psf(descriptor, channel: int) -> np.ndarray:
     if rc != 0:
         raise RuntimeError(f"oracle diffusion psf rc={rc}")
     return out
+
+
+# Worker threads of the FFT diffusion; Plan.render sets it to its own count.
+FFT_WORKERS = 1
+
+_DIFFUSION_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.c_int, C.c_int, C.c_void_p)
+
+
+def reflect101(i, n):
+    """Whole-sample mirror index (JuicerCudaDiffusion.cu:403-411)."""
+    period = 2 * (n - 1)
+    v = np.mod(i, period)
+    return np.where(v < n, v, period - v)
+
+
+def _psf_at(desc_ptr, channel: int, radius: int) -> np.ndarray:
+    """Normalized f32 PSF of one channel, (2r+1)^2, from fjo_diffusion_psf."""
+    n = 2 * radius + 1
+    psf = np.zeros(n * n, np.float32)
+    if lib().fjo_diffusion_psf(desc_ptr, channel, psf.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+        return None
+    return psf.reshape(n, n)
+
+
+def diffusion_fft(desc_ptr, radius: int, fraction: float, planes, width: int, height: int, workers: int = -1):
+    """fjo_diffusion's operator by FFT, in place: out = (1 - f) x + f (psf *
+    reflect101(x)) on three H x W double planes. The mirrored extension is
+    materialized (index maps as the direct convolution's) and convolved with
+    the radial, symmetric PSF in FP64 (scipy's pocketfft), keeping the valid
+    W x H part."""
+    import scipy.fft
+    import scipy.signal
+    rx = reflect101(np.arange(width + 2 * radius) - radius, width)
+    ry = reflect101(np.arange(height + 2 * radius) - radius, height)
+    with scipy.fft.set_workers(workers):
+        for ch in range(3):
+            psf = _psf_at(desc_ptr, ch, radius)
+            if psf is None:
+                continue
+            x = planes[ch]
+            ext = x[np.ix_(ry, rx)]
+            conv = scipy.signal.fftconvolve(ext, psf.astype(np.float64), mode="valid")
+            x *= (1.0 - fraction)
+            x += fraction * conv
+
+
+def _fft_diffusion_callback(desc_ptr, planes_ptr, width, height, user):
+    try:
+        from fj import abi
+        d = C.cast(desc_ptr, C.POINTER(abi.FjDiffusion)).contents
+        if width < 2 or height < 2 or d.radius_px <= 0:
+            return 2
+        planes = [np.ctypeslib.as_array(planes_ptr[c], shape=(height, width)) for c in range(3)]
+        diffusion_fft(desc_ptr, int(d.radius_px), float(d.scatter_fraction), planes, width, height,
+                      workers=FFT_WORKERS)
+        return 0
+    except Exception:  # a failure reaches the caller as the stage's return code
+        return 3
+
+
+_FFT_DIFFUSION = _DIFFUSION_FN(_fft_diffusion_callback)
 
 
 class Plan:
@@ -203,6 +276,20 @@ class Plan:
         sizes = [r ** 3 * 3] * 4 + [(r - 1) ** 3 * 3] * 2
         return r, [np.ctypeslib.as_array(p, shape=(n,)).copy() for p, n in zip(ptrs, sizes)]
 
+    def use_fft_diffusion(self, on: bool = True):
+        """Render the diffusion stages by FFT (diffusion_fft) instead of the
+        direct O(r^2) convolution: the same operator, feasible at the UHD
+        frame's 834 px radius."""
+        lib().fjo_plan_set_diffusion(self.handle, C.cast(_FFT_DIFFUSION, C.c_void_p) if on else None, None)
+        return self
+
+    def capture_grain_input(self, width: int, height: int):
+        """Arms a copy of the densities entering the grain stage of the next
+        render; returns the [3][H][W] float32 array it fills."""
+        self._grain_in = np.zeros((3, height, width), np.float32)
+        lib().fjo_plan_capture_grain_input(self.handle, *[_fp(self._grain_in[c]) for c in range(3)])
+        return self._grain_in
+
     def scan_lut(self, normalized_cmy):
         """Tricubic PCHIP lookup (log2 XYZ) at one normalized C/M/Y point."""
         n, out = _vec(normalized_cmy), np.zeros(3)
@@ -223,6 +310,8 @@ class Plan:
 
     def render(self, src: np.ndarray, threads: int = 1, want_dst: bool = True):
         """Returns (encoded float32 HxWxC or None, linear float64 HxWx3, error flags)."""
+        global FFT_WORKERS
+        FFT_WORKERS = max(1, int(threads))
         src = np.ascontiguousarray(src, dtype=np.float32)
         h, w, c = src.shape
         dst = np.zeros_like(src) if want_dst else None

@@ -82,6 +82,26 @@ def test_spatial_dir_with_unsharp_long_tail(linear):
     compare(got, src, dst, lin, linear)
 
 
+@pytest.mark.parametrize("shape,pitch_width,tail_um,tail_weight,max_sigma", [
+    ((640, 360), 3840, 200.0, 0.03, 60.0),     # the UHD frame's pitch: tails sigma 11.8 / 33.4 / 60.8 px
+    ((1152, 648), 8192, 400.0, 0.06, 259.0)])  # the 8192-wide frame's pitch, 400 um tail: sigma up to 259 px
+def test_spatial_dir_at_benchmark_pitch(shape, pitch_width, tail_um, tail_weight, max_sigma):
+    """Spatial DIR at the pixel pitch of the benchmarked frames, on frames
+    large enough to hold the tails, against the FP64 oracle."""
+    w, h = shape
+    pitch = 35000.0 / pitch_width
+    src = synth.frame(w, h, seed=w + 7 * h)
+    p = prepared.simple_view(prepared.load("PrintPreflash"), encode=False)
+    descriptors.spatial_dir(p.record, 20.0, tail_um, tail_weight, pitch, w, h)
+    sd = p.record.spatial_dir
+    sigmas = [sd.components[i].sigma_px for i in range(sd.component_count)]
+    assert max(sigmas) >= max_sigma, sigmas
+    got, _ = render_gpu(p, src)
+    _, lin, _ = oracle.Plan(p).render(src, threads=8)
+    err = compare(got, src, None, lin, True)
+    print(f"spatial DIR {shape} pitch {pitch:.4f} um sigmas {sigmas}: max {err.max():.3e} mean {err.mean():.3e}")
+
+
 def test_spatial_dir_moves_the_image():
     """DIR spatial tails must change the result vs point DIR (guards against a no-op filter)."""
     w, h = 160, 90

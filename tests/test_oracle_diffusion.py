@@ -111,3 +111,30 @@ def test_constant_linear_and_mirror():
             conv += psf[dy, dx] * ext[dy:dy + 29, dx:dx + 37]
     want = (1 - d.scatter_fraction) * a[0] + d.scatter_fraction * conv
     np.testing.assert_allclose(oracle.diffusion(d, a)[0], want, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("case", CASES[::3], ids=[c["output_array"] for c in CASES[::3]])
+def test_fft_path_on_cohort_images(case):
+    """The FFT form of the stage (oracle.diffusion_fft, used for frames whose
+    radius makes the direct sum infeasible) meets the same cohort bound."""
+    src = ARR[case["input_array"]].astype(np.float64)
+    want = ARR[case["output_array"]].astype(np.float64)
+    got = np.stack(oracle.diffusion_by_fft(stage_for(case), [src[..., c] for c in range(3)]), axis=-1)
+    scale = max(1.0, float(np.abs(src).max()))
+    np.testing.assert_allclose(got, want, rtol=3e-7, atol=3e-7 * scale)
+
+
+@pytest.mark.parametrize("shape,radius_scale", [((61, 37), 1.0), ((45, 23), 4.0), ((128, 96), 2.0)])
+def test_fft_path_equals_direct_sum(shape, radius_scale):
+    """FFT vs the direct FP64 sum on random planes, including radii beyond the
+    frame (the mirror repeats): agreement to FP64 rounding of the sums."""
+    w, h = shape
+    r = D.diffusion_recipe(FAMILY[META["family_order"][1]], 0.7, spatial_scale=radius_scale)
+    d = D.diffusion_stage(r, 1, 20.0, w, h)
+    rng = np.random.default_rng(w * h)
+    planes = [rng.uniform(0, 3, (h, w)) for _ in range(3)]
+    a = oracle.diffusion(d, planes)
+    b = oracle.diffusion_by_fft(d, planes)
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(y, x, rtol=0, atol=1e-12 * np.abs(x).max())
+    assert d.radius_px > 0
