@@ -189,7 +189,7 @@ FjStatus inspect_frame(const FjFrame* frame, FjCudaContext* out, FjErrorBuffer* 
 
 FjRenderOutcome render_locked(FjCuda* cuda, const FjCudaContext* context, const FjFrame* frame,
                               const FjSubmission* submission, const FjPreparedHostData* prepared,
-                              FjAbortCallback abort_callback, FjErrorBuffer* error) {
+                              FjAbortCallback abort_callback, FjErrorBuffer* error, const FjHostPreparation* host) {
     FjRenderOutcome outcome{};
     if (!cuda->accepting()) {
         outcome.status = fail(error, FJ_STATUS_PREPARATION_FAILURE, "CUDA owner admission is closed (runtime is shut down)");
@@ -224,7 +224,7 @@ FjRenderOutcome render_locked(FjCuda* cuda, const FjCudaContext* context, const 
             return outcome;
         }
     }
-    outcome = render_frame(*slot, *frame, *submission, *prepared, abort_callback, error);
+    outcome = render_frame(*slot, *frame, *submission, *prepared, abort_callback, error, host);
     if (outcome.status.category == FJ_STATUS_SUCCESS) write_error(error, "");
     return outcome;
 }

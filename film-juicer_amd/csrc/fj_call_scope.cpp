@@ -247,7 +247,6 @@ private:
     static FjRenderOutcome render(void* user, const FjCudaContext* context, const FjFrame* frame,
                                   const FjSubmission* submission, const FjPreparedHostData* prepared,
                                   const FjHostPreparation* host, FjAbortCallback abort_callback, FjErrorBuffer* err) {
-        (void)host;  // the scanner LUT and Gaussian weights are built natively
         FjRenderOutcome outcome{};
         CallScope* s = from(user);
         if (!s) {
@@ -266,7 +265,7 @@ private:
         try {
             char text[2048];
             FjErrorBuffer local{text, sizeof(text), 0};
-            outcome = render_locked(s->owner_, context, frame, submission, prepared, abort_callback, &local);
+            outcome = render_locked(s->owner_, context, frame, submission, prepared, abort_callback, &local, host);
             s->diagnostic_.assign(text, local.length);
             write_error(err, s->diagnostic_);
             if (outcome.status.category == FJ_STATUS_CONTEXT_LOSS) s->recovery_pending_ = true;

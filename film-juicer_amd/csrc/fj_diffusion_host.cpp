@@ -10,6 +10,7 @@
 // exponentials rounded to float, divided by the FP64 sum of the rounded
 // samples.
 #include "fj_diffusion_host.h"
+#include "fj_host_imports.h"
 
 #include <algorithm>
 #include <cmath>
@@ -188,8 +189,9 @@ bool diffusion_decode(const FjOptics& o, bool print_route, std::string& field) {
 
 bool diffusion_quadrant(const FjDiffusion& d, int channel, std::vector<double>& q) {
     double lam[kDiffusionComponents], w[3][kDiffusionComponents];
-    std::string field;
-    if (channel < 0 || channel > 2 || !diffusion_components(d, lam, w, field)) return false;
+    uint32_t category = 0;
+    std::string diag;
+    if (channel < 0 || channel > 2 || !stage_components(d, lam, w, category, diag)) return false;
     const int r = d.radius_px, n = r + 1;
     const double two_pi = 2.0 * kPi;
     q.assign(static_cast<size_t>(n) * n, 0.0);

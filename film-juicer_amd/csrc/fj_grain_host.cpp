@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "fj_detmath.h"
+#include "fj_host.h"
 #include "fj_spatial.h"
 
 #pragma clang fp contract(off)
@@ -347,7 +348,7 @@ hipError_t grain_prepare(DeviceState& dev, const FjPreparedHostData& p, GrainPla
     auto kernel = [&](const FjGrainGaussian& gg, GrainKernel& k) -> hipError_t {
         k = GrainKernel{};
         if (gg.radius <= 0) return hipSuccess;
-        std::vector<float> w = gaussian_weights(gg.radius, gg.sigma_px);
+        std::vector<float> w = kernel_weights(gg.radius, gg.sigma_px);
         if (w.empty()) return hipErrorInvalidValue;
         const void* wp = nullptr;
         const hipError_t r = dev.table(hash_bytes(w.data(), w.size() * 4, 0x6A55ull), w.data(), w.size() * 4, &wp);

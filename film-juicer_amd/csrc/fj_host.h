@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/fj/juicer_cuda_api.h"
+#include "../../include/fj/juicer_cuda_call_scope.h"
 #include "fj_device.h"
 
 namespace fjx {
@@ -33,6 +34,25 @@ struct ScannerLut {
 // ((c * r + m) * r + y) and channel.
 bool build_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& diagnostic,
                        uint64_t* fail_node = nullptr, uint32_t* fail_channel = nullptr);
+
+// The frame's host preparation (FjHostPreparation of a call scope's render):
+// while a scope object lives on a thread, kernel_weights and
+// frame_scanner_lut use the host's callbacks when it supplied them, and a
+// host failure is kept for the frame to report (take_host_failure).
+class HostPreparationScope {
+public:
+    explicit HostPreparationScope(const FjHostPreparation* host);
+    ~HostPreparationScope();
+    HostPreparationScope(const HostPreparationScope&) = delete;
+    HostPreparationScope& operator=(const HostPreparationScope&) = delete;
+
+private:
+    const FjHostPreparation* previous_;
+};
+bool take_host_failure(uint32_t& category, std::string& text);
+// Normalized Gaussian FIR weights (empty on failure).
+std::vector<float> kernel_weights(int radius, float sigma_px);
+bool frame_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& diagnostic);
 
 // kernels (fj_pixel.hip)
 // Persistent launch: each kernel gets the blocks that are resident at once on

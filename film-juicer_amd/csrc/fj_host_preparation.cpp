@@ -136,3 +136,148 @@ FJ_EXPORT FjStatus fjx_prepare_gaussian(void* /*user*/, const FjGaussianKernelIn
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The frame's host preparation: when a call scope's render supplies
+// FjHostPreparation callbacks, the frame's scanner LUT and Gaussian kernels
+// come from them (native/juicer_host_preparation.cpp:28-109); without them
+// the builds above are used.
+// ---------------------------------------------------------------------------
+namespace fjx {
+
+namespace {
+
+thread_local const FjHostPreparation* t_host = nullptr;
+thread_local bool t_failed = false;
+thread_local uint32_t t_category = 0;
+thread_local std::string t_text;
+
+void set_failure(uint32_t category, const std::string& text) {
+    if (t_failed) return;  // the first failure of the frame is reported
+    t_failed = true;
+    t_category = category;
+    t_text = text;
+}
+
+bool text_valid(const FjErrorBuffer& error, const char* text, size_t capacity) {
+    return error.length < capacity && text[error.length] == '\0';
+}
+
+bool span_valid(const FjFloatSpan& s, size_t count) {
+    return count && s.count == count && s.data && reinterpret_cast<uintptr_t>(s.data) % alignof(float) == 0;
+}
+
+}  // namespace
+
+HostPreparationScope::HostPreparationScope(const FjHostPreparation* host) : previous_(t_host) {
+    t_host = host;
+    t_failed = false;
+}
+
+HostPreparationScope::~HostPreparationScope() { t_host = previous_; }
+
+bool take_host_failure(uint32_t& category, std::string& text) {
+    if (!t_failed) return false;
+    t_failed = false;
+    category = t_category;
+    text = t_text;
+    return true;
+}
+
+std::vector<float> kernel_weights(int radius, float sigma_px) {
+    if (!t_host) return gaussian_weights(radius, sigma_px);
+    const FjHostPreparation& hp = *t_host;
+    auto invalid = [&]() {
+        set_failure(FJ_STATUS_PREPARATION_FAILURE, "Gaussian kernel projection invalid");
+        return std::vector<float>{};
+    };
+    if (!hp.user || !hp.scanner_lut || !hp.gaussian) return invalid();
+    FjGaussianKernelInput in{radius, sigma_px};
+    FjGaussianKernelView view{};
+    FjGaussianKernelFailure failure{};
+    char text[256] = {};
+    FjErrorBuffer error{text, sizeof(text), 0};
+    const FjStatus st = hp.gaussian(hp.user, &in, &view, &failure, &error);
+    static const uint32_t kCategories[6] = {FJ_STATUS_SUCCESS,             FJ_STATUS_UNSUPPORTED_INPUT,
+                                            FJ_STATUS_PREPARATION_FAILURE, FJ_STATUS_PREPARATION_FAILURE,
+                                            FJ_STATUS_ALLOCATION_FAILURE,  FJ_STATUS_INTERNAL_FAILURE};
+    if (!text_valid(error, text, sizeof(text)) || st.api != FJ_API_NONE || st.native_code != 0 || failure.reason >= 6 ||
+        st.category != kCategories[failure.reason])
+        return invalid();
+    if (st.category != FJ_STATUS_SUCCESS) {
+        uint32_t sigma_bits = 0;
+        std::memcpy(&sigma_bits, &view.sigma_px, 4);
+        if (view.radius || sigma_bits || view.weights.data || view.weights.count) return invalid();
+        set_failure(st.category, std::string(text, error.length));
+        return {};
+    }
+    const size_t count = radius >= 1 && radius <= 2048 ? static_cast<size_t>(radius) * 2 + 1 : 0;
+    if (error.length || view.radius != radius || std::memcmp(&view.sigma_px, &sigma_px, 4) != 0 ||
+        !span_valid(view.weights, count))
+        return invalid();
+    return std::vector<float>(view.weights.data, view.weights.data + count);
+}
+
+bool frame_scanner_lut(const ScannerLutInput& in, ScannerLut& out, std::string& diagnostic) {
+    if (!t_host) return build_scanner_lut(in, out, diagnostic);
+    const FjHostPreparation& hp = *t_host;
+    auto invalid = [&]() {
+        set_failure(FJ_STATUS_INTERNAL_FAILURE, "Scanner LUT preparation: invalid transport contract");
+        diagnostic = "invalid transport contract";
+        return false;
+    };
+    if (!hp.user || !hp.scanner_lut || !hp.gaussian) return invalid();
+    const size_t k = in.sample_count > 0 ? static_cast<size_t>(in.sample_count) : 0;
+    auto sp = [k](const float* p) { return FjFloatSpan{p, p ? k : 0}; };
+    FjScannerLutInput ci{};
+    ci.eps_c = sp(in.eps_c);
+    ci.eps_m = sp(in.eps_m);
+    ci.eps_y = sp(in.eps_y);
+    ci.weighted_x = sp(in.weighted_x);
+    ci.weighted_y = sp(in.weighted_y);
+    ci.weighted_z = sp(in.weighted_z);
+    ci.baseline_min = sp(in.baseline);
+    for (int c = 0; c < 3; ++c) {
+        ci.range_min_cmy[c] = in.range_min_cmy[c];
+        ci.inverse_span_cmy[c] = in.inverse_span_cmy[c];
+    }
+    ci.sample_count = in.sample_count;
+    ci.inv_yn = in.inv_yn;
+    ci.has_baseline = in.baseline ? 1u : 0u;
+    ci.medium = static_cast<uint32_t>(in.medium);
+    ci.resolution = static_cast<uint32_t>(in.resolution);
+    FjScannerLutView view{};
+    FjScannerLutFailure failure{};
+    char text[256] = {};
+    FjErrorBuffer error{text, sizeof(text), 0};
+    const FjStatus st = hp.scanner_lut(hp.user, &ci, &view, &failure, &error);
+    if (!text_valid(error, text, sizeof(text)) || st.api != FJ_API_NONE || st.native_code != 0) return invalid();
+    const FjFloatSpan arrays[6] = {view.log2_xyz, view.slope_c, view.slope_m, view.slope_y, view.cell_min, view.cell_max};
+    if (st.category != FJ_STATUS_SUCCESS) {
+        const uint32_t r = failure.reason;
+        const uint32_t want = r == FJ_SCANNER_LUT_REASON_RAW                                            ? FJ_STATUS_UNSUPPORTED_INPUT
+                              : r == FJ_SCANNER_LUT_REASON_SIZE || r == FJ_SCANNER_LUT_REASON_ALLOCATION ? FJ_STATUS_ALLOCATION_FAILURE
+                              : r == FJ_SCANNER_LUT_REASON_PANIC                                         ? FJ_STATUS_INTERNAL_FAILURE
+                                                                                                         : FJ_STATUS_PREPARATION_FAILURE;
+        bool empty = view.resolution == 0;
+        for (const FjFloatSpan& a : arrays) empty = empty && !a.data && !a.count;
+        const bool located = r == FJ_SCANNER_LUT_REASON_NONFINITE ? failure.channel <= 2 && failure.index != UINT64_MAX
+                                                                   : failure.channel == UINT32_MAX && failure.index == UINT64_MAX;
+        if (r < 1 || r > 6 || st.category != want || !empty || !located) return invalid();
+        set_failure(st.category, std::string(text, error.length));
+        diagnostic = std::string(text, error.length);
+        return false;
+    }
+    if (error.length || failure.reason || failure.channel != UINT32_MAX || failure.index != UINT64_MAX || failure.buffer ||
+        view.resolution != ci.resolution || ci.resolution < 2)
+        return invalid();
+    const size_t r = ci.resolution, nodes = r * r * r * 3, cells = (r - 1) * (r - 1) * (r - 1) * 3;
+    for (int i = 0; i < 6; ++i)
+        if (!span_valid(arrays[i], i < 4 ? nodes : cells)) return invalid();
+    out.resolution = static_cast<int>(r);
+    std::vector<float>* dst[6] = {&out.log2_xyz, &out.slope_c, &out.slope_m, &out.slope_y, &out.cell_min, &out.cell_max};
+    for (int i = 0; i < 6; ++i) dst[i]->assign(arrays[i].data, arrays[i].data + arrays[i].count);
+    return true;
+}
+
+}  // namespace fjx

@@ -120,7 +120,8 @@ FjStatus inspect_frame(const FjFrame* frame, FjCudaContext* out, FjErrorBuffer* 
 // fj_cuda_render's work with the owner's lock already held.
 FjRenderOutcome render_locked(FjCuda* cuda, const FjCudaContext* context, const FjFrame* frame,
                               const FjSubmission* submission, const FjPreparedHostData* prepared,
-                              FjAbortCallback abort_callback, FjErrorBuffer* error);
+                              FjAbortCallback abort_callback, FjErrorBuffer* error,
+                              const FjHostPreparation* host = nullptr);
 // Releases the native state of one device (context-loss retirement) with the
 // owner's lock held; false when releasing it reported an error.
 bool retire_device_locked(FjCuda* cuda, int32_t device_id);

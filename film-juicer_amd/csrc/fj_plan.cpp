@@ -7,6 +7,7 @@
 #include "fj_grain.h"
 #include "fj_grain_host.h"
 #include "fj_halation_host.h"
+#include "fj_host_imports.h"
 
 #include <algorithm>
 #include <cstddef>
@@ -487,6 +488,15 @@ bool check_spatial_dir(const FjPreparedHostData& p, const FjFrame& f, int max_li
     FftPlanDev plan;
     if (w > max_line || h > max_line || !make_fft_plan(w, plan) || !make_fft_plan(h, plan))
         return d.unsupported("spatial DIR frame side above the LDS line limit");
+    // A host that defines the boundary functions certifies each recursive
+    // component's reflected boundary on this frame.
+    uint32_t category = 0;
+    std::string diag;
+    if (!host_dir_boundaries(sd, w, h, category, diag)) {
+        d.category = category;
+        d.text = diag;
+        return false;
+    }
     return true;
 }
 
@@ -628,7 +638,7 @@ bool pack_scanner_lut(const FjPreparedHostData& p, PackedLut& out, std::string& 
     in.medium = print ? 1 : 0;
     in.resolution = static_cast<int>(p.scanner_lut.resolution);
     ScannerLut lut;
-    if (!build_scanner_lut(in, lut, diag)) return false;
+    if (!frame_scanner_lut(in, lut, diag)) return false;
     const int r = lut.resolution;
     const size_t nodes = size_t(r) * r * r;
     out.res = r;
@@ -789,7 +799,15 @@ hipError_t assemble(DeviceState& dev, const FjPreparedHostData& p, PixelParams& 
         PackedLut lut;
         std::string diag;
         if (!pack_scanner_lut(p, lut, diag)) {
-            d.text = "PreparationFailure " + diag;
+            uint32_t category = 0;
+            std::string text;
+            if (take_host_failure(category, text)) {
+                d.category = category;
+                d.text = text;
+            } else {
+                d.category = FJ_STATUS_PREPARATION_FAILURE;
+                d.text = "PreparationFailure " + diag;
+            }
             return hipSuccess;
         }
         e = dev.table(node_key, lut.nodes.data(), lut.nodes.size() * 4, &ptr);
@@ -872,6 +890,15 @@ FjRenderOutcome outcome_diag(uint32_t category, const std::string& text, FjError
     return o;
 }
 
+// The host preparation's failure when a builder that used it failed, else
+// the given one.
+FjRenderOutcome host_or(uint32_t category, const std::string& text, FjErrorBuffer* error) {
+    uint32_t host_category = 0;
+    std::string host_text;
+    if (take_host_failure(host_category, host_text)) return outcome_diag(host_category, host_text, error);
+    return outcome_diag(category, text, error);
+}
+
 // check_abort (JuicerCudaExecutor.cpp:2544-2559), polled before the frame's
 // preparation, after it, and after its launches are queued (:2816, :2871,
 // :3125). True when the frame ends here with `out`.
@@ -892,7 +919,11 @@ bool aborted(const FjAbortCallback& cb, FjRenderOutcome& out, FjErrorBuffer* err
 }  // namespace
 
 FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSubmission& submission,
-                             const FjPreparedHostData& prepared, FjAbortCallback abort_callback, FjErrorBuffer* error) {
+                             const FjPreparedHostData& prepared, FjAbortCallback abort_callback, FjErrorBuffer* error,
+                             const FjHostPreparation* host) {
+    // Scanner LUT and Gaussian kernels of this frame come from the host's
+    // callbacks when it supplied them.
+    HostPreparationScope host_scope(host);
     Diag d;
     if (!validate_prepared(prepared, d) || !validate_binding(prepared, frame, submission, d))
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, d.text, error);
@@ -908,8 +939,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     bool ok = false;
     hipError_t e = assemble(dev, prepared, pp, d, ok);
     if (e != hipSuccess) return outcome_hip(e, "table upload", error);
-    if (!ok) return outcome_diag(d.text.rfind("Preparation", 0) == 0 ? FJ_STATUS_PREPARATION_FAILURE : FJ_STATUS_UNSUPPORTED_INPUT,
-                                 d.text, error);
+    if (!ok) return outcome_diag(d.category, d.text, error);
 
     const FjRect& w = frame.render_window;
     const FjImage& s = frame.source;
@@ -970,9 +1000,9 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     for (const FjDiffusion* dd : diff_desc) {
         if (!dd) continue;
         double lam[kDiffusionComponents], wts[3][kDiffusionComponents];
-        std::string field;
-        if (!diffusion_components(*dd, lam, wts, field))
-            return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "InvalidDiffusionHostBehavior field=" + field, error);
+        uint32_t category = 0;
+        std::string diag;
+        if (!stage_components(*dd, lam, wts, category, diag)) return outcome_diag(category, diag, error);
         if (pp.io.width < 2 || pp.io.height < 2)
             return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput diffusion frame side below 2", error);
         int pw = 0, ph = 0;
@@ -989,12 +1019,33 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         bool gok = false, rejected = false;
         std::string gdiag;
         e = grain_prepare(dev, prepared, gp, gok, gdiag, rejected);
+        {
+            uint32_t category = 0;
+            std::string text;
+            if (take_host_failure(category, text)) return outcome_diag(category, text, error);
+        }
         if (e != hipSuccess) return outcome_hip(e, "grain tables", error);
         if (rejected) return outcome_hip(hipErrorInvalidValue, "print visual grain launch", error);
         if (!gok)
             return outcome_diag(gdiag.rfind("UnsupportedInput", 0) == 0 ? FJ_STATUS_UNSUPPORTED_INPUT
                                                                         : FJ_STATUS_PREPARATION_FAILURE,
                                 gdiag, error);
+    }
+
+    // Scanner-post Gaussian kernels, built before any launch so that a
+    // refused kernel leaves the frame untouched.
+    static const float kIdentityTap = 1.0f;
+    std::vector<float> lens_w, unsharp_w{kIdentityTap};
+    if (post) {
+        const FjScannerPostEffects& sp = prepared.scanner_post;
+        if (lens_active(sp)) {
+            lens_w = kernel_weights(sp.lens_blur_radius, sp.lens_blur_sigma_px);
+            if (lens_w.empty()) return host_or(FJ_STATUS_PREPARATION_FAILURE, "gaussian descriptor is invalid", error);
+        }
+        if (unsharp_active(sp)) {
+            unsharp_w = kernel_weights(sp.unsharp_radius, sp.unsharp_sigma_px);
+            if (unsharp_w.empty()) return host_or(FJ_STATUS_PREPARATION_FAILURE, "gaussian descriptor is invalid", error);
+        }
     }
 
     HalationStages hs;
@@ -1200,7 +1251,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
             HostTransfer ht;
             std::string diag;
             if (!dir_transfer(prepared.spatial_dir, pp.io.width, pp.io.height, ht, diag))
-                return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "MalformedPreparedHostData field=" + diag, error);
+                return host_or(FJ_STATUS_UNSUPPORTED_INPUT, "MalformedPreparedHostData field=" + diag, error);
             DctTransfer dt;
             e = upload_transfer(dev, ht, tkey, dt);
             if (e != hipSuccess) return outcome_hip(e, "DIR transfer upload", error);
@@ -1263,8 +1314,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         const FjScannerPostEffects& sp = prepared.scanner_post;
         const long long np = static_cast<long long>(pp.io.width) * pp.io.height;
         if (lens_active(sp)) {
-            std::vector<float> lw = gaussian_weights(sp.lens_blur_radius, sp.lens_blur_sigma_px);
-            if (lw.empty()) return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "gaussian descriptor is invalid", error);
+            const std::vector<float>& lw = lens_w;
             const void* lptr = nullptr;
             e = dev.table(hash_bytes(lw.data(), lw.size() * 4, 0x4C42ull), lw.data(), lw.size() * 4, &lptr);
             if (e != hipSuccess) return outcome_hip(e, "lens blur weights", error);
@@ -1287,10 +1337,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
             }
             if (e != hipSuccess) return outcome_hip(e, "lens blur", error);
         }
-        static const float kIdentityTap = 1.0f;
-        std::vector<float> w = unsharp_active(sp) ? gaussian_weights(sp.unsharp_radius, sp.unsharp_sigma_px)
-                                                  : std::vector<float>{kIdentityTap};
-        if (w.empty()) return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "gaussian descriptor is invalid", error);
+        const std::vector<float>& w = unsharp_w;
         const void* wptr = nullptr;
         e = dev.table(hash_bytes(w.data(), w.size() * 4, 0x5057ull), w.data(), w.size() * 4, &wptr);
         if (e != hipSuccess) return outcome_hip(e, "unsharp weights", error);
@@ -1496,9 +1543,9 @@ FjStatus diffusion_apply_planes(DeviceState& dev, const FjDiffusion& stage, floa
     if (width < 2 || height < 2 || !planes[0] || !planes[1] || !planes[2])
         return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput diffusion frame side below 2", error).status;
     double lam[kDiffusionComponents], wts[3][kDiffusionComponents];
-    std::string field;
-    if (!diffusion_components(stage, lam, wts, field))
-        return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "InvalidDiffusionHostBehavior field=" + field, error).status;
+    uint32_t category = 0;
+    std::string diag;
+    if (!stage_components(stage, lam, wts, category, diag)) return outcome_diag(category, diag, error).status;
     int pw = 0, ph = 0;
     diffusion_padded(stage, width, height, pw, ph);
     if (pw == 0 || ph == 0 || pw > dev.max_line || ph > dev.max_line)

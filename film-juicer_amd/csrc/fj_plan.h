@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/fj/juicer_cuda_api.h"
+#include "../../include/fj/juicer_cuda_call_scope.h"
 #include "fj_device.h"
 #include "fj_fft.h"
 
@@ -88,9 +89,10 @@ struct DeviceState {
     unsigned int drain_errors(bool wait_all);
 };
 
+// `host`: the call scope's host preparation, or null (native builds).
 FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSubmission& submission,
                              const FjPreparedHostData& prepared, FjAbortCallback abort_callback,
-                             FjErrorBuffer* error);
+                             FjErrorBuffer* error, const FjHostPreparation* host = nullptr);
 
 // Renders `count` frames of one device, frame i on worker stream i % streams
 // with workspace slot i % streams, all ordered after the work already queued

@@ -15,6 +15,8 @@
 #include <cstring>
 #include <vector>
 
+#include "fj_host.h"
+
 namespace fjx {
 
 std::vector<float> gaussian_weights(int radius, float sigma_px) {
@@ -86,7 +88,7 @@ bool dir_transfer(const FjSpatialDir& sd, int width, int height, HostTransfer& o
                     diag = "spatial_dir.component.fir";
                     return false;
                 }
-                const std::vector<float> w = gaussian_weights(g.radius, g.sigma_px);
+                const std::vector<float> w = kernel_weights(g.radius, g.sigma_px);
                 fir_eigenvalues(w, width, tx.data());
                 fir_eigenvalues(w, height, ty.data());
                 break;

@@ -337,5 +337,21 @@ FjCudaCallAccess = _s("FjCudaCallAccess", [
 FjCudaCompletion = _s("FjCudaCompletion", [
     ("result", FjCallCompletionResult), ("retirement", u32), ("invalidation", FjStatus), ("latch_cleared", u32)])
 
+# Optional host imports (juicer_diffusion_api.h, juicer_dir_boundary_api.h).
+FjDiffusionSample = _s("FjDiffusionSample", [
+    ("family", u32), ("group_weights", f64 * 3), ("group_lambda_um", f64 * 3), ("warmth", f64),
+    ("spatial_scale", f64), ("pixel_size_um", f64), ("radius_px", i32), ("hash", u64)])
+FjDiffusionComponents = _s("FjDiffusionComponents", [
+    ("lambda_px", f64 * 9), ("red_weights", f64 * 9), ("green_weights", f64 * 9), ("blue_weights", f64 * 9)])
+FjDiffusionFailure = _s("FjDiffusionFailure", [("reason", u32), ("stage", u32), ("field", u32)])
+FjDirBoundaryInput = _s("FjDirBoundaryInput", [("component", FjDirGaussian), ("width", i32), ("height", i32)])
+FjDirBoundaryView = _s("FjDirBoundaryView", [
+    ("weights", C.POINTER(f64)), ("count", size_t), ("identity", u64), ("width", i32), ("height", i32),
+    ("horizontal_length", i32), ("vertical_length", i32), ("horizontal_terminal_size", i32),
+    ("vertical_terminal_size", i32), ("horizontal_terminal", f64 * 9), ("vertical_terminal", f64 * 9)])
+FjDirBoundaryFailure = _s("FjDirBoundaryFailure", [
+    ("reason", u32), ("axis", u32), ("row", u32), ("length", i32), ("sigma_px", f32), ("sum", f64),
+    ("tolerance", f64)])
+
 ALL_TYPES = {name: obj for name, obj in globals().items()
              if isinstance(obj, type) and issubclass(obj, C.Structure)}

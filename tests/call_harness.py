@@ -11,6 +11,9 @@ LIB_PATH = os.path.join(HERE, "native", "libfj_call_harness.so")
 PRECHECK, ADMIT, INSPECT, DESCRIPTORS, PREVIEW, RENDER, RELEASE, COMPLETE = 1, 2, 3, 4, 5, 6, 7, 8
 PUBLIC_INSPECT, TRACE, RENDER_OTHER_CONTEXT, PUBLIC_DESTROY, PUBLIC_SHUTDOWN = 9, 10, 11, 12, 13
 MAX_OPS, TEXT = 16, 256
+# Host preparation behaviours of the driver's callbacks (fj_call_harness.c).
+HOST_NONE, HOST_DELEGATE, HOST_GAUSSIAN_FAIL, HOST_GAUSSIAN_HALF = 0, 1, 2, 3
+HOST_LUT_FAIL, HOST_GAUSSIAN_BROKEN, HOST_LUT_BROKEN = 4, 5, 6
 
 
 class Script(C.Structure):
@@ -21,12 +24,14 @@ class Script(C.Structure):
         ("submission", C.POINTER(abi.FjSubmission)), ("preview_source", abi.FjRect),
         ("preview_meter", abi.FjRect), ("preview_method", C.c_uint32), ("recovery_scope_entered", C.c_uint32),
         ("force_fatal", C.c_uint32), ("retire_on_completion", C.c_uint32), ("report_context_loss", C.c_uint32),
+        ("host_mode", C.c_uint32),
         ("status", abi.FjStatus * MAX_OPS), ("text", (C.c_char * TEXT) * MAX_OPS),
         ("context", abi.FjCudaContext), ("descriptors", abi.FjNativeFrameDescriptors),
         ("preview", abi.FjAutoExposure), ("outcome", abi.FjRenderOutcome),
         ("completed", abi.FjCallCompletionResult), ("completion_calls", C.c_uint32),
         ("completion_seen", abi.FjCallCompletionInput), ("completion_diagnostic", C.c_char * TEXT),
-        ("retirement", C.c_uint32), ("call_result", abi.FjCallCompletionResult)]
+        ("retirement", C.c_uint32), ("call_result", abi.FjCallCompletionResult),
+        ("host_calls", C.c_uint32 * 2)]
 
     def op_status(self, i):
         return self.status[i].category

@@ -8,6 +8,9 @@ Sources (all data files held by the reference's tests):
       x86-64 sizes/offsets/values of every FjPreparedHostData record.
   * tests/ffi/prepared_call_abi_facts.inc              -> call_abi_facts.json
       x86-64 sizes/offsets/values of the call-scope and frame-value records.
+  * tests/ffi/prepared-values.c                        -> boundary_abi_facts.json
+      C11 layout assertions of the diffusion-components and DIR-boundary
+      import records (juicer_diffusion_api.h, juicer_dir_boundary_api.h).
   * tests/ffi/fixtures/prepared-state-projection/linux-release.json
       -> prepared/<case>.npz + prepared/<case>.json
       Complete prepared products (exact f32 words) for named recipes:
@@ -268,11 +271,30 @@ def diffusion_fixtures() -> None:
         json.dump(meta, fh, indent=0)
 
 
+BOUNDARY_TYPES = ("FjDiffusionSample", "FjDiffusionComponents", "FjDiffusionFailure", "FjDirBoundaryInput",
+                  "FjDirBoundaryView", "FjDirBoundaryFailure")
+
+
+def boundary_abi_facts() -> None:
+    """Layouts of the diffusion-components and DIR-boundary import records,
+    from the reference's C11 layout assertions (tests/ffi/prepared-values.c)
+    -> boundary_abi_facts.json."""
+    text = open(f"{REF}/tests/ffi/prepared-values.c").read()
+    facts = {"types": {}, "fields": []}
+    for name in BOUNDARY_TYPES:
+        m = re.search(r"sizeof\(%s\) == (\d+) && _Alignof\(%s\) == (\d+)" % (name, name), text)
+        facts["types"][name] = [int(m.group(1)), int(m.group(2))]
+        for f, off in re.findall(r"offsetof\(%s, (\w+)\) == (\d+)" % name, text):
+            facts["fields"].append([name, f, int(off)])
+    json.dump(facts, open(os.path.join(HERE, "boundary_abi_facts.json"), "w"), indent=1)
+
+
 if __name__ == "__main__":
     what = sys.argv[1:] or ["abi", "prepared", "lut", "dir", "halation", "diffusion"]
     if "abi" in what:
         abi_facts()
         abi_facts("prepared_call_abi_facts.inc", "call_abi_facts.json")
+        boundary_abi_facts()
     if "prepared" in what:
         prepared_cases()
     if "lut" in what:

@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include "fj/fj_mi355x_ext.h"
 #include "fj/juicer_cuda_call_scope.h"
 
 enum {
@@ -45,6 +46,7 @@ typedef struct FjhScript {
     uint32_t force_fatal;
     uint32_t retire_on_completion; /* call fj_cuda_call_retire_context from the completion */
     uint32_t report_context_loss;  /* the completion input reports FJ_STATUS_CONTEXT_LOSS */
+    uint32_t host_mode;            /* FJH_HOST_*: the host preparation passed to render */
     /* results */
     FjStatus status[FJH_MAX_OPS];
     char text[FJH_MAX_OPS][FJH_TEXT];
@@ -58,7 +60,78 @@ typedef struct FjhScript {
     char completion_diagnostic[FJH_TEXT];
     uint32_t retirement;
     FjCallCompletionResult call_result;
+    uint32_t host_calls[2]; /* scanner LUT, Gaussian callbacks made */
 } FjhScript;
+
+/* Host preparation callbacks (the reference's Rust side builds the scanner
+ * LUT and Gaussian kernels; here the library's own builders stand in):
+ *   NONE: no host preparation (native builds);
+ *   DELEGATE: both callbacks return the builders' results;
+ *   GAUSSIAN_FAIL: the Gaussian callback refuses (sigma, preparation failure);
+ *   GAUSSIAN_HALF: the Gaussian callback returns the weights halved;
+ *   LUT_FAIL: the scanner-LUT callback refuses (non-finite node);
+ *   GAUSSIAN_BROKEN: the Gaussian view names another radius;
+ *   LUT_BROKEN: the scanner-LUT view names another resolution. */
+enum {
+    FJH_HOST_NONE = 0,
+    FJH_HOST_DELEGATE = 1,
+    FJH_HOST_GAUSSIAN_FAIL = 2,
+    FJH_HOST_GAUSSIAN_HALF = 3,
+    FJH_HOST_LUT_FAIL = 4,
+    FJH_HOST_GAUSSIAN_BROKEN = 5,
+    FJH_HOST_LUT_BROKEN = 6,
+};
+
+#define FJH_MAX_TAPS 4097
+static float g_half[FJH_MAX_TAPS];
+
+static void fjh_put(FjErrorBuffer* e, const char* text) {
+    size_t n = strlen(text);
+    if (n > e->capacity - 1) n = e->capacity - 1;
+    memcpy(e->data, text, n);
+    e->data[n] = '\0';
+    e->length = n;
+}
+
+static FjStatus fjh_host_lut(void* user, const FjScannerLutInput* in, FjScannerLutView* view,
+                             FjScannerLutFailure* failure, FjErrorBuffer* error) {
+    FjhScript* s = (FjhScript*)user;
+    s->host_calls[0]++;
+    if (s->host_mode == FJH_HOST_LUT_FAIL) {
+        memset(view, 0, sizeof(*view));
+        failure->reason = FJ_SCANNER_LUT_REASON_NONFINITE;
+        failure->channel = 1;
+        failure->index = 7;
+        failure->buffer = 0;
+        fjh_put(error, "host scanner LUT refused: non-finite node");
+        FjStatus st = {FJ_STATUS_PREPARATION_FAILURE, FJ_API_NONE, 0};
+        return st;
+    }
+    FjStatus st = fjx_prepare_scanner_lut(NULL, in, view, failure, error);
+    if (st.category == FJ_STATUS_SUCCESS && s->host_mode == FJH_HOST_LUT_BROKEN) view->resolution += 1;
+    return st;
+}
+
+static FjStatus fjh_host_gaussian(void* user, const FjGaussianKernelInput* in, FjGaussianKernelView* view,
+                                  FjGaussianKernelFailure* failure, FjErrorBuffer* error) {
+    FjhScript* s = (FjhScript*)user;
+    s->host_calls[1]++;
+    if (s->host_mode == FJH_HOST_GAUSSIAN_FAIL) {
+        memset(view, 0, sizeof(*view));
+        failure->reason = FJ_GAUSSIAN_KERNEL_SIGMA;
+        fjh_put(error, "host gaussian refused: sigma");
+        FjStatus st = {FJ_STATUS_PREPARATION_FAILURE, FJ_API_NONE, 0};
+        return st;
+    }
+    FjStatus st = fjx_prepare_gaussian(NULL, in, view, failure, error);
+    if (st.category != FJ_STATUS_SUCCESS) return st;
+    if (s->host_mode == FJH_HOST_GAUSSIAN_BROKEN) view->radius += 1;
+    if (s->host_mode == FJH_HOST_GAUSSIAN_HALF && view->weights.count <= FJH_MAX_TAPS) {
+        for (size_t i = 0; i < view->weights.count; ++i) g_half[i] = 0.5f * view->weights.data[i];
+        view->weights.data = g_half;
+    }
+    return st;
+}
 
 typedef struct FjhState {
     FjCuda* cuda;
@@ -118,7 +191,9 @@ static FjCallCompletionResult fjh_continuation(void* user, FjCudaCallAccess a) {
                 FjAbortCallback abort_cb = {NULL, NULL};
                 FjCudaContext ctx = s->context;
                 if (s->ops[i] == FJH_RENDER_OTHER_CONTEXT) ctx.context += 1000;
-                s->outcome = a.render(a.user, &ctx, s->frame, s->submission, s->prepared, NULL, abort_cb, &err);
+                FjHostPreparation host = {s, fjh_host_lut, fjh_host_gaussian};
+                s->outcome = a.render(a.user, &ctx, s->frame, s->submission, s->prepared,
+                                      s->host_mode == FJH_HOST_NONE ? NULL : &host, abort_cb, &err);
                 r = s->outcome.status;
                 rendered = 1;
                 break;

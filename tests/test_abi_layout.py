@@ -95,3 +95,30 @@ def test_call_scope_header_matches_reference_facts():
     for name in ("FjDirFrameMath", "FjGrainFrameMath", "FjEffectsFrameMath"):
         assert ctypes.sizeof(abi.ALL_TYPES[name]) == {"FjDirFrameMath": 512, "FjGrainFrameMath": 552,
                                                       "FjEffectsFrameMath": 208}[name]
+
+
+# Import records of the optional host functions (the reference's C11 layout
+# assertions in tests/ffi/prepared-values.c, extracted to
+# tests/golden/boundary_abi_facts.json).
+BOUNDARY_FACTS = json.load(open(os.path.join(HERE, "golden", "boundary_abi_facts.json")))
+
+
+def test_boundary_import_records_match_reference_facts():
+    for name, (size, align) in BOUNDARY_FACTS["types"].items():
+        t = abi.ALL_TYPES[name]
+        assert ctypes.sizeof(t) == size and ctypes.alignment(t) == align, name
+    for name, field, off in BOUNDARY_FACTS["fields"]:
+        assert getattr(abi.ALL_TYPES[name], field).offset == off, (name, field)
+    assert len(BOUNDARY_FACTS["types"]) == 6 and len(BOUNDARY_FACTS["fields"]) == 36
+    lines = ['#include <stddef.h>', '#include "fj/juicer_diffusion_api.h"', '#include "fj/juicer_dir_boundary_api.h"']
+    for name, (size, align) in BOUNDARY_FACTS["types"].items():
+        lines.append(f'_Static_assert(sizeof({name}) == {size} && _Alignof({name}) == {align}, "{name}");')
+    for name, field, off in BOUNDARY_FACTS["fields"]:
+        lines.append(f'_Static_assert(offsetof({name}, {field}) == {off}, "{name}.{field}");')
+    lines.append("int main(void) { return 0; }")
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "abi.c")
+        open(src, "w").write("\n".join(lines))
+        r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", src,
+                            "-o", os.path.join(d, "abi.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-4000:]

@@ -2,6 +2,7 @@
 its owner/argument contract without a GPU (no compute is issued here)."""
 import ctypes as C
 import os
+import subprocess
 import re
 
 import pytest
@@ -11,9 +12,14 @@ from fj import abi, runtime
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+# Headers whose functions the host defines and the library resolves at run
+# time (it must not define them itself).
+IMPORT_HEADERS = ("juicer_diffusion_api.h", "juicer_dir_boundary_api.h")
+
+
+def declared_functions(headers=None):
     names = []
-    for header in os.listdir(os.path.join(ROOT, "include", "fj")):
+    for header in headers or [h for h in os.listdir(os.path.join(ROOT, "include", "fj")) if h not in IMPORT_HEADERS]:
         text = open(os.path.join(ROOT, "include", "fj", header)).read()
         names += re.findall(r"^\s*(?:FjStatus|FjRenderOutcome|uint32_t|void|FjCallCompletionResult)\s+(fj_\w+)\s*\(",
                             text, re.M)
@@ -75,3 +81,15 @@ def test_unknown_handle_rejected():
     bogus = C.c_void_p(0x1234)
     assert lib.fj_cuda_check_terminal(bogus, None).category == abi.FJ_STATUS_PREPARATION_FAILURE
     assert lib.fj_cuda_check_terminal(None, None).category == abi.FJ_STATUS_UNSUPPORTED_INPUT
+
+
+def test_host_imports_are_resolved_not_defined():
+    """The optional host functions are looked up at run time: the library
+    neither defines nor links against them."""
+    names = declared_functions(IMPORT_HEADERS)
+    assert sorted(names) == ["fj_diffusion_components_prepare", "fj_dir_boundary_acquire", "fj_dir_boundary_describe",
+                             "fj_dir_boundary_release", "fj_dir_boundary_view"]
+    out = subprocess.run(["nm", "-D", runtime.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    for n in names:
+        assert n not in out, n
+    assert "dlsym" in out
