@@ -546,6 +546,30 @@ FJ_EXPORT FjStatus fjx_highlight_boost_apply(FjCuda* cuda, int32_t device_id, co
     }
 }
 
+FJ_EXPORT FjStatus fjx_glare_field(FjCuda* cuda, int32_t device_id, const FjScannerPostEffects* post, uint64_t seed,
+                                   uint32_t medium, int32_t origin_x, int32_t origin_y, int32_t width, int32_t height,
+                                   float* field, float* uniforms, uint64_t stream, FjErrorBuffer* error) {
+    try {
+        if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
+        if (!post) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null scanner post descriptor");
+        FjStatus st{};
+        DeviceState* dev = device_state(cuda, device_id, st, error);
+        if (!dev) return st;
+        const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        st = glare_field(*dev, *post, seed, medium, origin_x, origin_y, width, height, field, uniforms, s, error);
+        if (st.category != FJ_STATUS_SUCCESS) return st;
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(error, e, "scanner glare");
+        return st;
+    } catch (const std::bad_alloc&) {
+        return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fjx_glare_field");
+    }
+}
+
 FJ_EXPORT FjStatus fjx_capture_grain_input(FjCuda* cuda, int32_t device_id, float* density_c, float* density_m,
                                            float* density_y, int64_t samples, FjErrorBuffer* error) {
     try {

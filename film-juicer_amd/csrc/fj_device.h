@@ -72,6 +72,8 @@ struct ScanParams {
     int correction;
     float corr_slope, corr_offset;
     float xyz_to_rgb[9];    // xyz_to_rgb * cat02
+    const float* glare;     // scanner glare percent per pixel, or null
+    float glare_xyz[3];     // 0.01 x the scan illuminant's XYZ
 };
 
 struct ColorParams {

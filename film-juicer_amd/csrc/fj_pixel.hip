@@ -146,7 +146,7 @@ __device__ __forceinline__ void pixel_stage_body(const PixelParams& p, int tiles
             print_develop(p.pr, L.spec, L.print, D);
         }
         float rgb[3];
-        const bool ok = scan_colour(p.sc, p.co, D, rgb);
+        const bool ok = scan_colour(p.sc, p.co, D, p.sc.glare ? p.sc.glare[idx] : 0.0f, rgb);
         if (!ok) {
             errors |= 1u;
             rgb[0] = rgb[1] = rgb[2] = 0.0f;

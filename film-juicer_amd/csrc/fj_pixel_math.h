@@ -566,10 +566,11 @@ __device__ __forceinline__ void gamut_compress(const ColorParams& co, float rgb[
     mat3v(co.xyz_to_native, xyz, rgb);
 }
 
-// Scanner: normalize -> LUT -> exp2 -> correction -> colour -> gamut
-// (JuicerCudaScanPipeline.cu:1735-1811). False on a scan error.
+// Scanner: normalize -> LUT -> exp2 -> correction -> glare -> colour ->
+// gamut (JuicerCudaScanPipeline.cu:1735-1811). `glare` is the pixel's glare
+// percent (0 without glare). False on a scan error.
 __device__ __forceinline__ bool scan_colour(const ScanParams& sc, const ColorParams& co, const float D[3],
-                                            float rgb[3]) {
+                                            float glare, float rgb[3]) {
     float nrm[3], l2[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) nrm[c] = (D[c] - sc.dmin[c]) * sc.inv_span[c];
@@ -580,6 +581,8 @@ __device__ __forceinline__ bool scan_colour(const ScanParams& sc, const ColorPar
         const float s = cy / (xyz[1] + 1e-10f);
         xyz[0] *= s; xyz[1] *= s; xyz[2] *= s;
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xyz[c] = fmaf(glare, sc.glare_xyz[c], xyz[c]);
     mat3v(sc.xyz_to_rgb, xyz, rgb);
     if (!isfinite(rgb[0]) || !isfinite(rgb[1]) || !isfinite(rgb[2])) return false;
     if (co.gamut) {

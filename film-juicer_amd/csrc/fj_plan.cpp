@@ -4,6 +4,7 @@
 #include "fj_autoexposure.h"
 #include "fj_highlight.h"
 #include "fj_diffusion_host.h"
+#include "fj_glare.h"
 #include "fj_grain.h"
 #include "fj_grain_host.h"
 #include "fj_halation_host.h"
@@ -22,6 +23,18 @@
 #include "fj_spatial.h"
 
 namespace fjx {
+
+// FNV-1a over the little-endian bytes of 64-bit values (the reference's
+// Hash::hash_uint64_values, src/Hash.h:50-56): the glare stream's seed.
+uint64_t fnv1a_u64s(const uint64_t* v, int n) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (int i = 0; i < n; ++i)
+        for (int b = 0; b < 8; ++b) {
+            h ^= (v[i] >> (8 * b)) & 0xFFu;
+            h *= 0x100000001b3ull;
+        }
+    return h;
+}
 
 // ---------------------------------------------------------------------------
 // hashing: 64-bit words, multiply-xorshift mixing (content identity only)
@@ -376,10 +389,25 @@ bool gaussian_valid(float sigma, int radius) {
     return std::isfinite(sigma) && sigma > 0.0f && radius >= 1 && radius <= kMaxGaussianRadius;
 }
 
+// Scanner glare is generated when the post descriptor is bound and its
+// percent is finite and positive with a finite roughness
+// (JuicerCudaScanPipeline.cu:2361-2364, JuicerCudaExecutor.cpp:2405-2413),
+// and blurred with the Gaussian that the glare flag prepares
+// (ProcessRoot.cpp:3147-3154).
+bool glare_active(const FjScannerPostEffects& sp) {
+    return sp.hash != 0 && std::isfinite(sp.glare_percent) && sp.glare_percent > 0.0f &&
+           std::isfinite(sp.glare_roughness);
+}
+
+bool glare_blurred(const FjScannerPostEffects& sp) {
+    return (sp.flags & FJ_POST_GLARE) != 0 && sp.glare_blur_sigma_px > 0.0f;
+}
+
 bool post_kernels_valid(const FjScannerPostEffects& sp) {
     if (sp.hash == 0) return true;
     if (sp.lens_blur_sigma_px > 0.0f && !gaussian_valid(sp.lens_blur_sigma_px, sp.lens_blur_radius)) return false;
     if (sp.unsharp_sigma_px > 0.0f && !gaussian_valid(sp.unsharp_sigma_px, sp.unsharp_radius)) return false;
+    if (glare_blurred(sp) && !gaussian_valid(sp.glare_blur_sigma_px, sp.glare_blur_radius)) return false;
     return true;
 }
 
@@ -431,7 +459,6 @@ bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
         const bool print_route = p.route == FJ_ROUTE_NEGATIVE_PRINT || p.route == FJ_ROUTE_POSITIVE_PRINT;
         if (!print_route && a.method == FJ_METER_MEDIAN) return d.unsupported("QuantizedMedianNotAcceptedForPhase3");
     }
-    if (p.scanner_post.flags & FJ_POST_GLARE) return d.unsupported("scanner glare");
     if (!post_kernels_valid(p.scanner_post)) return d.preparation("gaussian descriptor is invalid");
     if (p.optics.flags & ~(FJ_OPTICS_CAMERA_DIFFUSION | FJ_OPTICS_ENLARGER_DIFFUSION | FJ_OPTICS_SCATTER_HALATION |
                            FJ_OPTICS_HIGHLIGHT_BOOST))
@@ -918,6 +945,50 @@ bool aborted(const FjAbortCallback& cb, FjRenderOutcome& out, FjErrorBuffer* err
 
 }  // namespace
 
+// Generates the glare field of a width x height window into `field` and,
+// with `weights`, blurs it through `tmp` (optics_glare_generate_kernel and
+// blur_plane_in_place, JuicerCudaScanPipeline.cu:273-314, :2361-2405).
+static hipError_t launch_glare(DeviceState& dev, const FjScannerPostEffects& sp, const std::vector<float>& weights,
+                        uint64_t seed, uint32_t medium, int origin_x, int origin_y, int width, int height, float* field,
+                        float* tmp, float* uniforms, hipStream_t stream) {
+    GlareArgs ga{};
+    ga.out = field;
+    ga.width = width;
+    ga.height = height;
+    ga.seed = seed;
+    ga.medium = medium;
+    ga.origin_x = origin_x;
+    ga.origin_y = origin_y;
+    ga.uniforms = uniforms;
+    // Lognormal of mean `percent` and deviation roughness x percent
+    // (lognormal_from_mean_std_device, JuicerCudaScanPipeline.cu:248-255),
+    // its parameters in FP64 once per frame.
+    const double mean = std::max(0.0, static_cast<double>(sp.glare_percent));
+    const double sd = std::max(0.0, static_cast<double>(sp.glare_roughness) * sp.glare_percent);
+    const double sigma_sq = std::log1p((sd * sd) / (mean * mean));
+    ga.sigma = static_cast<float>(std::sqrt(std::max(0.0, sigma_sq)));
+    ga.mu = static_cast<float>(std::log(std::max(1e-12, mean)) - 0.5 * sigma_sq);
+    hipError_t e = launch_glare_generate(ga, stream);
+    if (e != hipSuccess || weights.empty()) return e;
+    const void* gptr = nullptr;
+    e = dev.table(hash_bytes(weights.data(), weights.size() * 4, 0x474Cull), weights.data(), weights.size() * 4, &gptr);
+    if (e != hipSuccess) return e;
+    GrainBlurArgs h{};
+    h.in = field;
+    h.out = tmp;
+    h.weights = static_cast<const float*>(gptr);
+    h.radius = static_cast<int>(weights.size() / 2);
+    h.width = width;
+    h.height = height;
+    e = launch_grain_blur_h(h, stream);
+    if (e != hipSuccess) return e;
+    GrainBlurArgs v = h;
+    v.in = tmp;
+    v.out = field;
+    v.initialize = 1;
+    return launch_grain_blur_v(v, stream);
+}
+
 FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSubmission& submission,
                              const FjPreparedHostData& prepared, FjAbortCallback abort_callback, FjErrorBuffer* error,
                              const FjHostPreparation* host) {
@@ -1035,7 +1106,13 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     // Scanner-post Gaussian kernels, built before any launch so that a
     // refused kernel leaves the frame untouched.
     static const float kIdentityTap = 1.0f;
-    std::vector<float> lens_w, unsharp_w{kIdentityTap};
+    std::vector<float> lens_w, unsharp_w{kIdentityTap}, glare_w;
+    const bool glare = glare_active(prepared.scanner_post);
+    if (glare && glare_blurred(prepared.scanner_post)) {
+        const FjScannerPostEffects& sp = prepared.scanner_post;
+        glare_w = kernel_weights(sp.glare_blur_radius, sp.glare_blur_sigma_px);
+        if (glare_w.empty()) return host_or(FJ_STATUS_PREPARATION_FAILURE, "gaussian descriptor is invalid", error);
+    }
     if (post) {
         const FjScannerPostEffects& sp = prepared.scanner_post;
         if (lens_active(sp)) {
@@ -1076,6 +1153,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     const long long lens_off =
         (post && (lens_active(prepared.scanner_post) || unsharp_separate(prepared.scanner_post))) ? take(plane) : -1;
     const long long pad_off = padded_floats ? take(padded_floats) : -1;
+    const long long glare_off = glare ? take(plane) : -1;
+    const long long glare_tmp_off = glare_w.empty() ? -1 : take(plane);
     const long long hb_off =
         boost ? take(static_cast<long long>((sizeof(HbState) + sizeof(float) - 1) / sizeof(float))) : -1;
     const bool auto_exposure = (prepared.film_exposure.flags & FJ_FILM_AUTO_EXPOSURE) != 0;
@@ -1129,6 +1208,23 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         if (e != hipSuccess) return outcome_hip(e, "auto exposure", error);
         pp.ex.ae = &ae->scale;
         pp.ex.with_route = 1;
+    }
+    if (glare) {
+        // The scan's glare field, generated and blurred before any stage
+        // that scans (launch_focused_scan_linear_rgb,
+        // JuicerCudaScanPipeline.cu:2361-2405); seeded by the recipe, the
+        // frame token and the source origin (JuicerCudaExecutor.cpp:2416-2420).
+        const uint64_t seed_in[4] = {prepared.recipe_hash, submission.frame_token,
+                                     static_cast<uint64_t>(static_cast<int64_t>(frame.source.bounds.x1)),
+                                     static_cast<uint64_t>(static_cast<int64_t>(frame.source.bounds.y1))};
+        mark("scanner_glare");
+        e = launch_glare(dev, prepared.scanner_post, glare_w, fnv1a_u64s(seed_in, 4), pp.print ? 1u : 0u,
+                         frame.source.bounds.x1, frame.source.bounds.y1, pp.io.width, pp.io.height, ws + glare_off,
+                         glare_w.empty() ? nullptr : ws + glare_tmp_off, nullptr, stream);
+        if (e != hipSuccess) return outcome_hip(e, "scanner glare", error);
+        pp.sc.glare = ws + glare_off;
+        for (int c = 0; c < 3; ++c)
+            pp.sc.glare_xyz[c] = static_cast<float>(0.01 * static_cast<double>(prepared.output_color.illuminant_xyz[c]));
     }
     // Transfer tables first (built once per PSF and padded size).
     DiffusionStage diff_stage[2];
@@ -1490,6 +1586,34 @@ FjStatus grain_apply_planes(DeviceState& dev, const FjPreparedHostData& prepared
     if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
     put_error(error, "");
     return status_of(FJ_STATUS_SUCCESS);
+}
+
+FjStatus glare_field(DeviceState& dev, const FjScannerPostEffects& post, uint64_t seed, uint32_t medium, int origin_x,
+                     int origin_y, int width, int height, float* field, float* uniforms, hipStream_t stream,
+                     FjErrorBuffer* error) {
+    if (!glare_active(post) || !post_kernels_valid(post))
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "MalformedPreparedHostData field=scanner_post.glare", error).status;
+    if (!field || width <= 0 || height <= 0 || medium > 1)
+        return outcome_diag(FJ_STATUS_UNSUPPORTED_INPUT, "UnsupportedInput glare field", error).status;
+    std::vector<float> weights;
+    if (glare_blurred(post)) {
+        weights = kernel_weights(post.glare_blur_radius, post.glare_blur_sigma_px);
+        if (weights.empty())
+            return outcome_diag(FJ_STATUS_PREPARATION_FAILURE, "gaussian descriptor is invalid", error).status;
+    }
+    ++dev.frame_serial;
+    const size_t plane_bytes = static_cast<size_t>(width) * height * sizeof(float);
+    hipError_t e = dev.ensure_workspace(weights.empty() ? sizeof(float) : plane_bytes);
+    if (e != hipSuccess) return outcome_hip(e, "workspace", error).status;
+    e = dev.acquire_workspace(stream);
+    if (e != hipSuccess) return outcome_hip(e, "workspace order", error).status;
+    e = launch_glare(dev, post, weights, seed, medium, origin_x, origin_y, width, height, field,
+                     static_cast<float*>(dev.workspace()), uniforms, stream);
+    const hipError_t r = dev.release_workspace(stream);
+    if (e != hipSuccess) return outcome_hip(e, "scanner glare", error).status;
+    if (r != hipSuccess) return outcome_hip(r, "workspace order", error).status;
+    put_error(error, "");
+    return FjStatus{FJ_STATUS_SUCCESS, FJ_API_NONE, 0};
 }
 
 FjStatus highlight_boost_planes(DeviceState& dev, const FjHighlightBoost& boost, float* const planes[3],

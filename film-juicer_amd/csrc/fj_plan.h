@@ -116,6 +116,13 @@ FjStatus diffusion_apply_planes(DeviceState& dev, const FjDiffusion& stage, floa
 // `stream`; waits for the disposition and reports a failing one.
 FjStatus highlight_boost_planes(DeviceState& dev, const FjHighlightBoost& boost, float* const planes[3],
                                 long long samples, hipStream_t stream, FjErrorBuffer* error);
+// The scanner-glare field of a width x height window whose first pixel is
+// (origin_x, origin_y) of the frame's stream `seed` (medium 1 print, 0 film),
+// blurred when the descriptor's glare flag asks for it, into `field`;
+// `uniforms` (2 planes, optional) receives the raw FP32 uniforms.
+FjStatus glare_field(DeviceState& dev, const FjScannerPostEffects& post, uint64_t seed, uint32_t medium, int origin_x,
+                     int origin_y, int width, int height, float* field, float* uniforms, hipStream_t stream,
+                     FjErrorBuffer* error);
 // The checks a boost descriptor must pass (recipe bound, finite fields).
 bool highlight_boost_valid(const FjHighlightBoost& boost);
 

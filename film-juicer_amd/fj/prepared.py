@@ -273,6 +273,25 @@ def with_lens_blur(p: Prepared, sigma_px: float, radius: int | None = None) -> P
     return p
 
 
+def with_glare(p: Prepared, percent: float = 0.03, roughness: float = 0.7, blur_sigma_px: float = 0.5) -> Prepared:
+    """Turns on scanner glare (the plug-in's defaults: 0.03 %, roughness 0.7,
+    blur sigma 0.5 px; rust scanner/post_effects.rs:133-160): the glare flag,
+    percent and roughness, and the blur radius int(3 sigma + 0.5) capped at 75
+    (post_effects.rs:105-122)."""
+    import numpy as np
+    sp = p.record.scanner_post
+    sp.flags |= abi.FJ_POST_GLARE
+    sp.glare_percent = float(percent)
+    sp.glare_roughness = float(roughness)
+    sigma = np.float32(blur_sigma_px)
+    sp.glare_blur_sigma_px = float(sigma)
+    r = np.float32(3.0) * sigma + np.float32(0.5)
+    sp.glare_blur_radius = min(int(r), 75) if sigma > 0 else 0
+    if sp.hash == 0:
+        sp.hash = 0x474C415245
+    return p
+
+
 def with_highlight_boost(p: Prepared, width: int, height: int, protection: float, sharpness: float,
                          peak_gain: float) -> Prepared:
     """Turns on the camera-carrier highlight boost for a full width x height

@@ -88,6 +88,21 @@ FjStatus fjx_highlight_boost_apply_static(FjCuda* cuda, int32_t device_id, const
                                           float* green, float* blue, int64_t samples, uint64_t stream,
                                           FjErrorBuffer* error);
 
+/* The scanner-glare field of one frame window on caller device memory: the
+ * reference's optics_glare_generate_kernel and blur_plane_in_place
+ * (src/Cuda/Scan/JuicerCudaScanPipeline.cu:273-314, :2361-2405; internal
+ * there). Pixel (x, y) of the width x height window draws the Philox4x32-10
+ * stream of `seed` (the frame's FNV-1a of recipe hash, frame token and
+ * source origin, JuicerCudaExecutor.cpp:2416-2420) at counter
+ * {origin_x + x, origin_y + y, medium, n} (medium 1 print, 0 film) and gets
+ * the lognormal glare percent of the descriptor's percent and roughness;
+ * the field is blurred when FJ_POST_GLARE and a positive blur sigma ask for
+ * it. `uniforms` (2 x width x height floats, optional) receives the two raw
+ * FP32 uniforms per pixel. Waits for the stream before returning. */
+FjStatus fjx_glare_field(FjCuda* cuda, int32_t device_id, const FjScannerPostEffects* post, uint64_t seed,
+                         uint32_t medium, int32_t origin_x, int32_t origin_y, int32_t width, int32_t height,
+                         float* field, float* uniforms, uint64_t stream, FjErrorBuffer* error);
+
 /* Diagnostic with no counterpart in the reference: arms a one-shot copy of
  * the three C/M/Y film-density planes entering the visual-grain stage of the
  * next frame rendered on `device_id` (the developed density of the grain

@@ -179,6 +179,10 @@ struct FjoPlan {
     FjoDiffusionFn diffusion; /* optional replacement of fjo_diffusion */
     void* diffusion_user;
     float* grain_capture[3]; /* optional copy of the densities entering the grain stage */
+    /* The frame's glare stream inputs: floor(time in frames) and the source
+     * image origin (JuicerCudaExecutor.cpp:2416-2420). */
+    uint64_t frame_token;
+    int origin_x, origin_y;
 };
 
 static void set_message(char* msg, size_t cap, const char* text) {
@@ -393,6 +397,7 @@ typedef struct Frame {
     double* linear;   /* [h][w][3] */
     uint32_t errors;
     double ae;          /* metered auto-exposure scale, 1 when off */
+    double* glare;      /* scanner glare percent per pixel, or NULL */
     pthread_mutex_t lock;
 } Frame;
 
@@ -939,7 +944,7 @@ static void gamut_compress(const FjOutputColor* oc, double rgb[3]) {
 
 /* Scanner: normalize -> PCHIP LUT -> exp2 -> correction -> CAT02 -> RGB ->
  * gamut (JuicerCudaScanPipeline.cu:1735-1811). Returns 0 on scan error. */
-static int px_scan(const FjoPlan* pl, const double D[3], double rgb[3]) {
+static int px_scan(const FjoPlan* pl, const double D[3], double glare_percent, double rgb[3]) {
     const FjDensityBounds* sb = &pl->p.scanner_bounds;
     double nrm[3], l2[3];
     for (int c = 0; c < 3; ++c) nrm[c] = (D[c] - (double)sb->min_cmy[c]) * (double)sb->inv_span_cmy[c];
@@ -951,6 +956,10 @@ static int px_scan(const FjoPlan* pl, const double D[3], double rgb[3]) {
         const double sc = cy / (xyz[1] + 1e-10);
         for (int c = 0; c < 3; ++c) xyz[c] *= sc;
     }
+    /* scanner glare: a veiling fraction of the scan illuminant
+     * (JuicerCudaScanPipeline.cu:1801-1806) */
+    if (glare_percent != 0.0)
+        for (int c = 0; c < 3; ++c) xyz[c] += glare_percent * 0.01 * (double)pl->p.output_color.illuminant_xyz[c];
     double ad[3];
     mat3(pl->p.output_color.cat02, xyz, ad);
     mat3(pl->p.output_color.xyz_to_rgb, ad, rgb);
@@ -1478,6 +1487,107 @@ static void lens_plane(double* v, int W, int H, const float* w, int r) {
     free(h);
 }
 
+/* Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers:
+ * as easy as 1, 2, 3", SC 2011): ten rounds of the 4x32 S-box with the
+ * published multipliers and Weyl key increments; the key is bumped before
+ * every round but the first. */
+void fjo_philox4x32_10(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]) {
+    uint32_t k0 = key[0], k1 = key[1];
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    for (int round = 0; round < 10; ++round) {
+        if (round) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = (uint32_t)p1;
+        c2 = n2;
+        c3 = (uint32_t)p0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* The glare stream's n-th 32-bit draw at (x, y): OpenRAND's Philox with key
+ * {seed high, seed low} and counter {x, y, medium, n}, first output word
+ * (third_party/openrand/philox.h:64-125, GlareRngDevice at
+ * JuicerCudaScanPipeline.cu:214-228). */
+static uint32_t glare_draw(uint64_t seed, uint32_t x, uint32_t y, uint32_t medium, uint32_t n) {
+    const uint32_t key[2] = {(uint32_t)(seed >> 32), (uint32_t)seed};
+    const uint32_t ctr[4] = {x, y, medium, n};
+    uint32_t out[4];
+    fjo_philox4x32_10(key, ctr, out);
+    return out[0];
+}
+
+/* OpenRAND's u01 in FP32: (float)x * 2^-32 + 2^-33 (base_state.h:292-300). */
+static float u01f(uint32_t x) { return (float)x * 0x1p-32f + 0x1p-33f; }
+
+/* The glare field (optics_glare_generate_kernel, JuicerCudaScanPipeline.cu:
+ * 273-314): per pixel a lognormal sample with mean `percent` and standard
+ * deviation roughness x percent from one Box-Muller normal of the pixel's
+ * stream; the uniforms are the reference's FP32 words, the rest FP64. */
+void fjo_glare_uniforms(uint64_t seed, int64_t abs_x, int64_t abs_y, uint32_t medium, float u[2]) {
+    u[0] = u01f(glare_draw(seed, (uint32_t)(uint64_t)abs_x, (uint32_t)(uint64_t)abs_y, medium, 0));
+    u[1] = u01f(glare_draw(seed, (uint32_t)(uint64_t)abs_x, (uint32_t)(uint64_t)abs_y, medium, 1));
+}
+
+double fjo_glare_sample(uint64_t seed, int64_t abs_x, int64_t abs_y, uint32_t medium, float percent, float roughness) {
+    float u[2];
+    fjo_glare_uniforms(seed, abs_x, abs_y, medium, u);
+    const float u1 = fminf(fmaxf(u[0], 1e-7f), 1.0f), u2 = u[1];
+    const double n = sqrt(-2.0 * log((double)u1)) * cos(6.283185307179586476925286766559 * (double)u2);
+    const double mean = fmax(0.0, (double)percent), sd = fmax(0.0, (double)roughness * (double)percent);
+    const double sigma_sq = log(1.0 + (sd * sd) / (mean * mean));
+    const double sigma = sqrt(fmax(0.0, sigma_sq));
+    const double mu = log(fmax(1e-12, mean)) - 0.5 * sigma_sq;
+    const double g = exp(mu + sigma * n);
+    return isfinite(g) ? g : 0.0;
+}
+
+/* FNV-1a over the little-endian bytes of 64-bit values
+ * (Hash::hash_uint64_values, src/Hash.h:50-56). */
+static uint64_t fnv_u64s(const uint64_t* v, int n) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (int i = 0; i < n; ++i)
+        for (int b = 0; b < 8; ++b) {
+            h ^= (v[i] >> (8 * b)) & 0xFFu;
+            h *= 0x100000001b3ull;
+        }
+    return h;
+}
+
+/* The frame's glare seed: recipe hash, frame token and source origin
+ * (JuicerCudaExecutor.cpp:2416-2420). */
+uint64_t fjo_glare_seed(uint64_t recipe_hash, uint64_t frame_token, int origin_x, int origin_y) {
+    const uint64_t v[4] = {recipe_hash, frame_token, (uint64_t)(int64_t)origin_x, (uint64_t)(int64_t)origin_y};
+    return fnv_u64s(v, 4);
+}
+
+/* Glare is generated when the post descriptor is bound and its percent is
+ * finite and positive with a finite roughness (JuicerCudaScanPipeline.cu:
+ * 2361-2364, JuicerCudaExecutor.cpp:2405-2413); it is blurred when the glare
+ * flag prepared a kernel (ProcessRoot.cpp:3147-3154). */
+static int glare_active(const FjScannerPostEffects* sp) {
+    return sp->hash != 0 && isfinite(sp->glare_percent) && sp->glare_percent > 0.0f && isfinite(sp->glare_roughness);
+}
+
+static int glare_blurred(const FjScannerPostEffects* sp) {
+    return (sp->flags & FJ_POST_GLARE) && sp->glare_blur_sigma_px > 0.0f;
+}
+
+void fjo_glare_plane(const FjScannerPostEffects* sp, uint64_t seed, uint32_t medium, int origin_x, int origin_y, int W,
+                     int H, double* out) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+            out[(size_t)y * W + x] = fjo_glare_sample(seed, (int64_t)origin_x + x, (int64_t)origin_y + y, medium,
+                                                      sp->glare_percent, sp->glare_roughness);
+}
+
 static void unsharp_plane(double* v, int W, int H, const float* w, int r, double amount) {
     const size_t N = (size_t)W * H;
     double* h = (double*)malloc(sizeof(double) * N);
@@ -1559,7 +1669,7 @@ static uint32_t px_back(const FjoPlan* pl, Frame* f, size_t idx, double D[3]) {
         return 0;
     }
     if (pl->print) px_print(pl, D);
-    const int ok = px_scan(pl, D, rgb);
+    const int ok = px_scan(pl, D, f->glare ? f->glare[idx] : 0.0, rgb);
     if (!ok) rgb[0] = rgb[1] = rgb[2] = 0.0;
     memcpy(f->linear + idx * 3, rgb, sizeof(rgb));
     return ok ? 0u : 1u;
@@ -1577,7 +1687,7 @@ static void rows_enlarged(Frame* f, int y0, int y1) {
             double logp[3], D[3], rgb[3];
             for (int c = 0; c < 3; ++c) logp[c] = log10(fmax(f->plin[c][idx], 0.0) + 1e-10);
             print_curves(pl, logp, D);
-            const int ok = px_scan(pl, D, rgb);
+            const int ok = px_scan(pl, D, f->glare ? f->glare[idx] : 0.0, rgb);
             if (!ok) rgb[0] = rgb[1] = rgb[2] = 0.0;
             memcpy(f->linear + idx * 3, rgb, sizeof(rgb));
             if (!ok) errs |= 1u;
@@ -1755,7 +1865,35 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     pthread_mutex_init(&f.lock, NULL);
     int rc = 0;
     const int boost = (p->optics.flags & FJ_OPTICS_HIGHLIGHT_BOOST) != 0;
-    if (halation || camera_diffusion || boost) {
+    const FjScannerPostEffects* sp = &p->scanner_post;
+    /* A bound post descriptor prepares each Gaussian whose sigma is positive
+     * and refuses a non-finite sigma or a radius outside 1..75
+     * (ProcessRoot.cpp:3121-3158, JuicerCudaResources.cpp:6436-6448). */
+    if (sp->hash != 0 &&
+        ((sp->lens_blur_sigma_px > 0.0f &&
+          (!isfinite(sp->lens_blur_sigma_px) || sp->lens_blur_radius < 1 || sp->lens_blur_radius > 75)) ||
+         (sp->unsharp_sigma_px > 0.0f &&
+          (!isfinite(sp->unsharp_sigma_px) || sp->unsharp_radius < 1 || sp->unsharp_radius > 75)) ||
+         (glare_blurred(sp) &&
+          (!isfinite(sp->glare_blur_sigma_px) || sp->glare_blur_radius < 1 || sp->glare_blur_radius > 75)))) {
+        free(f.linear);
+        pthread_mutex_destroy(&f.lock);
+        return 4;
+    }
+    if (glare_active(sp)) {
+        /* generated and blurred before the scanner reads it
+         * (launch_focused_scan_linear_rgb, JuicerCudaScanPipeline.cu:2361-2405) */
+        const uint64_t seed = fjo_glare_seed(p->recipe_hash, plan->frame_token, plan->origin_x, plan->origin_y);
+        f.glare = (double*)malloc(sizeof(double) * N);
+        fjo_glare_plane(sp, seed, plan->print ? 1u : 0u, plan->origin_x, plan->origin_y, width, height, f.glare);
+        if (glare_blurred(sp)) {
+            float* w = (float*)malloc(sizeof(float) * (2 * (size_t)sp->glare_blur_radius + 1));
+            if (fjo_gaussian_weights(sp->glare_blur_radius, sp->glare_blur_sigma_px, w)) rc = 4;
+            else lens_plane(f.glare, width, height, w, sp->glare_blur_radius);
+            free(w);
+        }
+    }
+    if (rc == 0 && (halation || camera_diffusion || boost)) {
         /* expose_print_camera (JuicerCudaExecutor.cpp:1874-1989): carrier,
          * highlight boost, camera diffusion, then scatter/halation in place
          * on it. */
@@ -1794,16 +1932,6 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
         rc = plan_diffusion(plan, &p->optics.enlarger, f.plin, width, height);
         if (rc == 0) run_rows(&f, rows_enlarged, threads);
     }
-    const FjScannerPostEffects* sp = &p->scanner_post;
-    /* A bound post descriptor prepares each Gaussian whose sigma is positive
-     * and refuses a non-finite sigma or a radius outside 1..75
-     * (ProcessRoot.cpp:3121-3158, JuicerCudaResources.cpp:6436-6448). */
-    if (rc == 0 && sp->hash != 0 &&
-        ((sp->lens_blur_sigma_px > 0.0f &&
-          (!isfinite(sp->lens_blur_sigma_px) || sp->lens_blur_radius < 1 || sp->lens_blur_radius > 75)) ||
-         (sp->unsharp_sigma_px > 0.0f &&
-          (!isfinite(sp->unsharp_sigma_px) || sp->unsharp_radius < 1 || sp->unsharp_radius > 75))))
-        rc = 4;
     if (rc == 0 && sp->hash != 0 && sp->lens_blur_sigma_px > 0.0f) {
         /* lens blur before the unsharp mask (JuicerCudaScanPipeline.cu:2795-2809) */
         float* w = (float*)malloc(sizeof(float) * (2 * (size_t)sp->lens_blur_radius + 1));
@@ -1857,10 +1985,17 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
         free(f.plin[c]);
         free(f.dens[c]);
     }
+    free(f.glare);
     free(f.linear);
     pthread_mutex_destroy(&f.lock);
     if (error_flags) *error_flags = f.errors;
     return rc;
+}
+
+void fjo_plan_set_frame(FjoPlan* plan, uint64_t frame_token, int origin_x, int origin_y) {
+    plan->frame_token = frame_token;
+    plan->origin_x = origin_x;
+    plan->origin_y = origin_y;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1892,7 +2027,9 @@ int fjo_plan_scan_lut(const FjoPlan* plan, const double normalized_cmy[3], doubl
     return px_scan_lut(plan, normalized_cmy, log2_xyz);
 }
 
-int fjo_plan_scan(const FjoPlan* plan, const double density_cmy[3], double rgb[3]) { return px_scan(plan, density_cmy, rgb); }
+int fjo_plan_scan(const FjoPlan* plan, const double density_cmy[3], double rgb[3]) {
+    return px_scan(plan, density_cmy, 0.0, rgb);
+}
 
 void fjo_gamut_compress(const FjOutputColor* color, double rgb[3]) { gamut_compress(color, rgb); }
 

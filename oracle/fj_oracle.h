@@ -89,6 +89,20 @@ void fjo_plan_set_diffusion(FjoPlan* plan, FjoDiffusionFn fn, void* user);
 /* Makes fjo_render copy the C/M/Y float densities entering its grain stage
  * into three width x height buffers (NULL disarms). */
 void fjo_plan_capture_grain_input(FjoPlan* plan, float* c, float* m, float* y);
+/* Sets the frame inputs of the scanner-glare stream: the frame token
+ * (floor of the time in frames) and the source image origin (0 by default). */
+void fjo_plan_set_frame(FjoPlan* plan, uint64_t frame_token, int origin_x, int origin_y);
+
+/* Philox4x32-10 (Salmon et al. 2011) of one 128-bit counter under a 64-bit key. */
+void fjo_philox4x32_10(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]);
+/* Scanner glare: the frame's stream seed, one pixel's glare percent, and the
+ * unblurred field of a W x H window whose first pixel is (origin_x, origin_y)
+ * (medium 1 for print routes, 0 for film). */
+uint64_t fjo_glare_seed(uint64_t recipe_hash, uint64_t frame_token, int origin_x, int origin_y);
+void fjo_glare_uniforms(uint64_t seed, int64_t abs_x, int64_t abs_y, uint32_t medium, float u[2]);
+double fjo_glare_sample(uint64_t seed, int64_t abs_x, int64_t abs_y, uint32_t medium, float percent, float roughness);
+void fjo_glare_plane(const FjScannerPostEffects* post, uint64_t seed, uint32_t medium, int origin_x, int origin_y,
+                     int W, int H, double* out);
 
 /* Builds host state for one prepared record (scanner LUT, curves). Returns
  * NULL and writes a message on unsupported input. */

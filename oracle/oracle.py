@@ -70,6 +70,14 @@ def lib():
         L.fjo_plan_scan.argtypes = [C.c_void_p, dp, dp]
         L.fjo_gamut_compress.argtypes = [C.c_void_p, dp]
         L.fjo_encode.argtypes = [C.c_void_p, dp]
+        L.fjo_plan_set_frame.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int]
+        L.fjo_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.fjo_glare_seed.restype = C.c_uint64
+        L.fjo_glare_seed.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int]
+        L.fjo_glare_uniforms.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_uint32, fp]
+        L.fjo_glare_sample.restype = C.c_double
+        L.fjo_glare_sample.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_uint32, C.c_float, C.c_float]
+        L.fjo_glare_plane.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -283,6 +291,12 @@ class Plan:
         lib().fjo_plan_set_diffusion(self.handle, C.cast(_FFT_DIFFUSION, C.c_void_p) if on else None, None)
         return self
 
+    def set_frame(self, frame_token: int = 0, origin=(0, 0)):
+        """The frame inputs of the scanner-glare stream: floor of the time in
+        frames and the source image origin."""
+        lib().fjo_plan_set_frame(self.handle, frame_token & (2 ** 64 - 1), int(origin[0]), int(origin[1]))
+        return self
+
     def capture_grain_input(self, width: int, height: int):
         """Arms a copy of the densities entering the grain stage of the next
         render; returns the [3][H][W] float32 array it fills."""
@@ -412,3 +426,35 @@ def encode(output_color, rgb):
     v = _vec(rgb)
     lib().fjo_encode(C.byref(output_color), _dp(v))
     return v
+
+
+def philox4x32_10(key, ctr):
+    """Philox4x32-10 of one counter (4 x uint32) under a key (2 x uint32)."""
+    k = (C.c_uint32 * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+    c = (C.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    o = (C.c_uint32 * 4)()
+    lib().fjo_philox4x32_10(k, c, o)
+    return list(o)
+
+
+def glare_seed(recipe_hash: int, frame_token: int, origin_x: int, origin_y: int) -> int:
+    return lib().fjo_glare_seed(recipe_hash & (2 ** 64 - 1), frame_token & (2 ** 64 - 1), origin_x, origin_y)
+
+
+def glare_uniforms(seed: int, x: int, y: int, medium: int):
+    """The two FP32 uniforms of pixel (x, y)'s glare stream."""
+    u = np.zeros(2, np.float32)
+    lib().fjo_glare_uniforms(seed, x, y, medium, _fp(u))
+    return u
+
+
+def glare_sample(seed: int, x: int, y: int, medium: int, percent: float, roughness: float) -> float:
+    return lib().fjo_glare_sample(seed, x, y, medium, percent, roughness)
+
+
+def glare_plane(post, seed: int, medium: int, width: int, height: int, origin=(0, 0)) -> np.ndarray:
+    """The unblurred glare field (percent) of a width x height window whose
+    first pixel is `origin`, for an FjScannerPostEffects record."""
+    out = np.zeros((height, width), np.float64)
+    lib().fjo_glare_plane(C.byref(post), seed, medium, int(origin[0]), int(origin[1]), width, height, _dp(out))
+    return out

@@ -4,7 +4,11 @@
 //
 //   tests/scanner/output_encoding_oracle.cpp   ScannerOracle::apply_encoding
 //   src/GamutCompression.h                     Gamut::compress_output_oklch
+//   third_party/openrand/philox.h              openrand::Philox (the glare
+//                                              and grain fallback streams)
 #include <cstdint>
+
+#include <openrand/philox.h>
 
 #include "GamutCompression.h"
 #include "OutputColor.h"
@@ -37,6 +41,17 @@ void fjr_compress_output_oklch(const float* cmax, const float entry[9], const fl
     v.lightnessKnee = {lightness_knee[0], lightness_knee[1], lightness_knee[2]};
     v.chromaKnee = {chroma_knee[0], chroma_knee[1], chroma_knee[2]};
     Gamut::compress_output_oklch(v, rgb);
+}
+
+// `count` successive rand<uint32_t>() draws of openrand::Philox(seed, ctr,
+// global_seed, ctr1), and `count` rand<float>() draws of a second generator
+// with the same arguments.
+void fjr_philox_draws(uint64_t seed, uint32_t ctr, uint32_t global_seed, uint32_t ctr1, int count, uint32_t* words,
+                      float* floats) {
+    openrand::Philox a(seed, ctr, global_seed, ctr1);
+    for (int i = 0; i < count; ++i) words[i] = a.rand<uint32_t>();
+    openrand::Philox b(seed, ctr, global_seed, ctr1);
+    for (int i = 0; i < count; ++i) floats[i] = b.rand<float>();
 }
 
 }  // extern "C"

@@ -114,13 +114,13 @@ def test_uhd_sampled_parity():
 
 def test_unsupported_family_is_rejected_loudly():
     p = prepared.simple_view(prepared.load("NegativePrintHanatos"))
-    p.record.scanner_post.flags = abi.FJ_POST_GLARE
+    p.record.effects.hash = 0x5EED  # film/gate dust, scratches and weave are not rendered
     src = synth.frame(16, 16, seed=1)
     s = torch.from_numpy(src).cuda()
     d = torch.zeros_like(s)
     f = runtime.frame_for(s.data_ptr(), d.data_ptr(), 16, 16, 4, stream=torch.cuda.current_stream().cuda_stream)
     out, text = runtime.shared_renderer().render_raw(f, p.record)
-    assert out.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT and "glare" in text
+    assert out.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT and "film/gate effects" in text
 
 
 def test_malformed_span_and_identity():
