@@ -12,8 +12,8 @@ constexpr int kAeBins = 2048;      // median histogram bins (reference kMedianHi
 constexpr int kAeMaxBlocks = 256;  // 256-sample blocks of a preview of at most 256 x 256
 
 struct AePartial {
-    double sum_y[4];
-    double sum_w[4];
+    float sum_y[4];
+    float sum_w[4];
     float max_y;
     float pad;
 };

@@ -51,7 +51,7 @@ __device__ __forceinline__ bool sample_mean(const AeArgs& a, int x, int y, float
                        static_cast<long long>(px - d.source_bounds.x1) * a.comps;
     const float in[3] = {pix[0], pix[1], pix[2]};
     float raw[3];
-    camera_raw(a.ex, a.ex.mallett, in, raw);
+    camera_raw(a.ex, uniform_table(a.ex.mallett), in, raw);
     mean = (raw[0] + raw[1] + raw[2]) * (1.0f / 3.0f);
     return isfinite(mean);
 }
@@ -102,15 +102,15 @@ __device__ __forceinline__ void meter_weights(int method, int x, int y, int w, i
     }
 }
 
-__device__ __forceinline__ float scale_from(double y) {
+__device__ __forceinline__ float scale_from(float y) {
     // 2^(-log2(Y / 1)) with the reference's guards (:500-521).
-    double ev = 0.0;
-    if (y > 0.0 && isfinite(y)) {
-        const double c = -log(y) / log(2.0);
+    float ev = 0.0f;
+    if (y > 0.0f && isfinite(y)) {
+        const float c = -log2f(y);
         if (isfinite(c)) ev = c;
     }
-    const double s = exp2(ev);
-    return isfinite(s) && s > 0.0 ? static_cast<float>(s) : 1.0f;
+    const float s = exp2f(ev);
+    return isfinite(s) && s > 0.0f ? s : 1.0f;
 }
 
 // Weighted sums (or maxima for the median) of one block's samples.
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kThreads) void ae_partials_kernel(AeArgs a) {
     const int w = a.desc.preview_width, h = a.desc.preview_height;
     const int i = blockIdx.x * kThreads + threadIdx.x;
     const bool median = a.desc.method == FJ_METER_MEDIAN;
-    double sy[kLanes] = {}, sw[kLanes] = {};
+    float sy[kLanes] = {}, sw[kLanes] = {};
     float mx = 0.0f;
     if (i < w * h) {
         const int x = i % w, y = i / w;
@@ -132,13 +132,13 @@ __global__ __launch_bounds__(kThreads) void ae_partials_kernel(AeArgs a) {
 #pragma unroll
                 for (int l = 0; l < kLanes; ++l) {
                     const float v = isfinite(wt[l]) && wt[l] > 0.0f ? wt[l] : 0.0f;
-                    sy[l] = static_cast<double>(mean) * static_cast<double>(v);
-                    sw[l] = static_cast<double>(v);
+                    sy[l] = mean * v;
+                    sw[l] = v;
                 }
             }
         }
     }
-    __shared__ double s_y[kLanes][kThreads], s_w[kLanes][kThreads];
+    __shared__ float s_y[kLanes][kThreads], s_w[kLanes][kThreads];
     __shared__ float s_m[kThreads];
 #pragma unroll
     for (int l = 0; l < kLanes; ++l) {
@@ -173,9 +173,9 @@ __global__ __launch_bounds__(kThreads) void ae_partials_kernel(AeArgs a) {
 // (finalize_auto_exposure_from_sums_kernel, :462-522); for the median the
 // maximum and a cleared histogram for the next launch.
 __global__ __launch_bounds__(kThreads) void ae_finalize_kernel(AeArgs a, int blocks) {
-    __shared__ double s_y[kLanes][kThreads], s_w[kLanes][kThreads];
+    __shared__ float s_y[kLanes][kThreads], s_w[kLanes][kThreads];
     __shared__ float s_m[kThreads];
-    double sy[kLanes] = {}, sw[kLanes] = {};
+    float sy[kLanes] = {}, sw[kLanes] = {};
     float mx = 0.0f;
     for (int b = threadIdx.x; b < blocks; b += kThreads) {
         const AePartial& p = a.partials[b];
@@ -211,38 +211,38 @@ __global__ __launch_bounds__(kThreads) void ae_finalize_kernel(AeArgs a, int blo
         return;
     }
     if (threadIdx.x != 0) return;
-    double y = 0.0;
+    float y = 0.0f;
     bool has = false;
     auto lane_y = [&](int l) { return s_y[l][0]; };
     auto lane_w = [&](int l) { return s_w[l][0]; };
     if (method == FJ_METER_PARTIAL) {
-        const int l = lane_w(0) > 0.0 ? 0 : 3;
-        if (lane_w(l) > 0.0) {
+        const int l = lane_w(0) > 0.0f ? 0 : 3;
+        if (lane_w(l) > 0.0f) {
             y = lane_y(l) / lane_w(l);
             has = true;
         }
     } else if (method == FJ_METER_MULTI_ZONE) {
-        const double ring[3] = {0.50, 0.30, 0.20};
-        double sum = 0.0, total = 0.0;
+        const float ring[3] = {0.50f, 0.30f, 0.20f};
+        float sum = 0.0f, total = 0.0f;
         for (int l = 0; l < 3; ++l) {
-            if (lane_w(l) > 0.0) {
+            if (lane_w(l) > 0.0f) {
                 sum += ring[l] * lane_y(l) / lane_w(l);
                 total += ring[l];
             }
         }
-        if (total > 0.0) {
+        if (total > 0.0f) {
             y = sum / total;
             has = true;
-        } else if (lane_w(3) > 0.0) {
+        } else if (lane_w(3) > 0.0f) {
             y = lane_y(3) / lane_w(3);
             has = true;
         }
-    } else if (method == FJ_METER_HIGHLIGHT_WEIGHTED && lane_w(0) < 1e-12) {
-        if (lane_w(3) > 0.0) {
+    } else if (method == FJ_METER_HIGHLIGHT_WEIGHTED && lane_w(0) < 1e-12f) {
+        if (lane_w(3) > 0.0f) {
             y = lane_y(3) / lane_w(3);
             has = true;
         }
-    } else if (lane_w(0) > 0.0) {
+    } else if (lane_w(0) > 0.0f) {
         y = lane_y(0) / lane_w(0);
         has = true;
     }
@@ -298,9 +298,9 @@ __global__ void ae_median_kernel(AeArgs a) {
         }
         cumulative += c;
     }
-    const double width = static_cast<double>(max_y) / static_cast<double>(kAeBins);
-    const double fraction = in_bin > 0 ? static_cast<double>(target - cumulative) / static_cast<double>(in_bin) : 0.0;
-    *a.scale = scale_from((static_cast<double>(bin) + fraction) * width);
+    const float width = max_y / static_cast<float>(kAeBins);
+    const float fraction = in_bin > 0 ? static_cast<float>(target - cumulative) / static_cast<float>(in_bin) : 0.0f;
+    *a.scale = scale_from((static_cast<float>(bin) + fraction) * width);
 }
 
 }  // namespace

@@ -32,7 +32,20 @@ struct FftPlanDev {
     // 16,18,20,24,25}, one butterfly per thread, `threads` >= n / radix of every pass.
     int inplace = 0;
     int threads = 0;
+    // Bluestein plan (n not 2,3,5-smooth): the length-n DFT is the chirp-z
+    // convolution X_k = c_k sum_j (x_j c_j) b_(k-j), c_j = exp(-i pi j^2 / n),
+    // b = conj(c), evaluated with two in-place transforms of the smooth
+    // length m >= 2n - 1 that the pass fields above describe (their twiddles
+    // relative to table + m). table[chirp_offset + j] = c_j for j < n and
+    // table[spectrum_offset + k] = the length-m DFT of the wrapped b, k < m.
+    int bluestein = 0;
+    int m = 0;
+    int chirp_offset = 0;
+    int spectrum_offset = 0;
 };
+
+// Length of the transform the pass fields run (m for a Bluestein plan).
+__host__ __device__ inline int fft_length(const FftPlanDev& plan) { return plan.bluestein ? plan.m : plan.n; }
 
 // Up to three planes transformed by one launch (blockIdx.y selects the plane).
 struct DctRowsArgs {
@@ -87,13 +100,14 @@ struct TransposeArgs {
 
 // Host-side plan. A 2,3,5-smooth n gets the in-place engine's composite
 // radices (fewest passes, then the largest smallest radix, so the fewest
-// threads), odd radices first; any other n the ping-pong engine's radices (4 first, then 2, 3, 5,
-// then other primes).
+// threads), odd radices first; any other n a Bluestein plan over the
+// smallest smooth m >= 2n - 1 (n up to kMaxLdsFft / 2).
 bool make_fft_plan(int n, FftPlanDev& plan);
 // Smallest length >= n that has an in-place plan (0 if none up to kMaxLdsFft).
 int inplace_fft_length(int n);
-// exp(-2 pi i e / n) for e < n, then the per-pass twiddles of the plan, in
-// float2 pairs; and exp(-i pi k / 2n) for k < n.
+// exp(-2 pi i e / L) for e < L = fft_length(plan of n), then the per-pass
+// twiddles of the plan, in float2 pairs (then a Bluestein plan's chirp and
+// spectrum); and exp(-i pi k / 2n) for k < n.
 std::vector<float> fft_twiddles(int n);
 std::vector<float> dct_half_twiddles(int n);
 

@@ -367,35 +367,36 @@ __device__ __forceinline__ void pass_ip(float2* buf, int n, int ns, const float2
 // plan without 20 or 25 runs code with fewer live registers.
 template <int kMaxR>
 __device__ void fft_inplace(float2* buf, const FftPlanDev& plan, const float2* table) {
+    const int len = fft_length(plan);
     int ns = 1;
     for (int p = 0; p < plan.passes; ++p) {
         const int r = plan.radix[p];
-        const float2* stw = table + plan.n + plan.tw_offset[p];
-        const float2* stw1 = table + plan.n + plan.tw1_offset[p];
+        const float2* stw = table + len + plan.tw_offset[p];
+        const float2* stw1 = table + len + plan.tw1_offset[p];
         const unsigned magic = plan.ns_magic[p];
         switch (r) {
-            case 2: pass_ip<2>(buf, plan.n, ns, stw, stw1, magic); break;
-            case 3: pass_ip<3>(buf, plan.n, ns, stw, stw1, magic); break;
-            case 4: pass_ip<4>(buf, plan.n, ns, stw, stw1, magic); break;
-            case 5: pass_ip<5>(buf, plan.n, ns, stw, stw1, magic); break;
-            case 6: pass_ip<6>(buf, plan.n, ns, stw, stw1, magic); break;
-            case 8: pass_ip<8>(buf, plan.n, ns, stw, stw1, magic); break;
-            case 9: pass_ip<9>(buf, plan.n, ns, stw, stw1, magic); break;
-            case 10: pass_ip<10>(buf, plan.n, ns, stw, stw1, magic); break;
-            case 12: pass_ip<12>(buf, plan.n, ns, stw, stw1, magic); break;
-            case 15: pass_ip<15>(buf, plan.n, ns, stw, stw1, magic); break;
-            case 16: pass_ip<16>(buf, plan.n, ns, stw, stw1, magic); break;
+            case 2: pass_ip<2>(buf, len, ns, stw, stw1, magic); break;
+            case 3: pass_ip<3>(buf, len, ns, stw, stw1, magic); break;
+            case 4: pass_ip<4>(buf, len, ns, stw, stw1, magic); break;
+            case 5: pass_ip<5>(buf, len, ns, stw, stw1, magic); break;
+            case 6: pass_ip<6>(buf, len, ns, stw, stw1, magic); break;
+            case 8: pass_ip<8>(buf, len, ns, stw, stw1, magic); break;
+            case 9: pass_ip<9>(buf, len, ns, stw, stw1, magic); break;
+            case 10: pass_ip<10>(buf, len, ns, stw, stw1, magic); break;
+            case 12: pass_ip<12>(buf, len, ns, stw, stw1, magic); break;
+            case 15: pass_ip<15>(buf, len, ns, stw, stw1, magic); break;
+            case 16: pass_ip<16>(buf, len, ns, stw, stw1, magic); break;
             case 18:
-                if constexpr (kMaxR >= 18) pass_ip<18>(buf, plan.n, ns, stw, stw1, magic);
+                if constexpr (kMaxR >= 18) pass_ip<18>(buf, len, ns, stw, stw1, magic);
                 break;
             case 20:
-                if constexpr (kMaxR >= 20) pass_ip<20>(buf, plan.n, ns, stw, stw1, magic);
+                if constexpr (kMaxR >= 20) pass_ip<20>(buf, len, ns, stw, stw1, magic);
                 break;
             case 24:
-                if constexpr (kMaxR >= 24) pass_ip<24>(buf, plan.n, ns, stw, stw1, magic);
+                if constexpr (kMaxR >= 24) pass_ip<24>(buf, len, ns, stw, stw1, magic);
                 break;
             case 25:
-                if constexpr (kMaxR >= 25) pass_ip<25>(buf, plan.n, ns, stw, stw1, magic);
+                if constexpr (kMaxR >= 25) pass_ip<25>(buf, len, ns, stw, stw1, magic);
                 break;
             default: break;  // not produced by the planner for in-place plans
         }
@@ -404,14 +405,43 @@ __device__ void fft_inplace(float2* buf, const FftPlanDev& plan, const float2* t
 }
 
 // Engines: 0 = ping-pong over two buffers; 16 / 25 = in place with radices up
-// to 16 / 25.
+// to 16 / 25; kBluestein = a Bluestein plan, in place over its length m.
+constexpr int kBluestein = 125;
 template <int kEng>
 constexpr bool kInplace = kEng != 0;
+
+// Length-n DFT of buf[0..n) by the chirp-z convolution of a Bluestein plan,
+// in place in buf[0..m): x c, forward m-point transform, x the kernel
+// spectrum, conjugate, forward again (the inverse up to conjugation and 1/m),
+// then x c. Callers synchronize before (buf written) as for any line transform.
+__device__ void bluestein_dft(float2* buf, const FftPlanDev& plan, const float2* table) {
+    const int n = plan.n, m = plan.m;
+    const float2* chirp = table + plan.chirp_offset;
+    const float2* spectrum = table + plan.spectrum_offset;
+    for (int j = threadIdx.x; j < m; j += blockDim.x) buf[j] = j < n ? cmul(buf[j], chirp[j]) : make_float2(0.0f, 0.0f);
+    __syncthreads();
+    fft_inplace<25>(buf, plan, table);
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        const float2 z = cmul(buf[k], spectrum[k]);
+        buf[k] = make_float2(z.x, -z.y);
+    }
+    __syncthreads();
+    fft_inplace<25>(buf, plan, table);
+    const float inv = 1.0f / static_cast<float>(m);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const float2 z = buf[k];
+        buf[k] = cmul(make_float2(z.x * inv, -z.y * inv), chirp[k]);
+    }
+    __syncthreads();
+}
 
 // Forward FFT with the plan's engine; returns the buffer holding the result.
 template <int kEng>
 __device__ __forceinline__ float2* fft_line(float2* a, float2* b, const FftPlanDev& plan, const float2* table) {
-    if constexpr (kInplace<kEng>) {
+    if constexpr (kEng == kBluestein) {
+        bluestein_dft(a, plan, table);
+        return a;
+    } else if constexpr (kInplace<kEng>) {
         fft_inplace<kEng>(a, plan, table);
         return a;
     } else {
@@ -759,7 +789,7 @@ __global__ __launch_bounds__(256) void transpose_kernel(TransposeArgs t) {
 // launchers
 // ---------------------------------------------------------------------------
 size_t dct_lds_bytes(const FftPlanDev& plan) {
-    const size_t n = static_cast<size_t>(plan.n);
+    const size_t n = static_cast<size_t>(fft_length(plan));
     return (plan.inplace ? n : 2 * n) * sizeof(float2);
 }
 
@@ -772,6 +802,7 @@ static int row_threads(const FftPlanDev& plan) {
 }
 
 static int engine(const FftPlanDev& plan) {
+    if (plan.bluestein) return kBluestein;
     if (!plan.inplace) return 0;
     int max_r = 0;
     for (int p = 0; p < plan.passes; ++p) max_r = plan.radix[p] > max_r ? plan.radix[p] : max_r;
@@ -784,6 +815,7 @@ hipError_t launch_dct2_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
     switch (engine(a.plan)) {
         case 16: hipLaunchKernelGGL(dct2_rows_kernel<16>, grid, block, lds, s, a); break;
         case 25: hipLaunchKernelGGL(dct2_rows_kernel<25>, grid, block, lds, s, a); break;
+        case kBluestein: hipLaunchKernelGGL(dct2_rows_kernel<kBluestein>, grid, block, lds, s, a); break;
         default: hipLaunchKernelGGL(dct2_rows_kernel<0>, grid, block, lds, s, a); break;
     }
     return hipGetLastError();
@@ -796,6 +828,7 @@ hipError_t launch_dct3_rows(const DctRowsArgs& a, int planes, hipStream_t s) {
     switch (engine(a.plan)) {
         case 16: hipLaunchKernelGGL(dct3_rows_kernel<16>, grid, block, lds, s, a); break;
         case 25: hipLaunchKernelGGL(dct3_rows_kernel<25>, grid, block, lds, s, a); break;
+        case kBluestein: hipLaunchKernelGGL(dct3_rows_kernel<kBluestein>, grid, block, lds, s, a); break;
         default: hipLaunchKernelGGL(dct3_rows_kernel<0>, grid, block, lds, s, a); break;
     }
     return hipGetLastError();
@@ -807,6 +840,9 @@ static void launch_filter(const DctRowsArgs& a, const DctTransfer& tf, dim3 grid
     switch (engine(a.plan)) {
         case 16: hipLaunchKernelGGL((dct_filter_rows_kernel<kFull, 16>), grid, block, lds, s, a, tf); break;
         case 25: hipLaunchKernelGGL((dct_filter_rows_kernel<kFull, 25>), grid, block, lds, s, a, tf); break;
+        case kBluestein:
+            hipLaunchKernelGGL((dct_filter_rows_kernel<kFull, kBluestein>), grid, block, lds, s, a, tf);
+            break;
         default: hipLaunchKernelGGL((dct_filter_rows_kernel<kFull, 0>), grid, block, lds, s, a, tf); break;
     }
 }
@@ -850,7 +886,11 @@ hipError_t fft_kernels_init(int* max_lds_n) {
                              reinterpret_cast<const void*>(dct_filter_rows_kernel<false, 25>),
                              reinterpret_cast<const void*>(dct_filter_rows_kernel<true, 0>),
                              reinterpret_cast<const void*>(dct_filter_rows_kernel<true, 16>),
-                             reinterpret_cast<const void*>(dct_filter_rows_kernel<true, 25>)};
+                             reinterpret_cast<const void*>(dct_filter_rows_kernel<true, 25>),
+                             reinterpret_cast<const void*>(dct2_rows_kernel<kBluestein>),
+                             reinterpret_cast<const void*>(dct3_rows_kernel<kBluestein>),
+                             reinterpret_cast<const void*>(dct_filter_rows_kernel<false, kBluestein>),
+                             reinterpret_cast<const void*>(dct_filter_rows_kernel<true, kBluestein>)};
     *max_lds_n = kMaxLdsFft;
     for (const void* k : kernels) {
         if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {

@@ -2,6 +2,7 @@
 // into butterfly radices and double-precision twiddle tables.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <map>
 #include <vector>
 
@@ -81,30 +82,16 @@ bool make_fft_plan(int n, FftPlanDev& plan) {
     plan.n = n;
     std::vector<int> radices;
     int threads = 0;
-    if (inplace_radices(n, radices, threads)) {
-        plan.inplace = 1;
-        plan.threads = threads;
-    } else {
-        if (n > kMaxPingPongFft) return false;
-        radices.clear();
-        int m = n;
-        while (m % 4 == 0) {
-            radices.push_back(4);
-            m /= 4;
-        }
-        while (m % 2 == 0) {
-            radices.push_back(2);
-            m /= 2;
-        }
-        for (int p = 3; p * p <= m || p <= m; p += 2) {
-            while (m % p == 0) {
-                radices.push_back(p);
-                m /= p;
-            }
-            if (m == 1) break;
-        }
-        if (m != 1) radices.push_back(m);
+    if (!inplace_radices(n, radices, threads)) {
+        // Chirp-z over the smallest smooth length that holds the linear
+        // convolution of two n-point sequences.
+        const int m = inplace_fft_length(2 * n - 1);
+        if (m == 0 || !inplace_radices(m, radices, threads)) return false;
+        plan.bluestein = 1;
+        plan.m = m;
     }
+    plan.inplace = 1;
+    plan.threads = threads;
     if (static_cast<int>(radices.size()) > kMaxFftPasses) return false;
     plan.passes = static_cast<int>(radices.size());
     int ns = 1, offset = 0;
@@ -121,6 +108,10 @@ bool make_fft_plan(int n, FftPlanDev& plan) {
         offset += ns;
         ns *= radices[i];
     }
+    if (plan.bluestein) {
+        plan.chirp_offset = plan.m + offset;
+        plan.spectrum_offset = plan.chirp_offset + n;
+    }
     return true;
 }
 
@@ -132,28 +123,84 @@ int inplace_fft_length(int n) {
     return 0;
 }
 
+namespace {
+
+// Forward DFT of a 2,3,5-smooth length in double precision (recursive
+// decimation in time), for the Bluestein kernel spectrum.
+void dft_smooth(std::vector<std::complex<double>>& x) {
+    const size_t n = x.size();
+    if (n <= 1) return;
+    size_t r = 0;
+    for (size_t p : {size_t(2), size_t(3), size_t(5)})
+        if (n % p == 0) {
+            r = p;
+            break;
+        }
+    const size_t q = n / r;
+    std::vector<std::vector<std::complex<double>>> sub(r, std::vector<std::complex<double>>(q));
+    for (size_t s = 0; s < r; ++s)
+        for (size_t j = 0; j < q; ++j) sub[s][j] = x[j * r + s];
+    for (auto& v : sub) dft_smooth(v);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (size_t k = 0; k < n; ++k) {
+        std::complex<double> acc = 0.0;
+        for (size_t s = 0; s < r; ++s) {
+            const double a = -two_pi * static_cast<double>((s * k) % n) / static_cast<double>(n);
+            acc += sub[s][k % q] * std::complex<double>(std::cos(a), std::sin(a));
+        }
+        x[k] = acc;
+    }
+}
+
+}  // namespace
+
 std::vector<float> fft_twiddles(int n) {
     FftPlanDev plan;
     make_fft_plan(n, plan);
+    const int len = fft_length(plan);
     size_t stage = 0;
     for (int p = 0, ns = 1; p < plan.passes; ns *= plan.radix[p], ++p) stage += static_cast<size_t>(ns) * plan.radix[p];
-    std::vector<float> t(2 * (static_cast<size_t>(n) + stage));
+    const size_t extra = plan.bluestein ? static_cast<size_t>(n) + static_cast<size_t>(plan.m) : 0;
+    std::vector<float> t(2 * (static_cast<size_t>(len) + stage + extra));
     const double two_pi = 6.283185307179586476925286766559;
     auto put = [&](size_t i, long long e) {
-        const double a = -two_pi * static_cast<double>(e % n) / n;
+        const double a = -two_pi * static_cast<double>(e % len) / len;
         t[2 * i] = static_cast<float>(std::cos(a));
         t[2 * i + 1] = static_cast<float>(std::sin(a));
     };
-    for (int e = 0; e < n; ++e) put(static_cast<size_t>(e), e);
+    for (int e = 0; e < len; ++e) put(static_cast<size_t>(e), e);
     for (int p = 0, ns = 1; p < plan.passes; ns *= plan.radix[p], ++p) {
         const int R = plan.radix[p];
-        const int stride = n / (ns * R);
+        const int stride = len / (ns * R);
         for (int k = 0; k < ns; ++k)
             for (int r = 1; r < R; ++r)
-                put(static_cast<size_t>(n) + plan.tw_offset[p] + static_cast<size_t>(k) * (R - 1) + (r - 1),
+                put(static_cast<size_t>(len) + plan.tw_offset[p] + static_cast<size_t>(k) * (R - 1) + (r - 1),
                     static_cast<long long>(r) * k * stride);
         for (int k = 0; k < ns; ++k)
-            put(static_cast<size_t>(n) + plan.tw1_offset[p] + static_cast<size_t>(k), static_cast<long long>(k) * stride);
+            put(static_cast<size_t>(len) + plan.tw1_offset[p] + static_cast<size_t>(k), static_cast<long long>(k) * stride);
+    }
+    if (plan.bluestein) {
+        // c_j = exp(-i pi j^2 / n) with j^2 reduced mod 2n (exact in integers).
+        const double pi = 3.14159265358979323846264338327950;
+        std::vector<std::complex<double>> c(n);
+        for (int j = 0; j < n; ++j) {
+            const long long e = (static_cast<long long>(j) * j) % (2LL * n);
+            const double a = -pi * static_cast<double>(e) / n;
+            c[j] = std::complex<double>(std::cos(a), std::sin(a));
+            t[2 * (plan.chirp_offset + static_cast<size_t>(j))] = static_cast<float>(c[j].real());
+            t[2 * (plan.chirp_offset + static_cast<size_t>(j)) + 1] = static_cast<float>(c[j].imag());
+        }
+        // b = conj(c) at offsets 0..n-1 and m-1..m-n+1 (the negative ones).
+        std::vector<std::complex<double>> b(plan.m, 0.0);
+        for (int j = 0; j < n; ++j) {
+            b[j] = std::conj(c[j]);
+            if (j) b[plan.m - j] = std::conj(c[j]);
+        }
+        dft_smooth(b);
+        for (int k = 0; k < plan.m; ++k) {
+            t[2 * (plan.spectrum_offset + static_cast<size_t>(k))] = static_cast<float>(b[k].real());
+            t[2 * (plan.spectrum_offset + static_cast<size_t>(k)) + 1] = static_cast<float>(b[k].imag());
+        }
     }
     return t;
 }

@@ -16,7 +16,11 @@ constexpr unsigned int kHbPeakOverflowBit = 1u << 10;
 struct HbState {
     float partial_max[kHbMaxBlocks];
     unsigned int partial_invalid[kHbMaxBlocks];
-    double protection, range, shape, inv_norm, delta;
+    // Per-frame curve constants, prepared in FP64 by one thread and handed
+    // to the per-pixel FP32 pass: the protection as a float pair (p = hi +
+    // lo), 1 / range, the shape and the boost amplitude delta / g(shape).
+    // The exponent's slope shape / range is a float pair too (c = hi + lo).
+    float protection_hi, protection_lo, inv_range, slope_hi, slope_lo, amplitude;
     float maximum, peak;
     int applied;
 };

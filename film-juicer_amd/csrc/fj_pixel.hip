@@ -19,8 +19,8 @@ namespace fjx {
 struct LdsLayout {
     float* film;        // 4n or 7n floats
     float* print;       // 4n' floats
-    float4* spec;       // 2k float4
-    float4* mallett;    // 2k float4
+    UniformF4 spec;     // 2k float4, scalar loads from the device table
+    UniformF4 mallett;  // 2k float4, scalar loads from the device table
 };
 
 template <bool kPrint>
@@ -28,14 +28,10 @@ __device__ __forceinline__ LdsLayout stage_tables(const PixelParams& p, float* l
     LdsLayout L;
     const int film_count = (p.dev.dir_mode ? 7 : 4) * p.dev.n;
     const int print_count = kPrint ? 4 * p.pr.n : 0;
-    const int spec_count = kPrint ? 2 * p.pr.k : 0;
-    const int mallett_count = p.ex.mallett ? 2 * p.ex.mallett_k : 0;
-    L.spec = reinterpret_cast<float4*>(lds);
-    L.mallett = L.spec + spec_count;
-    L.film = reinterpret_cast<float*>(L.mallett + mallett_count);
+    L.spec = uniform_table(p.pr.spec);
+    L.mallett = uniform_table(p.ex.mallett);
+    L.film = lds;
     L.print = L.film + film_count;
-    for (int i = threadIdx.x; i < spec_count; i += blockDim.x) L.spec[i] = p.pr.spec[i];
-    for (int i = threadIdx.x; i < mallett_count; i += blockDim.x) L.mallett[i] = p.ex.mallett[i];
     for (int i = threadIdx.x; i < film_count; i += blockDim.x) L.film[i] = p.dev.curves[i];
     for (int i = threadIdx.x; i < print_count; i += blockDim.x) L.print[i] = p.pr.curves[i];
     __syncthreads();
@@ -45,7 +41,7 @@ __device__ __forceinline__ LdsLayout stage_tables(const PixelParams& p, float* l
 // Film exposure of one pixel: from the camera carrier with the route
 // correction (JuicerCudaDeviceHelpers.cuh:746-755, :838-848) when a carrier
 // exists, else straight from the source pixel.
-__device__ __forceinline__ void film_exposure(const PixelParams& p, const float4* mallett, const float in[4],
+__device__ __forceinline__ void film_exposure(const PixelParams& p, UniformF4 mallett, const float in[4],
                                               long long idx, float E[3]) {
     if (p.carrier[0]) {
 #pragma unroll
@@ -71,8 +67,8 @@ __device__ __forceinline__ void pixel_stage_body(const PixelParams& p, int tiles
     } else {
         L.film = const_cast<float*>(p.dev.curves);
         L.print = const_cast<float*>(p.pr.curves);
-        L.spec = const_cast<float4*>(p.pr.spec);
-        L.mallett = const_cast<float4*>(p.ex.mallett);
+        L.spec = uniform_table(p.pr.spec);
+        L.mallett = uniform_table(p.ex.mallett);
     }
     const int width = p.io.width;
     const long long total = static_cast<long long>(width) * p.io.height;
@@ -205,9 +201,7 @@ namespace fjx {
 size_t fused_lds_bytes(const PixelParams& p) {
     const size_t film = static_cast<size_t>(p.dev.dir_mode ? 7 : 4) * p.dev.n;
     const size_t prn = p.print ? static_cast<size_t>(4) * p.pr.n : 0;
-    const size_t spec = p.print ? static_cast<size_t>(8) * p.pr.k : 0;
-    const size_t mallett = p.ex.mallett ? static_cast<size_t>(8) * p.ex.mallett_k : 0;
-    return (film + prn + spec + mallett) * sizeof(float);
+    return (film + prn) * sizeof(float);
 }
 
 hipError_t launch_clear_word(unsigned int* word, hipStream_t stream) {

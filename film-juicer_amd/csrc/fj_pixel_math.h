@@ -63,17 +63,32 @@ __device__ __forceinline__ void tc_coord(float q, int& base, float& frac) {
     }
 }
 
+// A wave-uniform table (the spectral loops read entry i for every lane at
+// once) viewed through the constant address space: loads at a uniform index
+// become scalar loads into SGPRs, with no LDS staging and no VGPRs.
+typedef __attribute__((address_space(4))) const float UniformF;
+struct UniformF4 {
+    const UniformF* f;  // 4 floats per entry
+};
+__device__ __forceinline__ UniformF4 uniform_table(const float4* p) {
+    return UniformF4{(const UniformF*)p};  // address-space cast (a C-style cast is the only form allowed)
+}
+__device__ __forceinline__ float4 uniform_at(UniformF4 t, int i) {
+    return make_float4(t.f[4 * i], t.f[4 * i + 1], t.f[4 * i + 2], t.f[4 * i + 3]);
+}
+
 // Mallett 2019 spectral reconstruction exposure in B/G/R
 // (JuicerCudaDeviceHelpers.cuh:619-668, :699-703).
-__device__ __forceinline__ void mallett_exposure(const ExposureParams& ex, const float4* table, const float rgb[3],
+__device__ __forceinline__ void mallett_exposure(const ExposureParams& ex, UniformF4 table, const float rgb[3],
                                                  float E[3]) {
     float s[3];
     mat3v(ex.rgb_to_srgb, rgb, s);
     const float r = fmaxf(0.0f, fin0(s[0])), g = fmaxf(0.0f, fin0(s[1])), b = fmaxf(0.0f, fin0(s[2]));
     float eb = 0.0f, eg = 0.0f, er = 0.0f;
+#pragma unroll 4
     for (int i = 0; i < ex.mallett_k; ++i) {
-        const float4 w = table[2 * i];
-        const float4 sens = table[2 * i + 1];
+        const float4 w = uniform_at(table, 2 * i);
+        const float4 sens = uniform_at(table, 2 * i + 1);
         const float spd = fmaf(r, w.x, fmaf(g, w.y, b * w.z));
         eb = fmaf(spd, sens.x, eb);
         eg = fmaf(spd, sens.y, eg);
@@ -88,7 +103,7 @@ __device__ __forceinline__ void mallett_exposure(const ExposureParams& ex, const
 // (reconstruct_film_raw_device, JuicerCudaDeviceHelpers.cuh:509-617,
 // :670-708): XYZ -> chromaticity -> tri2quad -> bicubic TC LUT -> x sum;
 // Mallett inputs take the spectral path above.
-__device__ __forceinline__ void camera_raw(const ExposureParams& ex, const float4* mallett_table, const float in[3],
+__device__ __forceinline__ void camera_raw(const ExposureParams& ex, UniformF4 mallett_table, const float in[3],
                                            float E[3]) {
     float rgb[3];
     if (ex.decode_cctf) {
@@ -160,7 +175,7 @@ __device__ __forceinline__ float exposure_scale(const ExposureParams& ex) {
 }
 
 // Camera film linear exposure in B/G/R: the raw exposure times the scale.
-__device__ __forceinline__ void camera_exposure(const ExposureParams& ex, const float4* mallett_table,
+__device__ __forceinline__ void camera_exposure(const ExposureParams& ex, UniformF4 mallett_table,
                                                 const float in[3], float E[3]) {
     camera_raw(ex, mallett_table, in, E);
     const float s = exposure_scale(ex);
@@ -369,11 +384,13 @@ __device__ __forceinline__ unsigned int develop_dir_final(const DevelopParams& d
 // 81-λ print exposure over the compacted active wavelengths, then the print
 // curves (JuicerCudaDeviceHelpers.cuh:866-1071). A NaN transmitted sample
 // contributes zero, as in the reference.
-__device__ __forceinline__ void print_inner(const PrintParams& pr, const float4* spec, const float D[3], float inner[3]) {
+__device__ __forceinline__ void print_inner(const PrintParams& pr, UniformF4 spec, const float D[3],
+                                            float inner[3]) {
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll 4
     for (int i = 0; i < pr.k; ++i) {
-        const float4 e = spec[2 * i];
-        const float4 s = spec[2 * i + 1];
+        const float4 e = uniform_at(spec, 2 * i);
+        const float4 s = uniform_at(spec, 2 * i + 1);
         const float ex = fmaf(D[0], e.x, fmaf(D[1], e.y, fmaf(D[2], e.z, e.w)));
         float t = hw_exp2(ex);
         t = (t == t) ? t : 0.0f;
@@ -386,7 +403,7 @@ __device__ __forceinline__ void print_inner(const PrintParams& pr, const float4*
     inner[2] = fmaf(a2, pr.normalizer, pr.preflash[2]);
 }
 
-__device__ __forceinline__ void print_develop(const PrintParams& pr, const float4* spec, const float* curves,
+__device__ __forceinline__ void print_develop(const PrintParams& pr, UniformF4 spec, const float* curves,
                                               float D[3]) {
     float inner[3];
     print_inner(pr, spec, D, inner);
@@ -400,7 +417,8 @@ __device__ __forceinline__ void print_develop(const PrintParams& pr, const float
 
 // Enlarger-linear print exposure, the quantity the enlarger diffusion blurs
 // (print_linear_for_diffusion_device, JuicerCudaDeviceHelpers.cuh:997-1010).
-__device__ __forceinline__ void print_linear(const PrintParams& pr, const float4* spec, const float D[3], float lin[3]) {
+__device__ __forceinline__ void print_linear(const PrintParams& pr, UniformF4 spec, const float D[3],
+                                             float lin[3]) {
     float inner[3];
     print_inner(pr, spec, D, inner);
 #pragma unroll

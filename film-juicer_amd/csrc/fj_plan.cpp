@@ -512,8 +512,11 @@ bool check_spatial_dir(const FjPreparedHostData& p, const FjFrame& f, int max_li
         !same_extent(sd.full_frame_extent, f.full_frame_extent))
         return d.malformed("spatial_dir.frame");
     const int w = f.render_window.x2 - f.render_window.x1, h = f.render_window.y2 - f.render_window.y1;
-    FftPlanDev plan;
-    if (w > max_line || h > max_line || !make_fft_plan(w, plan) || !make_fft_plan(h, plan))
+    // Each side's line transform (a Bluestein plan's m-point one for a side
+    // that is not 2,3,5-smooth) must fit the LDS engine.
+    FftPlanDev plan_w, plan_h;
+    if (!make_fft_plan(w, plan_w) || !make_fft_plan(h, plan_h) || fft_length(plan_w) > max_line ||
+        fft_length(plan_h) > max_line)
         return d.unsupported("spatial DIR frame side above the LDS line limit");
     // A host that defines the boundary functions certifies each recursive
     // component's reflected boundary on this frame.
