@@ -570,6 +570,24 @@ FJ_EXPORT FjStatus fjx_glare_field(FjCuda* cuda, int32_t device_id, const FjScan
     }
 }
 
+FJ_EXPORT FjStatus fjx_build_tc_lut(FjCuda* cuda, int32_t device_id, const FjFilmTcLutInput* input, float* samples,
+                                    size_t count, uint64_t stream, FjErrorBuffer* error) {
+    try {
+        if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
+        if (!input) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null TC LUT input");
+        FjStatus st{};
+        DeviceState* dev = device_state(cuda, device_id, st, error);
+        if (!dev) return st;
+        return build_tc_lut(*dev, *input, samples, count, reinterpret_cast<hipStream_t>(stream), error);
+    } catch (const std::bad_alloc&) {
+        return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "film_tc_lut allocation failure");
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fjx_build_tc_lut");
+    }
+}
+
 FJ_EXPORT FjStatus fjx_capture_grain_input(FjCuda* cuda, int32_t device_id, float* density_c, float* density_m,
                                            float* density_y, int64_t samples, FjErrorBuffer* error) {
     try {

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/fj/juicer_cuda_api.h"
+#include "../../include/fj/fj_mi355x_ext.h"
 #include "../../include/fj/juicer_cuda_call_scope.h"
 #include "fj_device.h"
 #include "fj_fft.h"
@@ -123,6 +124,9 @@ FjStatus highlight_boost_planes(DeviceState& dev, const FjHighlightBoost& boost,
 FjStatus glare_field(DeviceState& dev, const FjScannerPostEffects& post, uint64_t seed, uint32_t medium, int origin_x,
                      int origin_y, int width, int height, float* field, float* uniforms, hipStream_t stream,
                      FjErrorBuffer* error);
+// The film TC-LUT build (fjx_build_tc_lut) on `dev`'s workspace.
+FjStatus build_tc_lut(DeviceState& dev, const FjFilmTcLutInput& input, float* samples, size_t count,
+                      hipStream_t stream, FjErrorBuffer* error);
 // The checks a boost descriptor must pass (recipe bound, finite fields).
 bool highlight_boost_valid(const FjHighlightBoost& boost);
 

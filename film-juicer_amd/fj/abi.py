@@ -283,6 +283,13 @@ FjScannerLutFailure = _s("FjScannerLutFailure", [
 FjGaussianKernelInput = _s("FjGaussianKernelInput", [("radius", i32), ("sigma_px", f32)])
 FjGaussianKernelView = _s("FjGaussianKernelView", [("radius", i32), ("sigma_px", f32), ("weights", FjFloatSpan)])
 FjGaussianKernelFailure = _s("FjGaussianKernelFailure", [("reason", u32)])
+# Film TC-LUT build input (fj_mi355x_ext.h; the reference's test record of
+# tests/ffi/juicer_test_values.h:964-973).
+FjFilmTcLutInput = _s("FjFilmTcLutInput", [
+    ("spectra", FjFloatSpan), ("sensitivity_rgb", FjFloatSpan), ("reference_illuminant", FjFloatSpan),
+    ("projection_white_xyz", f32 * 3), ("spectral_blur", f32), ("method", u32), ("apply_surface", u32),
+    ("surface_rgb", FjFloatSpan), ("compression_active", u32), ("hull_available", u32),
+    ("hull_center_xy", f32 * 2), ("hull_xy", FjFloatSpan)])
 FjFrameGeometry = _s("FjFrameGeometry", [("full_frame", FjExtent), ("pixel_size_um", f32)])
 FjDirFrameMath = _s("FjDirFrameMath", [
     ("present", u32), ("component_count", u32), ("recipe_hash", u64), ("render_extent", FjExtent),

@@ -75,6 +75,9 @@ def lib():
         L.fjx_highlight_boost_apply.restype = abi.FjStatus
         L.fjx_highlight_boost_apply.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjHighlightBoost), C.c_void_p,
                                                 C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64, eb]
+        L.fjx_build_tc_lut.restype = abi.FjStatus
+        L.fjx_build_tc_lut.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjFilmTcLutInput), C.c_void_p, C.c_size_t,
+                                       C.c_uint64, eb]
         L.fjx_glare_field.restype = abi.FjStatus
         L.fjx_glare_field.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjScannerPostEffects), C.c_uint64,
                                       C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,

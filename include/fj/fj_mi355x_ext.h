@@ -103,6 +103,37 @@ FjStatus fjx_glare_field(FjCuda* cuda, int32_t device_id, const FjScannerPostEff
                          uint32_t medium, int32_t origin_x, int32_t origin_y, int32_t width, int32_t height,
                          float* field, float* uniforms, uint64_t stream, FjErrorBuffer* error);
 
+/* Film TC-LUT build input: the reference's FjFilmTcLutInput
+ * (tests/ffi/juicer_test_values.h:964-973; layout identical). Spans are host
+ * memory: spectra 192 x 192 x 81 (C, M, wavelength), sensitivity_rgb 81 x 3,
+ * reference_illuminant 81, surface_rgb 3 x 15 (read when method is Hanatos
+ * and apply_surface is 1), hull_xy 1025 x 2 (read when compression_active and
+ * hull_available). method: 0 Hanatos, 2 Arctic. */
+typedef struct FjFilmTcLutInput {
+    FjFloatSpan spectra, sensitivity_rgb, reference_illuminant;
+    float projection_white_xyz[3];
+    float spectral_blur;
+    uint32_t method, apply_surface;
+    FjFloatSpan surface_rgb;
+    uint32_t compression_active, hull_available;
+    float hull_center_xy[2];
+    FjFloatSpan hull_xy;
+} FjFilmTcLutInput;
+
+/* Builds the film TC LUT on the GPU (the reference's host build
+ * build_tc_lut, rust/film-juicer-core/src/reconstruction.rs:594-677, reached
+ * through fj_test_reconstruction_tc_lut): optional spectral blur, the
+ * sensitivity integration (Arctic: relit and divided by the projected white's
+ * neutral response; Hanatos: optional surface correction) and the
+ * input-gamut remap, in the reference's FP32 operations and order, so the
+ * 147456 words (R, G, B, 0 per cell) equal the reference's. `samples` is host
+ * memory of `count` >= 147456 floats; waits for `stream`. Failures: the
+ * reference's preparation failures and diagnostics (TcError), unsupported
+ * input for malformed spans or tags (the table is then left untouched;
+ * after a preparation failure it is zeroed). */
+FjStatus fjx_build_tc_lut(FjCuda* cuda, int32_t device_id, const FjFilmTcLutInput* input, float* samples,
+                          size_t count, uint64_t stream, FjErrorBuffer* error);
+
 /* Diagnostic with no counterpart in the reference: arms a one-shot copy of
  * the three C/M/Y film-density planes entering the visual-grain stage of the
  * next frame rendered on `device_id` (the developed density of the grain

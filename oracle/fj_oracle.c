@@ -2034,3 +2034,15 @@ int fjo_plan_scan(const FjoPlan* plan, const double density_cmy[3], double rgb[3
 void fjo_gamut_compress(const FjOutputColor* color, double rgb[3]) { gamut_compress(color, rgb); }
 
 void fjo_encode(const FjOutputColor* color, double rgb[3]) { px_encode(color, rgb); }
+
+/* FNV-1a 64 of a byte string (the reference's table digests,
+ * tests/ffi/tc_lut_test.cpp:71-80). */
+uint64_t fjo_fnv1a64(const void* data, size_t bytes) {
+    const unsigned char* p = (const unsigned char*)data;
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (size_t i = 0; i < bytes; ++i) {
+        h ^= p[i];
+        h *= 0x100000001b3ull;
+    }
+    return h;
+}

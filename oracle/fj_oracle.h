@@ -93,6 +93,42 @@ void fjo_plan_capture_grain_input(FjoPlan* plan, float* c, float* m, float* y);
  * (floor of the time in frames) and the source image origin (0 by default). */
 void fjo_plan_set_frame(FjoPlan* plan, uint64_t frame_token, int origin_x, int origin_y);
 
+/* Film TC-LUT build (rust reconstruction.rs:594-677): inputs as the
+ * reference's FjFilmTcLutInput (tests/ffi/juicer_test_values.h:964-973) with
+ * plain pointers; `out` receives 192 x 192 x 4 floats (R, G, B, 0). Returns 0
+ * or one of the FJO_TC_* failures (the reference's TcError variants). */
+typedef struct FjoTcLutInput {
+    const float* spectra;              /* 192 x 192 x 81, (C, M, wavelength) */
+    const float* sensitivity_rgb;      /* 81 x 3 */
+    const float* reference_illuminant; /* 81 */
+    float projection_white_xyz[3];
+    float spectral_blur;
+    uint32_t method;                   /* 0 Hanatos, 2 Arctic */
+    uint32_t apply_surface;
+    const float* surface_rgb;          /* 3 x 15 */
+    uint32_t compression_active, hull_available;
+    float hull_center_xy[2];
+    const float* hull_xy;              /* 1025 x 2 */
+} FjoTcLutInput;
+enum {
+    FJO_TC_INVALID_ILLUMINANT = 1,
+    FJO_TC_INVALID_BLUR,
+    FJO_TC_INVALID_KERNEL,
+    FJO_TC_IMPOSSIBLE_KERNEL,
+    FJO_TC_INVALID_ARCTIC_WHITE,
+    FJO_TC_INVALID_NEUTRAL,
+    FJO_TC_INVALID_SURFACE_WHITE,
+    FJO_TC_NONFINITE_INTEGRATED,
+    FJO_TC_NONFINITE_BLURRED,
+    FJO_TC_MISSING_HULL,
+    FJO_TC_UNUSABLE_COMPRESSED,
+    FJO_TC_NONFINITE_REMAPPED,
+    FJO_TC_UNUSABLE_SAMPLING,
+};
+int fjo_tc_lut_build(const FjoTcLutInput* input, float* out);
+/* FNV-1a 64 of `bytes` bytes. */
+uint64_t fjo_fnv1a64(const void* data, size_t bytes);
+
 /* Philox4x32-10 (Salmon et al. 2011) of one 128-bit counter under a 64-bit key. */
 void fjo_philox4x32_10(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]);
 /* Scanner glare: the frame's stream seed, one pixel's glare percent, and the

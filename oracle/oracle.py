@@ -71,6 +71,9 @@ def lib():
         L.fjo_gamut_compress.argtypes = [C.c_void_p, dp]
         L.fjo_encode.argtypes = [C.c_void_p, dp]
         L.fjo_plan_set_frame.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int]
+        L.fjo_tc_lut_build.argtypes = [C.c_void_p, fp]
+        L.fjo_fnv1a64.restype = C.c_uint64
+        L.fjo_fnv1a64.argtypes = [C.c_void_p, C.c_size_t]
         L.fjo_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
         L.fjo_glare_seed.restype = C.c_uint64
         L.fjo_glare_seed.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int]
@@ -458,3 +461,48 @@ def glare_plane(post, seed: int, medium: int, width: int, height: int, origin=(0
     out = np.zeros((height, width), np.float64)
     lib().fjo_glare_plane(C.byref(post), seed, medium, int(origin[0]), int(origin[1]), width, height, _dp(out))
     return out
+
+
+class FjoTcLutInput(C.Structure):
+    _fields_ = [("spectra", C.POINTER(C.c_float)), ("sensitivity_rgb", C.POINTER(C.c_float)),
+                ("reference_illuminant", C.POINTER(C.c_float)), ("projection_white_xyz", C.c_float * 3),
+                ("spectral_blur", C.c_float), ("method", C.c_uint32), ("apply_surface", C.c_uint32),
+                ("surface_rgb", C.POINTER(C.c_float)), ("compression_active", C.c_uint32),
+                ("hull_available", C.c_uint32), ("hull_center_xy", C.c_float * 2), ("hull_xy", C.POINTER(C.c_float))]
+
+
+TC_FAILURES = {
+    1: "MalformedRequiredResource component=film_tc_lut requirement=finite_nonnegative_reference_illuminant",
+    2: "MalformedRequiredResource component=film_tc_lut method=hanatos2025 requirement=finite_blur",
+    3: "MalformedRequiredResource component=film_tc_lut method=hanatos2025 requirement=finite_blur_kernel",
+    4: "MalformedRequiredResource component=film_tc_lut requirement=representable_blur_kernel",
+    5: "MalformedRequiredResource component=film_tc_lut method=arctic2026beta04 requirement=finite_d65_projection_white",
+    6: "MalformedRequiredResource component=film_tc_lut method=arctic2026beta04 "
+       "requirement=finite_positive_neutral_response",
+    7: "MalformedRequiredResource component=film_tc_lut method=hanatos2025 requirement=finite_surface_reference_white",
+    8: "MalformedRequiredResource component=film_tc_lut requirement=finite_integrated_samples",
+    9: "MalformedRequiredResource component=film_tc_lut method=hanatos2025 requirement=finite_blurred_spectra",
+    10: "MissingRequiredResource component=film_tc_lut requirement=input_compression_hull",
+    11: "MalformedRequiredResource component=input_gamut_remap requirement=finite_compressed_coordinates",
+    12: "MalformedRequiredResource component=input_gamut_remap requirement=finite_remapped_tc_lut",
+    13: "MalformedRequiredResource component=film_tc_lut requirement=usable_sampling_coordinates",
+}
+
+
+def tc_lut_build(spectra, sensitivity, illuminant, white, blur, method, surface, surface_params, compression,
+                 hull_available, hull_center, hull_xy):
+    """The TC-LUT build; returns (failure code 0..13, 192 x 192 x 4 float32 table)."""
+    arrays = [np.ascontiguousarray(a, dtype=np.float32) for a in
+              (spectra, sensitivity, illuminant, surface_params, hull_xy)]
+    rec = FjoTcLutInput(_fp(arrays[0]), _fp(arrays[1]), _fp(arrays[2]), (C.c_float * 3)(*white), blur, method,
+                        1 if surface else 0, _fp(arrays[3]), 1 if compression else 0, 1 if hull_available else 0,
+                        (C.c_float * 2)(*hull_center), _fp(arrays[4]))
+    out = np.zeros((192, 192, 4), np.float32)
+    rc = lib().fjo_tc_lut_build(C.byref(rec), _fp(out))
+    return rc, out
+
+
+def fnv1a64(array) -> int:
+    """FNV-1a 64 of an array's little-endian bytes."""
+    a = np.ascontiguousarray(array)
+    return lib().fjo_fnv1a64(a.ctypes.data_as(C.c_void_p), a.nbytes)

@@ -122,3 +122,18 @@ def test_boundary_import_records_match_reference_facts():
         r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", src,
                             "-o", os.path.join(d, "abi.o")], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr[-4000:]
+
+
+def test_tc_lut_input_layout():
+    """FjFilmTcLutInput has the reference's x86-64 layout
+    (tests/ffi/juicer_test_values.h:964-973: three 16-byte spans, a float[3]
+    white, the blur, two tags, a span, two flags, a float[2] centre, a span)."""
+    import ctypes as C
+    from fj import abi
+    t = abi.FjFilmTcLutInput
+    want = {"spectra": 0, "sensitivity_rgb": 16, "reference_illuminant": 32, "projection_white_xyz": 48,
+            "spectral_blur": 60, "method": 64, "apply_surface": 68, "surface_rgb": 72, "compression_active": 88,
+            "hull_available": 92, "hull_center_xy": 96, "hull_xy": 104}
+    for name, off in want.items():
+        assert getattr(t, name).offset == off, name
+    assert C.sizeof(t) == 120
