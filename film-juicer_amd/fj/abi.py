@@ -290,6 +290,25 @@ FjFilmTcLutInput = _s("FjFilmTcLutInput", [
     ("projection_white_xyz", f32 * 3), ("spectral_blur", f32), ("method", u32), ("apply_surface", u32),
     ("surface_rgb", FjFloatSpan), ("compression_active", u32), ("hull_available", u32),
     ("hull_center_xy", f32 * 2), ("hull_xy", FjFloatSpan)])
+# Print preparation records (fj_mi355x_ext.h; the reference's checked test
+# records of tests/ffi/juicer_test_values.h:870-947).
+FjIlluminant = _s("FjIlluminant", [("samples", f32 * 81)])
+FjPrintIlluminantParams = _s("FjPrintIlluminantParams", [
+    ("edges_nm", f32 * 4), ("transitions_nm", f32 * 4), ("cmy_cc", f32 * 3)])
+FjPrintIlluminantFailure = _s("FjPrintIlluminantFailure", [("reason", u32), ("sample", u32)])
+FjPrintPreflashInput = _s("FjPrintPreflashInput", [
+    ("film_base_density", FjFloatSpan), ("filtered_illuminant", FjFloatSpan), ("print_sensitivity_cmy", FjFloatSpan)])
+FjPrintPreflashRaw = _s("FjPrintPreflashRaw", [("cmy", f32 * 3)])
+FjPrintPreflashFailure = _s("FjPrintPreflashFailure", [("reason", u32), ("channel", u32)])
+FjPrintBalanceInput = _s("FjPrintBalanceInput", [
+    ("film_base_density", FjFloatSpan), ("film_channel_density_cmy", FjFloatSpan),
+    ("print_sensitivity_rgb", FjFloatSpan), ("filtered_main_illuminant", FjFloatSpan),
+    ("baseline_density_cmy", f32 * 3), ("compensated_density_cmy", f32 * 3), ("reference_hash", u64),
+    ("normalization_mode", u32)])
+FjPrintBalance = _s("FjPrintBalance", [
+    ("baseline_raw_rgb", f32 * 3), ("compensated_raw_rgb", f32 * 3), ("factor_midgray", f32),
+    ("factor_midgray_comp", f32), ("normalizer", f32), ("filtered_main_illuminant_hash", u64), ("hash", u64)])
+FjPrintBalanceFailure = _s("FjPrintBalanceFailure", [("reason", u32), ("channel", u32)])
 FjFrameGeometry = _s("FjFrameGeometry", [("full_frame", FjExtent), ("pixel_size_um", f32)])
 FjDirFrameMath = _s("FjDirFrameMath", [
     ("present", u32), ("component_count", u32), ("recipe_hash", u64), ("render_extent", FjExtent),

@@ -91,6 +91,16 @@ def lib():
         L.fjx_prepare_gaussian.restype = abi.FjStatus
         L.fjx_prepare_gaussian.argtypes = [C.c_void_p, C.POINTER(abi.FjGaussianKernelInput),
                                            C.POINTER(abi.FjGaussianKernelView), C.POINTER(abi.FjGaussianKernelFailure), eb]
+        L.fjx_prepare_print_illuminant.restype = abi.FjStatus
+        L.fjx_prepare_print_illuminant.argtypes = [C.POINTER(abi.FjPrintIlluminantParams), abi.FjFloatSpan,
+                                                   C.POINTER(abi.FjIlluminant),
+                                                   C.POINTER(abi.FjPrintIlluminantFailure), eb]
+        L.fjx_prepare_print_preflash.restype = abi.FjStatus
+        L.fjx_prepare_print_preflash.argtypes = [C.POINTER(abi.FjPrintPreflashInput), C.POINTER(abi.FjPrintPreflashRaw),
+                                                 C.POINTER(abi.FjPrintPreflashFailure), eb]
+        L.fjx_prepare_print_balance.restype = abi.FjStatus
+        L.fjx_prepare_print_balance.argtypes = [C.POINTER(abi.FjPrintBalanceInput), C.POINTER(abi.FjPrintBalance),
+                                                C.POINTER(abi.FjPrintBalanceFailure), eb]
         _lib = L
     return _lib
 

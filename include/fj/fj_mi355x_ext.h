@@ -170,6 +170,115 @@ FjStatus fjx_prepare_scanner_lut(void* user, const FjScannerLutInput* input, FjS
 FjStatus fjx_prepare_gaussian(void* user, const FjGaussianKernelInput* input, FjGaussianKernelView* view,
                               FjGaussianKernelFailure* failure, FjErrorBuffer* error);
 
+/* ---- Print preparation (host) ----------------------------------------------
+ * The three 81-sample preparations behind FjPrint's main illuminant, preflash
+ * and normalizer, with the reference's checked test ABI (records of
+ * tests/ffi/juicer_test_values.h:870-947, entry points of
+ * tests/ffi/juicer_test_api.h:662-671, checks and failure envelopes of
+ * rust/film-juicer-plugin/src/print_bridge.rs:187-640). Each zeroes its output
+ * and sets the failure to reason NONE (sample/channel 0xFFFFFFFF) first;
+ * malformed records or spans give FJ_STATUS_UNSUPPORTED_INPUT with reason RAW,
+ * a failure of the computation FJ_STATUS_PREPARATION_FAILURE with the
+ * reason and the first failing sample or channel, and the diagnostic
+ * "print <kind> reason=R sample|channel=N". No device is used. */
+typedef struct FjIlluminant {
+    float samples[81];
+} FjIlluminant;
+
+typedef struct FjPrintIlluminantParams {
+    float edges_nm[4];
+    float transitions_nm[4];
+    float cmy_cc[3];
+} FjPrintIlluminantParams;
+
+typedef struct FjPrintIlluminantFailure {
+    uint32_t reason;
+    uint32_t sample;
+} FjPrintIlluminantFailure;
+
+#define FJ_PRINT_ILLUMINANT_REASON_NONE UINT32_C(0)
+#define FJ_PRINT_ILLUMINANT_REASON_RAW UINT32_C(1)
+#define FJ_PRINT_ILLUMINANT_REASON_SAMPLE UINT32_C(2)
+#define FJ_PRINT_ILLUMINANT_REASON_ENERGY UINT32_C(3)
+#define FJ_PRINT_ILLUMINANT_REASON_PANIC UINT32_C(4)
+
+typedef struct FjPrintPreflashInput {
+    FjFloatSpan film_base_density;
+    FjFloatSpan filtered_illuminant;
+    FjFloatSpan print_sensitivity_cmy;
+} FjPrintPreflashInput;
+
+typedef struct FjPrintPreflashRaw {
+    float cmy[3];
+} FjPrintPreflashRaw;
+
+typedef struct FjPrintPreflashFailure {
+    uint32_t reason;
+    uint32_t channel;
+} FjPrintPreflashFailure;
+
+#define FJ_PRINT_PREFLASH_REASON_NONE UINT32_C(0)
+#define FJ_PRINT_PREFLASH_REASON_RAW UINT32_C(1)
+#define FJ_PRINT_PREFLASH_REASON_CHANNEL UINT32_C(2)
+#define FJ_PRINT_PREFLASH_REASON_PANIC UINT32_C(3)
+
+typedef struct FjPrintBalanceInput {
+    FjFloatSpan film_base_density;
+    FjFloatSpan film_channel_density_cmy;
+    FjFloatSpan print_sensitivity_rgb;
+    FjFloatSpan filtered_main_illuminant;
+    float baseline_density_cmy[3];
+    float compensated_density_cmy[3];
+    uint64_t reference_hash;
+    uint32_t normalization_mode;
+} FjPrintBalanceInput;
+
+typedef struct FjPrintBalance {
+    float baseline_raw_rgb[3];
+    float compensated_raw_rgb[3];
+    float factor_midgray;
+    float factor_midgray_comp;
+    float normalizer;
+    uint64_t filtered_main_illuminant_hash;
+    uint64_t hash;
+} FjPrintBalance;
+
+typedef struct FjPrintBalanceFailure {
+    uint32_t reason;
+    uint32_t channel;
+} FjPrintBalanceFailure;
+
+#define FJ_PRINT_BALANCE_REASON_NONE UINT32_C(0)
+#define FJ_PRINT_BALANCE_REASON_RAW UINT32_C(1)
+#define FJ_PRINT_BALANCE_REASON_BASELINE_RAW UINT32_C(2)
+#define FJ_PRINT_BALANCE_REASON_COMPENSATED_RAW UINT32_C(3)
+#define FJ_PRINT_BALANCE_REASON_BALANCE UINT32_C(4)
+#define FJ_PRINT_BALANCE_REASON_PANIC UINT32_C(5)
+
+/* The dichroic-filtered enlarger illuminant (rust print.rs:591-639, reached
+ * through fj_test_print_illuminant_prepare): per sample the C, M and Y
+ * erf-edge filters at 380 + 5 s nm, blended with the CC transmittances
+ * 10^(-cc/100), times the 81-sample `source`. FP32 throughout, erf as the
+ * reference's pinned Rust libm 0.2.15 erff, powf as the platform's. */
+FjStatus fjx_prepare_print_illuminant(const FjPrintIlluminantParams* params, FjFloatSpan source,
+                                      FjIlluminant* out_illuminant, FjPrintIlluminantFailure* out_failure,
+                                      FjErrorBuffer* error);
+
+/* The preflash raw CMY integral (rust print.rs:538-571, reached through
+ * fj_test_print_preflash_prepare): FP64 sum over the 81 samples of
+ * 10^-base x filtered light (NaN light counts 0) x finite sensitivity,
+ * checked finite and non-negative, then narrowed to f32. */
+FjStatus fjx_prepare_print_preflash(const FjPrintPreflashInput* input, FjPrintPreflashRaw* out_raw,
+                                    FjPrintPreflashFailure* out_failure, FjErrorBuffer* error);
+
+/* The print balance (rust print.rs:435-522, reached through
+ * fj_test_print_balance_prepare): FP64 baseline and compensated raw RGB
+ * integrals, the mid-grey factors exp(-mean ln raw), the normalizer of
+ * `normalization_mode` (0..3; 4..255 fail as an invalid balance, above 255
+ * as a raw input) and the FNV-1a identities of the illuminant and balance. */
+FjStatus fjx_prepare_print_balance(const FjPrintBalanceInput* input, FjPrintBalance* out_balance,
+                                   FjPrintBalanceFailure* out_failure, FjErrorBuffer* error);
+
 #ifdef __cplusplus
 }
 #endif

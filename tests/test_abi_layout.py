@@ -137,3 +137,43 @@ def test_tc_lut_input_layout():
     for name, off in want.items():
         assert getattr(t, name).offset == off, name
     assert C.sizeof(t) == 120
+
+
+def test_print_preparation_layouts():
+    """The print-preparation records have the reference's layouts (the
+    compile-time assertions of rust film-juicer-plugin/src/print_bridge.rs:50-54,
+    644-679), both in ctypes and in the C header compiled by gcc."""
+    import ctypes as C
+    import subprocess
+    import tempfile
+    from fj import abi
+    want = {
+        "FjPrintIlluminantParams": (44, {"edges_nm": 0, "transitions_nm": 16, "cmy_cc": 32}),
+        "FjPrintIlluminantFailure": (8, {"reason": 0, "sample": 4}),
+        "FjIlluminant": (324, {"samples": 0}),
+        "FjPrintPreflashInput": (48, {"film_base_density": 0, "filtered_illuminant": 16,
+                                      "print_sensitivity_cmy": 32}),
+        "FjPrintPreflashRaw": (12, {"cmy": 0}),
+        "FjPrintBalanceInput": (104, {"film_base_density": 0, "film_channel_density_cmy": 16,
+                                      "print_sensitivity_rgb": 32, "filtered_main_illuminant": 48,
+                                      "baseline_density_cmy": 64, "compensated_density_cmy": 76,
+                                      "reference_hash": 88, "normalization_mode": 96}),
+        "FjPrintBalance": (56, {"baseline_raw_rgb": 0, "compensated_raw_rgb": 12, "factor_midgray": 24,
+                                "factor_midgray_comp": 28, "normalizer": 32,
+                                "filtered_main_illuminant_hash": 40, "hash": 48}),
+        "FjPrintBalanceFailure": (8, {"reason": 0, "channel": 4}),
+    }
+    lines = ["#include <stddef.h>", "#include \"fj/fj_mi355x_ext.h\""]
+    for name, (size, fields) in want.items():
+        t = getattr(abi, name)
+        assert C.sizeof(t) == size, name
+        lines.append(f"_Static_assert(sizeof({name}) == {size}, \"{name}\");")
+        for field, off in fields.items():
+            assert getattr(t, field).offset == off, (name, field)
+            lines.append(f"_Static_assert(offsetof({name}, {field}) == {off}, \"{name}.{field}\");")
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "print.c")
+        open(src, "w").write("\n".join(lines) + "\n")
+        r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", src,
+                            "-o", os.path.join(d, "print.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-4000:]
