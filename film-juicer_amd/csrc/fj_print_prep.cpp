@@ -15,6 +15,8 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <new>
+#include <vector>
 
 #include "../../include/fj/fj_mi355x_ext.h"
 
@@ -146,6 +148,27 @@ float libm_erff(float x) {
     }
     const float y = ix < 0x40c00000u ? 1.0f - erfc_tail(ix, x) : 1.0f - word_float(0x03800000u);
     return negative ? -y : y;
+}
+
+// erfcf of the same library: the erf polynomial below 0.84375 (as 1 - erf, or
+// 0.5 - (x - 0.5 + x y) above 1/4 to keep the small result exact), the erfc
+// tails up to 28, and 0 or 2 beyond.
+float libm_erfcf(float x) {
+    uint32_t ix = float_word(x);
+    const bool negative = (ix >> 31) != 0;
+    ix &= 0x7fffffffu;
+    if (ix >= 0x7f800000u) return (negative ? 2.0f : 0.0f) + 1.0f / x;  // NaN, 0 at +inf, 2 at -inf
+    if (ix < 0x3f580000u) {                                              // |x| < 0.84375
+        if (ix < 0x23800000u) return 1.0f - x;                           // |x| < 2^-56
+        const float z = x * x;
+        const float r = horner(kPp, z), s = one_plus(kQq, z);
+        const float y = r / s;
+        if (negative || ix < 0x3e800000u) return 1.0f - (x + x * y);  // x < 1/4
+        return 0.5f - (x - 0.5f + x * y);
+    }
+    if (ix < 0x41e00000u) return negative ? 2.0f - erfc_tail(ix, x) : erfc_tail(ix, x);  // |x| < 28
+    const float tiny = word_float(0x03800000u);
+    return negative ? 2.0f - tiny : tiny * tiny;
 }
 
 // ---- the checked ABI ---------------------------------------------------------
@@ -320,6 +343,63 @@ bool balance_scalars(const FjPrintBalanceInput& in, FjPrintBalance& b) {
            b.filtered_main_illuminant_hash != 0 && b.hash != 0;
 }
 
+// print_gamma_model (profile/sampled.rs:212-236): centres and sigmas divided by
+// the gamma in FP64, sigmas floored at 0.05 by a plain comparison (a NaN sigma
+// stays NaN); a gamma of exactly 1 leaves every bit as it is.
+FjDensityCurveModel print_gamma_model(const FjDensityCurveModel& m, double gamma) {
+    FjDensityCurveModel adjusted = m;
+    if (gamma != 1.0) {
+        for (int ch = 0; ch < 3; ++ch)
+            for (int layer = 0; layer < 3; ++layer) {
+                adjusted.centers[ch][layer] /= gamma;
+                const double sigma = adjusted.sigmas[ch][layer] / gamma;
+                adjusted.sigmas[ch][layer] = sigma < 0.05 ? 0.05 : sigma;
+            }
+    }
+    return adjusted;
+}
+
+struct DensityFault {
+    uint32_t reason = FJ_PRINT_DENSITY_REASON_NONE, channel = kNoIndex, layer = kNoIndex;
+};
+
+// DensityCurveModel::sample (profile.rs:482-524): FP32 from the narrowed
+// exposure and coefficients, amplitude x 0.5 erfc(-z / sqrt 2) per layer,
+// totals summed in layer order. Writes total[3] and layers[layer][channel].
+DensityFault sample_density(const FjDensityCurveModel& m, bool positive, double log_exposure, float total_out[3],
+                            float layers_out[3][3]) {
+    constexpr float kInverseSqrtTwo = 0.7071067811865475244f;
+    const float exposure = static_cast<float>(log_exposure);
+    const float sign = positive ? -1.0f : 1.0f;
+    for (int ch = 0; ch < 3; ++ch) {
+        float total = 0.0f;
+        for (int layer = 0; layer < 3; ++layer) {
+            const float center = static_cast<float>(m.centers[ch][layer]);
+            const float amplitude = static_cast<float>(m.amplitudes[ch][layer]);
+            const float sigma = static_cast<float>(m.sigmas[ch][layer]);
+            const float z = sign * (exposure - center) / sigma;
+            const float value = amplitude * (0.5f * libm_erfcf(-z * kInverseSqrtTwo));
+            if (!std::isfinite(value)) {
+                DensityFault f;
+                f.reason = FJ_PRINT_DENSITY_REASON_NONFINITE_LAYER;
+                f.channel = static_cast<uint32_t>(ch);
+                f.layer = static_cast<uint32_t>(layer);
+                return f;
+            }
+            layers_out[layer][ch] = value;
+            total += value;
+        }
+        if (!std::isfinite(total)) {
+            DensityFault f;
+            f.reason = FJ_PRINT_DENSITY_REASON_NONFINITE_TOTAL;
+            f.channel = static_cast<uint32_t>(ch);
+            return f;
+        }
+        total_out[ch] = total;
+    }
+    return DensityFault{};
+}
+
 }  // namespace
 
 extern "C" {
@@ -400,6 +480,84 @@ FJ_EXPORT FjStatus fjx_prepare_print_balance(const FjPrintBalanceInput* input, F
     }
     if (diagnostic_valid) write_diagnostic(error, "balance", "channel", reason, channel);
     if (valid_failure) *failure = FjPrintBalanceFailure{reason, channel};
+    return status(category);
+}
+
+FJ_EXPORT FjStatus fjx_prepare_print_density(const FjPrintDensityInput* input, float* totals, float* layers,
+                                             size_t capacity, uint64_t* print_hash, FjPrintDensityFailure* failure,
+                                             FjErrorBuffer* error) {
+    const bool valid_failure = valid_extent(failure, 1), valid_hash = valid_extent(print_hash, 1);
+    if (valid_failure) *failure = FjPrintDensityFailure{FJ_PRINT_DENSITY_REASON_NONE, kNoIndex, kNoIndex, kNoIndex};
+    if (valid_hash) *print_hash = 0;
+    const bool diagnostic_valid = admit_diagnostic(error);
+    const size_t count = input && valid_extent(input, 1) ? input->count : 0;
+    const bool valid_totals = count && capacity >= count && count <= SIZE_MAX / 9 && valid_extent(totals, count * 3);
+    const bool valid_layers = !layers || (valid_totals && valid_extent(layers, count * 9));
+    if (valid_totals) std::memset(totals, 0, count * 3 * sizeof(float));
+    if (valid_totals && layers && valid_layers) std::memset(layers, 0, count * 9 * sizeof(float));
+    uint32_t category = FJ_STATUS_UNSUPPORTED_INPUT, reason = FJ_PRINT_DENSITY_REASON_RAW;
+    uint32_t index = kNoIndex, channel = kNoIndex, layer = kNoIndex;
+    if (valid_failure && valid_hash && diagnostic_valid && valid_totals && valid_layers &&
+        valid_extent(input->log_exposure, count) && input->polarity <= FJ_POLARITY_POSITIVE && input->reserved == 0) {
+        category = FJ_STATUS_PREPARATION_FAILURE;
+        const double gamma = input->gamma;
+        if (!std::isfinite(gamma) || gamma <= 0.0) {
+            reason = FJ_PRINT_DENSITY_REASON_INVALID_GAMMA;
+        } else {
+            const FjDensityCurveModel adjusted = print_gamma_model(input->model, gamma);
+            const bool positive = input->polarity == FJ_POLARITY_POSITIVE;
+            std::vector<float> t, l;
+            try {
+                t.resize(count * 3);
+                l.resize(count * 9);
+            } catch (const std::bad_alloc&) {
+                if (valid_failure) *failure = FjPrintDensityFailure{reason, index, channel, layer};
+                return status(FJ_STATUS_ALLOCATION_FAILURE);
+            }
+            DensityFault fault;
+            size_t i = 0;
+            for (; i < count; ++i) {
+                fault = sample_density(adjusted, positive, input->log_exposure[i], &t[i * 3],
+                                       reinterpret_cast<float(*)[3]>(&l[i * 9]));
+                if (fault.reason != FJ_PRINT_DENSITY_REASON_NONE) break;
+            }
+            if (fault.reason != FJ_PRINT_DENSITY_REASON_NONE) {
+                reason = fault.reason;
+                index = static_cast<uint32_t>(i);
+                channel = fault.channel;
+                layer = fault.layer;
+            } else {
+                // PrintDensityCurves identity (sampled.rs:264-274): raw words, zero signs kept.
+                uint64_t h = kFnvOffset;
+                const uint64_t n = count;
+                fnv_bytes(h, &n, 8);
+                for (size_t k = 0; k < count; ++k) {
+                    const float narrowed = static_cast<float>(input->log_exposure[k]);
+                    fnv_bytes(h, &narrowed, 4);
+                }
+                fnv_bytes(h, t.data(), t.size() * sizeof(float));
+                if (h == 0) {
+                    reason = FJ_PRINT_DENSITY_REASON_ZERO_HASH;
+                } else {
+                    std::memcpy(totals, t.data(), t.size() * sizeof(float));
+                    if (layers) std::memcpy(layers, l.data(), l.size() * sizeof(float));
+                    *print_hash = h;
+                    return status(FJ_STATUS_SUCCESS);
+                }
+            }
+        }
+    }
+    if (diagnostic_valid && error && error->capacity) {
+        char text[128];
+        std::snprintf(text, sizeof text, "print density reason=%u index=%u channel=%u layer=%u", reason, index,
+                      channel, layer);
+        size_t n = std::strlen(text);
+        if (n > error->capacity - 1) n = error->capacity - 1;
+        std::memcpy(error->data, text, n);
+        error->data[n] = '\0';
+        error->length = n;
+    }
+    if (valid_failure) *failure = FjPrintDensityFailure{reason, index, channel, layer};
     return status(category);
 }
 

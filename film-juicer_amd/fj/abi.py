@@ -309,6 +309,13 @@ FjPrintBalance = _s("FjPrintBalance", [
     ("baseline_raw_rgb", f32 * 3), ("compensated_raw_rgb", f32 * 3), ("factor_midgray", f32),
     ("factor_midgray_comp", f32), ("normalizer", f32), ("filtered_main_illuminant_hash", u64), ("hash", u64)])
 FjPrintBalanceFailure = _s("FjPrintBalanceFailure", [("reason", u32), ("channel", u32)])
+FjDensityCurveModel = _s("FjDensityCurveModel", [
+    ("centers", (C.c_double * 3) * 3), ("amplitudes", (C.c_double * 3) * 3), ("sigmas", (C.c_double * 3) * 3)])
+FjPrintDensityInput = _s("FjPrintDensityInput", [
+    ("model", FjDensityCurveModel), ("gamma", C.c_double), ("polarity", u32), ("reserved", u32),
+    ("log_exposure", C.POINTER(C.c_double)), ("count", size_t)])
+FjPrintDensityFailure = _s("FjPrintDensityFailure", [("reason", u32), ("index", u32), ("channel", u32),
+                                                     ("layer", u32)])
 FjFrameGeometry = _s("FjFrameGeometry", [("full_frame", FjExtent), ("pixel_size_um", f32)])
 FjDirFrameMath = _s("FjDirFrameMath", [
     ("present", u32), ("component_count", u32), ("recipe_hash", u64), ("render_extent", FjExtent),

@@ -101,6 +101,10 @@ def lib():
         L.fjx_prepare_print_balance.restype = abi.FjStatus
         L.fjx_prepare_print_balance.argtypes = [C.POINTER(abi.FjPrintBalanceInput), C.POINTER(abi.FjPrintBalance),
                                                 C.POINTER(abi.FjPrintBalanceFailure), eb]
+        L.fjx_prepare_print_density.restype = abi.FjStatus
+        L.fjx_prepare_print_density.argtypes = [C.POINTER(abi.FjPrintDensityInput), C.c_void_p, C.c_void_p,
+                                                C.c_size_t, C.POINTER(C.c_uint64),
+                                                C.POINTER(abi.FjPrintDensityFailure), eb]
         _lib = L
     return _lib
 

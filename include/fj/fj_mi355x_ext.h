@@ -279,6 +279,55 @@ FjStatus fjx_prepare_print_preflash(const FjPrintPreflashInput* input, FjPrintPr
 FjStatus fjx_prepare_print_balance(const FjPrintBalanceInput* input, FjPrintBalance* out_balance,
                                    FjPrintBalanceFailure* out_failure, FjErrorBuffer* error);
 
+/* A profile's parametric density model (rust film-juicer-core/src/profile.rs:
+ * 454-525): per channel C/M/Y and layer, a cumulative Gaussian of the log
+ * exposure, amplitude x Phi(+-(logE - center) / sigma); FP64 coefficients,
+ * [channel][layer]. */
+typedef struct FjDensityCurveModel {
+    double centers[3][3];
+    double amplitudes[3][3];
+    double sigmas[3][3];
+} FjDensityCurveModel;
+
+typedef struct FjPrintDensityInput {
+    FjDensityCurveModel model; /* the print profile's unadjusted model */
+    double gamma;              /* print gamma: finite and > 0; 1 leaves the model's bits untouched */
+    uint32_t polarity;         /* FJ_POLARITY_NEGATIVE / _POSITIVE of the profile */
+    uint32_t reserved;         /* 0 */
+    const double* log_exposure; /* the profile's authored exposure axis */
+    size_t count;              /* >= 1 samples */
+} FjPrintDensityInput;
+
+typedef struct FjPrintDensityFailure {
+    uint32_t reason;
+    uint32_t index;   /* failing sample, or 0xFFFFFFFF */
+    uint32_t channel; /* failing channel, or 0xFFFFFFFF */
+    uint32_t layer;   /* failing layer, or 0xFFFFFFFF */
+} FjPrintDensityFailure;
+
+#define FJ_PRINT_DENSITY_REASON_NONE UINT32_C(0)
+#define FJ_PRINT_DENSITY_REASON_RAW UINT32_C(1)
+#define FJ_PRINT_DENSITY_REASON_INVALID_GAMMA UINT32_C(2)
+#define FJ_PRINT_DENSITY_REASON_NONFINITE_LAYER UINT32_C(3)
+#define FJ_PRINT_DENSITY_REASON_NONFINITE_TOTAL UINT32_C(4)
+#define FJ_PRINT_DENSITY_REASON_ZERO_HASH UINT32_C(5)
+
+/* The print density curves of a print gamma (rust profile/sampled.rs:212-276,
+ * PrintDensityCurves): the model adjusted for the gamma in FP64 (centres and
+ * sigmas divided by it, sigmas floored at 0.05, NaN kept), then sampled at
+ * every authored exposure in FP32 with erfc as the reference's pinned Rust
+ * libm 0.2.15 erfcf. Writes `totals` (count x 3, exposure-major C/M/Y),
+ * optionally `layers` (count x 9, [sample][layer][channel]) and `print_hash`,
+ * FNV-1a of the u64 count, the f32-narrowed axis and the totals' raw words.
+ * The axis is not admitted here: it is a completed profile's (finite-or-inf,
+ * ascending once narrowed); a NaN exposure fails as a non-finite layer.
+ * Outputs are zeroed first and written only on success. Failures: unsupported
+ * input for malformed records or short outputs (reason RAW); preparation
+ * failure for an invalid gamma, a non-finite layer or total (with its sample,
+ * channel and layer) or a zero identity. No device is used. */
+FjStatus fjx_prepare_print_density(const FjPrintDensityInput* input, float* totals, float* layers, size_t capacity,
+                                   uint64_t* print_hash, FjPrintDensityFailure* out_failure, FjErrorBuffer* error);
+
 #ifdef __cplusplus
 }
 #endif
