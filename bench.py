@@ -36,6 +36,9 @@ PEAK_HBM_GBS = 8000.0
 # scaled by the factors a plain copy kernel of known size reads (x2.0 fetch, x1.0 write).
 FILTER_FETCH_KIB = 253524.7
 FILTER_WRITE_KIB = 142384.3
+# VALUBusy % of the three timed chain kernels at this workload (rocprofv3 SQ
+# counters: 100 * SQ_ACTIVE_INST_VALU * 4 / (1024 SIMDs * GRBM_GUI_ACTIVE / 8)).
+CHAIN_VALU_BUSY_PCT = {"pixel_chain": 50, "print_linear": 95, "print_scan": 76}
 
 
 def smooth_length(n: int) -> int:
@@ -393,6 +396,7 @@ def main():
             "frac": achieved_tflops / PEAK_FP32_TFLOPS,
             "flop_per_px": census,
             "kernel_ms": chain_ms,
+            "valu_busy_pct": CHAIN_VALU_BUSY_PCT,
         },
         "roofline_dct": None if dct_ms is None else {
             "bound": "hbm",

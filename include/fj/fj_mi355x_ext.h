@@ -1,7 +1,10 @@
 /*
  * fj_mi355x_ext.h — extensions of the MI355X render library beyond the
- * drop-in boundary of juicer_cuda_api.h. These entry points have no
- * counterpart in the reference; they serve benchmarks and diagnostics.
+ * drop-in boundary of juicer_cuda_api.h: stage-level entry points the
+ * reference keeps internal (grain, diffusion, highlight boost, glare), batch
+ * rendering and diagnostics, and the host preparation of the print route's
+ * tables, which the reference runs in its Rust host (each declaration cites
+ * the reference function it computes).
  */
 #ifndef FJ_MI355X_EXT_H
 #define FJ_MI355X_EXT_H
