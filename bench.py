@@ -233,6 +233,68 @@ def batched(reps: int = 2):
             "batched_over_plain": timed["plain"] / timed["batched"], "batched_equals_plain": bool(same)}
 
 
+def differentiable(reps: int = 5, fit_steps: int = 200):
+    """The differentiable print route at 8 x 1024x1024 (the fit workload:
+    Portra 400 Mallett -> filtered enlarger + preflash -> Portra Endura,
+    spatial DIR, LUT scanner, OkLch gamut, linear output; grain held fixed
+    pathwise as the increment of the library's grain stage). Times the
+    forward and forward + backward (adjoint gradient of an MSE loss with
+    respect to the input RGB and the seven parameters) per frame, each after
+    an untimed pass, then runs the fit: 200 Adam steps on the
+    Gauss-Newton-preconditioned adjoint gradient from the record's parameters
+    to hidden ones (each moved within +-25 % of its range)."""
+    import numpy as np
+    import torch
+
+    from fj import fit, synth, workloads
+    b, w, h = 8, 1024, 1024
+    p, _ = workloads.fit_print(w, h)
+    model = fit.Model(p)
+    src = torch.stack([torch.from_numpy(synth.frame(w, h, seed=synth.DEFAULT_SEED + 100 + i)) for i in range(b)])
+    src = src.to("cuda:0")
+    theta0 = fit.nominal(p.record)
+    hidden = fit.hidden_params(theta0, seed=20261203)
+    gp, _ = workloads.fit_print(w, h, grain=True)
+    delta = fit.grain_increment(model, gp, src, hidden)
+    target = fit.render(model, src, hidden, delta)
+    t = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+
+    def fwd():
+        with torch.no_grad():
+            fit.PrintRoute.apply(src, t, delta, model)
+
+    def fwd_bwd():
+        t.grad = None
+        s = src.detach().requires_grad_(True)
+        loss = fit.loss_of(fit.PrintRoute.apply(s, t, delta, model), target)
+        loss.backward()
+
+    timed = {}
+    for name, fn in (("forward", fwd), ("forward_backward", fwd_bwd)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        timed[name] = (time.perf_counter() - t0) / reps / b * 1e3
+    t0 = time.perf_counter()
+    got, losses = fit.fit(model, src, target, delta, theta0, steps=fit_steps)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    spans = np.array([fit.RANGES[n][1] - fit.RANGES[n][0] for n in fit.PARAMS])
+    err = np.abs(np.array(got) - np.array(hidden)) / spans
+    return {"workload": "8 x 1024x1024 RGBA f32 print route (Mallett Portra 400 -> Portra Endura, preflash 0.01, "
+                        "spatial DIR 20/200 um at 0.03, LUT scanner, OkLch gamut, linear output), grain fixed "
+                        "pathwise; gradients w.r.t. input RGB and 7 parameters",
+            "forward_ms_per_frame": timed["forward"], "forward_backward_ms_per_frame": timed["forward_backward"],
+            "fwd_bwd_over_fwd": timed["forward_backward"] / timed["forward"],
+            "fit": {"steps": fit_steps, "optimizer": "Adam on the Gauss-Newton-preconditioned adjoint gradient",
+                    "seconds": fit_s, "loss_first": losses[0], "loss_last": losses[-1],
+                    "max_param_error_pct_of_range": float(100 * err.max()),
+                    "param_error_pct_of_range": dict(zip(fit.PARAMS, [float(100 * e) for e in err]))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +302,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-frame plain vs batched measurement")
+    ap.add_argument("--no-fit", action="store_true", help="skip the differentiable-route measurement and fit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB float32 buffer three times (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -431,6 +494,8 @@ def main():
     result["config1"] = config1(not args.no_cpu_baseline)
     if not args.no_batch:
         result["batched"] = batched()
+    if not args.no_fit:
+        result["differentiable"] = differentiable()
     if not args.no_cpu_baseline:
         lin_rec, _ = workloads.uhd_print(WIDTH, HEIGHT, encode=False)
         lin_dst = torch.empty_like(src)

@@ -546,6 +546,55 @@ FJ_EXPORT FjStatus fjx_highlight_boost_apply(FjCuda* cuda, int32_t device_id, co
     }
 }
 
+FJ_EXPORT size_t fjx_fit_saved_floats(int32_t width, int32_t height) {
+    if (width <= 0 || height <= 0) return 0;
+    return static_cast<size_t>(9) * static_cast<size_t>(width) * static_cast<size_t>(height);
+}
+
+FJ_EXPORT FjStatus fjx_fit_forward(FjCuda* cuda, int32_t device_id, const FjPreparedHostData* nominal,
+                                   const FjFitParams* params, const float* source, int32_t width, int32_t height,
+                                   const float* grain_delta_cmy, float* destination, float* saved, uint64_t stream,
+                                   FjErrorBuffer* error) {
+    try {
+        if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
+        if (!nominal || !params) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null fit record or parameters");
+        FjStatus st = fit_precheck(*nominal, *params, width, height, error);
+        if (st.category != FJ_STATUS_SUCCESS) return st;
+        DeviceState* dev = device_state(cuda, device_id, st, error);
+        if (!dev) return st;
+        return fit_forward(*dev, *nominal, *params, source, width, height, grain_delta_cmy, destination, saved,
+                           reinterpret_cast<hipStream_t>(stream), error);
+    } catch (const std::bad_alloc&) {
+        return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fjx_fit_forward");
+    }
+}
+
+FJ_EXPORT FjStatus fjx_fit_backward(FjCuda* cuda, int32_t device_id, const FjPreparedHostData* nominal,
+                                    const FjFitParams* params, const float* source, int32_t width, int32_t height,
+                                    const float* saved, const float* grad_destination, float* grad_source,
+                                    float* grad_params, uint64_t stream, FjErrorBuffer* error) {
+    try {
+        if (bad_error_buffer(error)) return make_status(FJ_STATUS_UNSUPPORTED_INPUT);
+        NativeCall call(cuda);
+        if (!call.ok()) return call.refuse(error);
+        if (!nominal || !params) return fail(error, FJ_STATUS_UNSUPPORTED_INPUT, "null fit record or parameters");
+        FjStatus st = fit_precheck(*nominal, *params, width, height, error);
+        if (st.category != FJ_STATUS_SUCCESS) return st;
+        DeviceState* dev = device_state(cuda, device_id, st, error);
+        if (!dev) return st;
+        return fit_backward(*dev, *nominal, *params, source, width, height, saved, grad_destination, grad_source,
+                            grad_params, reinterpret_cast<hipStream_t>(stream), error);
+    } catch (const std::bad_alloc&) {
+        return fail(error, FJ_STATUS_ALLOCATION_FAILURE, "host allocation failed");
+    } catch (...) {
+        return fail(error, FJ_STATUS_INTERNAL_FAILURE, "unexpected exception in fjx_fit_backward");
+    }
+}
+
 FJ_EXPORT FjStatus fjx_glare_field(FjCuda* cuda, int32_t device_id, const FjScannerPostEffects* post, uint64_t seed,
                                    uint32_t medium, int32_t origin_x, int32_t origin_y, int32_t width, int32_t height,
                                    float* field, float* uniforms, uint64_t stream, FjErrorBuffer* error) {

@@ -126,6 +126,15 @@ hipError_t DeviceState::release() {
         if (error_events[i]) keep(hipEventDestroy(error_events[i]));
     for (int i = 0; i <= kMaxStages; ++i)
         if (stage_events[i]) keep(hipEventDestroy(stage_events[i]));
+    for (int i = 0; i < kFitRing; ++i) {
+        if (fit_events[i]) keep(hipEventDestroy(fit_events[i]));
+        fit_events[i] = nullptr;
+        fit_pending[i] = false;
+    }
+    if (fit_ring) keep(hipFree(fit_ring));
+    if (fit_staging) keep(hipHostFree(fit_staging));
+    fit_ring = nullptr;
+    fit_staging = nullptr;
     if (error_words) keep(hipFree(error_words));
     if (error_host) keep(hipHostFree(error_host));
     error_words = nullptr;
@@ -705,7 +714,8 @@ enum TableKind : uint64_t { kTcLut = 1, kFilmCurves, kPrintSpec, kPrintCurves, k
 
 uint64_t table_key(TableKind kind, uint64_t content) { return hash_bytes(&content, sizeof(content), 0xF11F0000ull + kind); }
 
-hipError_t assemble(DeviceState& dev, const FjPreparedHostData& p, PixelParams& pp, Diag& d, bool& ok) {
+hipError_t assemble(DeviceState& dev, const FjPreparedHostData& p, PixelParams& pp, Diag& d, bool& ok,
+                    const GivenTables* given = nullptr) {
     ok = false;
     const FjFilmExposure& ex = p.film_exposure;
     // Exposure: one input matrix (adaptation folded in) and one scale.
@@ -756,10 +766,15 @@ hipError_t assemble(DeviceState& dev, const FjPreparedHostData& p, PixelParams& 
 
     // Development.
     const FjFilmDevelopment& dv = p.film_development;
-    std::vector<float> film = pack_film_curves(p);
-    e = dev.table(table_key(kFilmCurves, hash_bytes(film.data(), film.size() * 4, 21)), film.data(), film.size() * 4, &ptr);
-    if (e != hipSuccess) return e;
-    pp.dev.curves = static_cast<const float*>(ptr);
+    if (given && given->film_curves) {
+        pp.dev.curves = given->film_curves;
+    } else {
+        std::vector<float> film = pack_film_curves(p);
+        e = dev.table(table_key(kFilmCurves, hash_bytes(film.data(), film.size() * 4, 21)), film.data(),
+                      film.size() * 4, &ptr);
+        if (e != hipSuccess) return e;
+        pp.dev.curves = static_cast<const float*>(ptr);
+    }
     pp.dev.n = static_cast<int>(dv.log_exposure.count);
     for (int b = 0; b < 3; ++b) pp.dev.gamma_bgr[b] = dv.gamma_rgb[2 - b];
     const FjDirCouplers& dir = p.dir_couplers;
@@ -795,12 +810,18 @@ hipError_t assemble(DeviceState& dev, const FjPreparedHostData& p, PixelParams& 
                     return hipSuccess;
                 }
         }
-        int k = 0;
-        std::vector<float> spec = pack_print_spectra(pr, k);
-        e = dev.table(table_key(kPrintSpec, hash_bytes(spec.data(), spec.size() * 4, 31)), spec.data(), spec.size() * 4, &ptr);
-        if (e != hipSuccess) return e;
-        pp.pr.spec = static_cast<const float4*>(ptr);
-        pp.pr.k = k;
+        if (given && given->print_spectra) {
+            pp.pr.spec = given->print_spectra;
+            pp.pr.k = given->print_k;
+        } else {
+            int k = 0;
+            std::vector<float> spec = pack_print_spectra(pr, k);
+            e = dev.table(table_key(kPrintSpec, hash_bytes(spec.data(), spec.size() * 4, 31)), spec.data(),
+                          spec.size() * 4, &ptr);
+            if (e != hipSuccess) return e;
+            pp.pr.spec = static_cast<const float4*>(ptr);
+            pp.pr.k = k;
+        }
         pp.pr.normalizer = pr.normalizer;
         const float pf = preflash && pr.preflash_exposure > 0.0f ? pr.preflash_exposure : 0.0f;
         for (int c = 0; c < 3; ++c) pp.pr.preflash[c] = pf > 0.0f ? pr.preflash_raw_cmy[c] * pf : 0.0f;
@@ -951,6 +972,52 @@ bool aborted(const FjAbortCallback& cb, FjRenderOutcome& out, FjErrorBuffer* err
 // Generates the glare field of a width x height window into `field` and,
 // with `weights`, blurs it through `tmp` (optics_glare_generate_kernel and
 // blur_plane_in_place, JuicerCudaScanPipeline.cu:273-314, :2361-2405).
+hipError_t dir_transfer_cached(DeviceState& dev, const FjSpatialDir& sd, int width, int height, DctTransfer& out,
+                               std::string& diag, bool& ok) {
+    ok = false;
+    const uint64_t tkey = hash_bytes(&sd.components, sizeof(sd.components),
+                                     static_cast<uint64_t>(width) << 32 | static_cast<uint32_t>(height)) ^
+                          (static_cast<uint64_t>(sd.component_count) << 56);
+    auto cached = dev.transfers.find(tkey);
+    if (cached == dev.transfers.end()) {
+        HostTransfer ht;
+        if (!dir_transfer(sd, width, height, ht, diag)) return hipSuccess;
+        DctTransfer dt;
+        const hipError_t e = upload_transfer(dev, ht, tkey, dt);
+        if (e != hipSuccess) return e;
+        if (dev.transfers.size() > 64) dev.transfers.clear();
+        cached = dev.transfers.emplace(tkey, dt).first;
+    }
+    out = cached->second;
+    ok = true;
+    return hipSuccess;
+}
+
+bool scanner_post_active(const FjScannerPostEffects& sp) { return post_active(sp) || glare_active(sp); }
+
+bool record_valid(const FjPreparedHostData& p, uint32_t& category, std::string& text) {
+    Diag d;
+    if (validate_prepared(p, d)) return true;
+    category = d.category;
+    text = d.text;
+    return false;
+}
+
+std::vector<float> packed_film_curves(const FjPreparedHostData& p) { return pack_film_curves(p); }
+
+std::vector<float> packed_print_spectra(const FjPrint& pr, int& k) { return pack_print_spectra(pr, k); }
+
+hipError_t assemble_pixel_params(DeviceState& dev, const FjPreparedHostData& p, PixelParams& pp, uint32_t& category,
+                                 std::string& text, bool& ok, const GivenTables* given) {
+    Diag d;
+    const hipError_t e = assemble(dev, p, pp, d, ok, given);
+    if (!ok) {
+        category = d.category;
+        text = d.text;
+    }
+    return e;
+}
+
 static hipError_t launch_glare(DeviceState& dev, const FjScannerPostEffects& sp, const std::vector<float>& weights,
                         uint64_t seed, uint32_t medium, int origin_x, int origin_y, int width, int height, float* field,
                         float* tmp, float* uniforms, hipStream_t stream) {
@@ -1342,22 +1409,13 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         job.width = pp.io.width;
         job.height = pp.io.height;
         job.planes = 3;
-        const uint64_t tkey = hash_bytes(&prepared.spatial_dir.components, sizeof(prepared.spatial_dir.components),
-                                         static_cast<uint64_t>(job.width) << 32 | static_cast<uint32_t>(job.height)) ^
-                              (static_cast<uint64_t>(prepared.spatial_dir.component_count) << 56);
-        auto cached = dev.transfers.find(tkey);
-        if (cached == dev.transfers.end()) {
-            HostTransfer ht;
+        {
             std::string diag;
-            if (!dir_transfer(prepared.spatial_dir, pp.io.width, pp.io.height, ht, diag))
-                return host_or(FJ_STATUS_UNSUPPORTED_INPUT, "MalformedPreparedHostData field=" + diag, error);
-            DctTransfer dt;
-            e = upload_transfer(dev, ht, tkey, dt);
+            bool tok = false;
+            e = dir_transfer_cached(dev, prepared.spatial_dir, pp.io.width, pp.io.height, job.transfer, diag, tok);
             if (e != hipSuccess) return outcome_hip(e, "DIR transfer upload", error);
-            if (dev.transfers.size() > 64) dev.transfers.clear();
-            cached = dev.transfers.emplace(tkey, dt).first;
+            if (!tok) return host_or(FJ_STATUS_UNSUPPORTED_INPUT, "MalformedPreparedHostData field=" + diag, error);
         }
-        job.transfer = cached->second;
         for (int c = 0; c < 3; ++c) {
             job.in[c] = pp.dir_planes[c];
             job.out[c] = pp.dir_planes[c];

@@ -69,6 +69,18 @@ struct DeviceState {
     float* grain_capture[3] = {};
     long long grain_capture_samples = 0;
 
+    // Upload ring of the differentiable route's per-parameter tables: kFitRing
+    // device slots of kFitSlotFloats fed from pinned staging by async copies
+    // on the caller's stream; a slot is reused after the event recorded
+    // behind its last reader.
+    static constexpr int kFitRing = 64;
+    static constexpr size_t kFitSlotFloats = size_t(1) << 16;
+    float* fit_ring = nullptr;       // device [kFitRing][kFitSlotFloats]
+    float* fit_staging = nullptr;    // pinned host, same shape
+    hipEvent_t fit_events[kFitRing] = {};
+    bool fit_pending[kFitRing] = {};
+    int fit_next = 0;
+
     // Stage timing of the last traced frame (FJ_FRAME_TRACE_VERBOSE).
     static constexpr int kMaxStages = 32;
     hipEvent_t stage_events[kMaxStages + 1] = {};
@@ -89,6 +101,31 @@ struct DeviceState {
     // Collects completed error words of earlier frames; returns OR of their bits.
     unsigned int drain_errors(bool wait_all);
 };
+
+// The DCT-domain transfer function of a spatial-DIR descriptor at a frame
+// size, built once and cached per device (ok false with `diag` on a malformed
+// descriptor).
+hipError_t dir_transfer_cached(DeviceState& dev, const FjSpatialDir& sd, int width, int height, DctTransfer& out,
+                               std::string& diag, bool& ok);
+// Whether a scanner post descriptor asks for lens blur, unsharp or glare.
+bool scanner_post_active(const FjScannerPostEffects& sp);
+// The span and tag checks of a render call's prepared record.
+bool record_valid(const FjPreparedHostData& p, uint32_t& category, std::string& text);
+// Device tables of one record placed by the caller (the fit's upload ring):
+// when given, the assembly uses them instead of uploading and caching the
+// record's own film curves and print spectra.
+struct GivenTables {
+    const float* film_curves = nullptr;     // packed_film_curves layout
+    const float4* print_spectra = nullptr;  // packed_print_spectra layout
+    int print_k = 0;
+};
+// The packed layouts the pixel kernels read.
+std::vector<float> packed_film_curves(const FjPreparedHostData& p);
+std::vector<float> packed_print_spectra(const FjPrint& pr, int& k);
+// The per-pixel stage parameters of a prepared record (tables uploaded and
+// cached by content); ok false with the status category and diagnostic.
+hipError_t assemble_pixel_params(DeviceState& dev, const FjPreparedHostData& p, PixelParams& pp, uint32_t& category,
+                                 std::string& text, bool& ok, const GivenTables* given = nullptr);
 
 // `host`: the call scope's host preparation, or null (native builds).
 FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSubmission& submission,
@@ -121,6 +158,17 @@ FjStatus highlight_boost_planes(DeviceState& dev, const FjHighlightBoost& boost,
 // (origin_x, origin_y) of the frame's stream `seed` (medium 1 print, 0 film),
 // blurred when the descriptor's glare flag asks for it, into `field`;
 // `uniforms` (2 planes, optional) receives the raw FP32 uniforms.
+// The differentiable print route (fj_fit.cpp): the host-side checks of a
+// record and parameters (no device), one frame's forward, saving
+// its state, and the backward from that state.
+FjStatus fit_precheck(const FjPreparedHostData& nominal, const FjFitParams& th, int width, int height,
+                      FjErrorBuffer* error);
+FjStatus fit_forward(DeviceState& dev, const FjPreparedHostData& nominal, const FjFitParams& th, const float* source,
+                     int width, int height, const float* delta, float* destination, float* saved, hipStream_t stream,
+                     FjErrorBuffer* error);
+FjStatus fit_backward(DeviceState& dev, const FjPreparedHostData& nominal, const FjFitParams& th, const float* source,
+                      int width, int height, const float* saved, const float* grad_destination, float* grad_source,
+                      float* grad_params, hipStream_t stream, FjErrorBuffer* error);
 FjStatus glare_field(DeviceState& dev, const FjScannerPostEffects& post, uint64_t seed, uint32_t medium, int origin_x,
                      int origin_y, int width, int height, float* field, float* uniforms, hipStream_t stream,
                      FjErrorBuffer* error);

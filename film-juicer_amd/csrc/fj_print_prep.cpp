@@ -402,6 +402,25 @@ DensityFault sample_density(const FjDensityCurveModel& m, bool positive, double 
 
 }  // namespace
 
+// The enlarger's dichroic filter curves (C, M, Y per wavelength) of the
+// reference's filter model (print.rs:6-8 edges and transitions, :616-633).
+namespace fjx {
+void dichroic_filters(float filters[81][3]) {
+    static const float kEdges[4] = {516.0f, 500.0f, 610.0f, 607.0f};
+    static const float kTransitions[4] = {12.0f, 8.0f, 8.0f, 8.0f};
+    for (int s = 0; s < 81; ++s) {
+        const float wavelength = 380.0f + 5.0f * static_cast<float>(s);
+        const float y = 0.5f * libm_erff((wavelength - kEdges[0]) / kTransitions[0]) + 0.5f;
+        const float m = wavelength <= 550.0f ? -libm_erff((wavelength - kEdges[1]) / kTransitions[1])
+                                             : libm_erff((wavelength - kEdges[2]) / kTransitions[2]);
+        const float c = -0.5f * libm_erff((wavelength - kEdges[3]) / kTransitions[3]) + 0.5f;
+        filters[s][0] = c;
+        filters[s][1] = 0.5f * m + 0.5f;
+        filters[s][2] = y;
+    }
+}
+}  // namespace fjx
+
 extern "C" {
 
 FJ_EXPORT FjStatus fjx_prepare_print_illuminant(const FjPrintIlluminantParams* params, FjFloatSpan source,

@@ -42,6 +42,8 @@ FJ_POLARITY_NEGATIVE = 0
 FJ_POLARITY_POSITIVE = 1
 FJ_RAW_HANATOS_2025 = 0
 FJ_RAW_MALLETT_2019 = 1
+FJ_RAW_ARCTIC_2026_BETA04 = 2
+FJ_INPUT_DWG, FJ_INPUT_BT2020, FJ_INPUT_ACES2065_1, FJ_INPUT_SRGB_REC709 = 0, 1, 2, 3
 FJ_FILM_DECODE_CCTF = 1
 FJ_FILM_ADAPT_INPUT = 2
 FJ_FILM_AUTO_EXPOSURE = 4
@@ -55,6 +57,7 @@ FJ_POST_GLARE = 1
 FJ_COLOR_ENCODE_CCTF = 1
 FJ_COLOR_INPUT_IS_OUTPUT_SPACE = 2
 FJ_COLOR_GAMUT_COMPRESSION = 4
+FJ_OUTPUT_ENCODING_SCHEMA = 1
 FJ_PRINT_PREFLASH = 1
 FJ_OPTICS_CAMERA_DIFFUSION = 1
 FJ_OPTICS_ENLARGER_DIFFUSION = 2
@@ -388,3 +391,12 @@ FjDirBoundaryFailure = _s("FjDirBoundaryFailure", [
 
 ALL_TYPES = {name: obj for name, obj in globals().items()
              if isinstance(obj, type) and issubclass(obj, C.Structure)}
+
+
+class FjFitParams(C.Structure):
+    """The differentiable print route's parameters (fj_mi355x_ext.h)."""
+    _fields_ = [("exposure_ev", f32), ("print_exposure", f32), ("filter_y_cc", f32), ("filter_m_cc", f32),
+                ("film_gamma", f32), ("print_gamma", f32), ("dir_k", f32), ("reserved", f32)]
+
+
+FJ_FIT_PARAM_COUNT = 7

@@ -105,6 +105,16 @@ def lib():
         L.fjx_prepare_print_density.argtypes = [C.POINTER(abi.FjPrintDensityInput), C.c_void_p, C.c_void_p,
                                                 C.c_size_t, C.POINTER(C.c_uint64),
                                                 C.POINTER(abi.FjPrintDensityFailure), eb]
+        L.fjx_fit_saved_floats.restype = C.c_size_t
+        L.fjx_fit_saved_floats.argtypes = [C.c_int32, C.c_int32]
+        L.fjx_fit_forward.restype = abi.FjStatus
+        L.fjx_fit_forward.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjPreparedHostData),
+                                      C.POINTER(abi.FjFitParams), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_uint64, eb]
+        L.fjx_fit_backward.restype = abi.FjStatus
+        L.fjx_fit_backward.argtypes = [C.c_void_p, C.c_int32, C.POINTER(abi.FjPreparedHostData),
+                                       C.POINTER(abi.FjFitParams), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, eb]
         _lib = L
     return _lib
 

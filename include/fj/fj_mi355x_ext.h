@@ -331,6 +331,79 @@ typedef struct FjPrintDensityFailure {
 FjStatus fjx_prepare_print_density(const FjPrintDensityInput* input, float* totals, float* layers, size_t capacity,
                                    uint64_t* print_hash, FjPrintDensityFailure* out_failure, FjErrorBuffer* error);
 
+/* ---- differentiable print route (no counterpart in the reference) --------
+ *
+ * The print route as a differentiable function of its input RGB and seven
+ * parameters, for fitting a recipe to target frames. The route is the
+ * reference's chain (Mallett camera exposure, film curves with negative
+ * Langmuir DIR donors, the spatial DIR core + tails, 81-wavelength filtered
+ * print exposure with preflash, print curves, scanner LUT, colour, OkLch
+ * gamut, linear output) with the visual grain held fixed pathwise: the grain
+ * stage's density increment is an input (`grain_delta_cmy`, for instance the
+ * difference fjx_grain_apply makes on a frame's densities) added where the
+ * grain stage adds it, and a constant of the gradient.
+ *
+ * The parameters act on the nominal record as the reference's recipe does:
+ *   exposure_ev     camera exposure compensation (film_exposure.manual_exposure_ev)
+ *   print_exposure  print exposure (print.exposure)
+ *   filter_y_cc,    enlarger Y and M filtration added to the record's
+ *   filter_m_cc     (print.main_cc_cmy, print.rs:113-176): the record's
+ *                   filtered illuminant times F(cc0 + x) / F(cc0) per
+ *                   channel, F = 1 - (1 - f)(1 - 10^(-cc/100)) of the
+ *                   dichroic filter curves f (print.rs:616-646), cc0 the
+ *                   record's M and Y; the print balance normalizer is the
+ *                   record's
+ *   film_gamma      film development gamma (film_development.gamma_rgb, all
+ *                   three channels; the curves are read at gamma * log E)
+ *   print_gamma     print development gamma: the print curves read at
+ *                   gamma * log E, the density model's centres and widths
+ *                   divided by gamma (profile/sampled.rs:212-236)
+ *   dir_k           the Langmuir donor K relative to the record's (donor
+ *                   K = dir_k * the record's), with the DIR-compensated
+ *                   curve axes recomputed as film/dir.rs:234-310 does.
+ * Records the fit does not cover are refused as unsupported input: a route
+ * other than negative print, TC-LUT reconstruction, input decoding, auto
+ * exposure, camera optics, active grain (the increment replaces it), film
+ * effects, scanner post effects or glare, an encoded output, and a spatial
+ * DIR that is off or not negative-donor. */
+typedef struct FjFitParams {
+    float exposure_ev;
+    float print_exposure;
+    float filter_y_cc;
+    float filter_m_cc;
+    float film_gamma;
+    float print_gamma;
+    float dir_k;
+    float reserved; /* 0 */
+} FjFitParams;
+#define FJ_FIT_PARAM_COUNT 7
+
+/* Floats of the state a forward saves for its backward: nine planes of
+ * width x height (log raw exposure, filtered DIR corrections, densities with
+ * the grain increment). */
+size_t fjx_fit_saved_floats(int32_t width, int32_t height);
+
+/* Forward of one frame on `device_id`, asynchronously on `stream`. Device
+ * pointers: `source` and `destination` are packed RGBA float32 rows (4 x
+ * width floats per row); `grain_delta_cmy` is three packed C/M/Y planes of
+ * width x height or NULL (no grain); `saved` holds fjx_fit_saved_floats.
+ * The destination receives the linear output RGB and the source alpha. */
+FjStatus fjx_fit_forward(FjCuda* cuda, int32_t device_id, const FjPreparedHostData* nominal, const FjFitParams* params,
+                         const float* source, int32_t width, int32_t height, const float* grain_delta_cmy,
+                         float* destination, float* saved, uint64_t stream, FjErrorBuffer* error);
+
+/* Backward of one frame: given the forward's `saved` state and the gradient
+ * of a loss with respect to the destination (packed RGBA; the alpha lane is
+ * ignored), writes the gradient with respect to the source RGB into
+ * `grad_source` (packed RGBA, alpha lane 0) and the gradient with respect to
+ * the seven parameters into `grad_params` (8 floats of device memory, in
+ * FjFitParams order, the eighth 0). The parameters and record must be the
+ * forward's. Asynchronous on `stream`. */
+FjStatus fjx_fit_backward(FjCuda* cuda, int32_t device_id, const FjPreparedHostData* nominal, const FjFitParams* params,
+                          const float* source, int32_t width, int32_t height, const float* saved,
+                          const float* grad_destination, float* grad_source, float* grad_params, uint64_t stream,
+                          FjErrorBuffer* error);
+
 #ifdef __cplusplus
 }
 #endif

@@ -183,6 +183,11 @@ struct FjoPlan {
      * image origin (JuicerCudaExecutor.cpp:2416-2420). */
     uint64_t frame_token;
     int origin_x, origin_y;
+    /* Differentiable print route: the print curves read at print_gamma *
+     * log E, and a fixed C/M/Y density increment in place of the grain stage
+     * (NULL: none). */
+    double print_gamma;
+    const float* fit_delta[3];
 };
 
 static void set_message(char* msg, size_t cap, const char* text) {
@@ -201,6 +206,7 @@ FjoPlan* fjo_plan_create(const FjPreparedHostData* src, char* msg, size_t cap) {
     }
     FjoPlan* plan = (FjoPlan*)calloc(1, sizeof(FjoPlan));
     plan->p = *src;
+    plan->print_gamma = 1.0;
     plan->print = src->route == FJ_ROUTE_NEGATIVE_PRINT || src->route == FJ_ROUTE_POSITIVE_PRINT;
     const int r = (int)src->scanner_lut.resolution;
     plan->res = r;
@@ -393,6 +399,7 @@ typedef struct Frame {
     double* carrier[3]; /* camera film linear exposure B/G/R, or NULL */
     double* plin[3];    /* enlarger print-linear exposure C/M/Y, or NULL */
     float* dens[3];     /* film density C/M/Y planes for the grain stage, or NULL */
+    double* dens64[3];  /* the same in FP64 when a fixed increment replaces the grain */
     int dens_out;       /* the back end stores density instead of printing */
     double* linear;   /* [h][w][3] */
     uint32_t errors;
@@ -803,7 +810,7 @@ static void print_curves(const FjoPlan* pl, const double logp[3], double D[3]) {
     const int n = (int)pr->log_exposure.count;
     for (int c = 0; c < 3; ++c) {
         const Curve cv = {pr->log_exposure.data, pr->density_cmy.data + c, n, 3};
-        D[c] = curve_at(&cv, logp[c], 1.0);
+        D[c] = curve_at(&cv, logp[c], pl->print_gamma);
     }
 }
 
@@ -1398,6 +1405,13 @@ int fjo_diffusion(const FjDiffusion* d, double* const planes[3], int W, int H) {
     return 0;
 }
 
+void fjo_plan_set_fit(FjoPlan* plan, double print_gamma, const float* c, const float* m, const float* y) {
+    plan->print_gamma = print_gamma;
+    plan->fit_delta[0] = c;
+    plan->fit_delta[1] = c ? m : NULL;
+    plan->fit_delta[2] = c ? y : NULL;
+}
+
 void fjo_plan_capture_grain_input(FjoPlan* plan, float* c, float* m, float* y) {
     plan->grain_capture[0] = c;
     plan->grain_capture[1] = c ? m : NULL;
@@ -1658,7 +1672,10 @@ static uint32_t px_back(const FjoPlan* pl, Frame* f, size_t idx, double D[3]) {
     if (f->dens_out) {
         /* visual grain follows on the density planes (develop_print_capture,
          * JuicerCudaExecutor.cpp:2198-2280) */
-        for (int c = 0; c < 3; ++c) f->dens[c][idx] = (float)D[c];
+        for (int c = 0; c < 3; ++c) {
+            if (f->dens64[0]) f->dens64[c][idx] = D[c];
+            else f->dens[c][idx] = (float)D[c];
+        }
         return 0;
     }
     if (pl->print && f->plin[0]) {
@@ -1702,7 +1719,7 @@ static void rows_density(Frame* f, int y0, int y1) {
         for (int x = 0; x < f->w; ++x) {
             const size_t idx = (size_t)y * f->w + x;
             double D[3];
-            for (int c = 0; c < 3; ++c) D[c] = (double)f->dens[c][idx];
+            for (int c = 0; c < 3; ++c) D[c] = f->dens64[0] ? f->dens64[c][idx] : (double)f->dens[c][idx];
             errs |= px_back(f->plan, f, idx, D);
         }
     add_errors(f, errs);
@@ -1906,8 +1923,12 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     }
     if (enlarger_diffusion)
         for (int c = 0; c < 3; ++c) f.plin[c] = (double*)malloc(sizeof(double) * N);
-    if (grain) {
-        for (int c = 0; c < 3; ++c) f.dens[c] = (float*)malloc(sizeof(float) * N);
+    const int fixed_grain = plan->fit_delta[0] != NULL;
+    if (grain || fixed_grain) {
+        for (int c = 0; c < 3; ++c) {
+            if (fixed_grain) f.dens64[c] = (double*)malloc(sizeof(double) * N);
+            else f.dens[c] = (float*)malloc(sizeof(float) * N);
+        }
         f.dens_out = 1;
     }
     if (rc != 0) {
@@ -1920,10 +1941,13 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
     } else {
         run_rows(&f, rows_chain, threads);
     }
-    if (rc == 0 && grain) {
-        if (plan->grain_capture[0])
+    if (rc == 0 && (grain || fixed_grain)) {
+        if (plan->grain_capture[0] && !fixed_grain)
             for (int c = 0; c < 3; ++c) memcpy(plan->grain_capture[c], f.dens[c], sizeof(float) * N);
-        rc = fjo_grain_apply(p, f.dens, width, height);
+        if (grain && !fixed_grain) rc = fjo_grain_apply(p, f.dens, width, height);
+        if (rc == 0 && fixed_grain)
+            for (int c = 0; c < 3; ++c)
+                for (size_t i = 0; i < N; ++i) f.dens64[c][i] += (double)plan->fit_delta[c][i];
         f.dens_out = 0;
         if (rc == 0) run_rows(&f, rows_density, threads);
     }
@@ -1984,6 +2008,7 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
         free(f.carrier[c]);
         free(f.plin[c]);
         free(f.dens[c]);
+        free(f.dens64[c]);
     }
     free(f.glare);
     free(f.linear);

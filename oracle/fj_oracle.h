@@ -86,6 +86,11 @@ typedef int (*FjoDiffusionFn)(const FjDiffusion* d, double* const planes[3], int
 /* Makes fjo_render use `fn` for its camera and enlarger diffusion stages
  * (NULL restores the direct convolution). */
 void fjo_plan_set_diffusion(FjoPlan* plan, FjoDiffusionFn fn, void* user);
+/* The differentiable print route's two departures from a render: the print
+ * curves read at print_gamma * log E (1 restores the render), and a fixed
+ * C/M/Y density increment (three width x height planes, NULL for none) added
+ * where the grain stage adds its increment, on a record whose grain is off. */
+void fjo_plan_set_fit(FjoPlan* plan, double print_gamma, const float* c, const float* m, const float* y);
 /* Makes fjo_render copy the C/M/Y float densities entering its grain stage
  * into three width x height buffers (NULL disarms). */
 void fjo_plan_capture_grain_input(FjoPlan* plan, float* c, float* m, float* y);

@@ -60,6 +60,7 @@ def lib():
         L.fjo_auto_exposure.argtypes = [C.c_void_p, fp, C.c_int, C.c_int, C.c_int, C.c_ssize_t]
         L.fjo_plan_set_diffusion.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.fjo_plan_capture_grain_input.argtypes = [C.c_void_p, fp, fp, fp]
+        L.fjo_plan_set_fit.argtypes = [C.c_void_p, C.c_double, fp, fp, fp]
         L.fjo_film_raw.argtypes = [C.c_void_p, fp, dp, dp, dp, dp]
         L.fjo_tc_exposure.argtypes = [fp, dp, dp]
         L.fjo_print_integral.argtypes = [C.c_void_p, dp, dp]
@@ -300,6 +301,16 @@ class Plan:
         lib().fjo_plan_set_frame(self.handle, frame_token & (2 ** 64 - 1), int(origin[0]), int(origin[1]))
         return self
 
+    def set_fit(self, print_gamma: float = 1.0, delta=None):
+        """The differentiable print route's departures from a render: the
+        print curves read at print_gamma * log E, and a fixed C/M/Y density
+        increment (a [3][H][W] array, or None) added where the grain stage adds
+        its own, on a record whose grain is off."""
+        self._fit_delta = None if delta is None else np.ascontiguousarray(delta, dtype=np.float32)
+        d = self._fit_delta
+        lib().fjo_plan_set_fit(self.handle, float(print_gamma), *([None] * 3 if d is None else [_fp(d[c]) for c in range(3)]))
+        return self
+
     def capture_grain_input(self, width: int, height: int):
         """Arms a copy of the densities entering the grain stage of the next
         render; returns the [3][H][W] float32 array it fills."""
@@ -506,3 +517,91 @@ def fnv1a64(array) -> int:
     """FNV-1a 64 of an array's little-endian bytes."""
     a = np.ascontiguousarray(array)
     return lib().fjo_fnv1a64(a.ctypes.data_as(C.c_void_p), a.nbytes)
+
+
+# ---------------------------------------------------------------------------
+# The differentiable print route's parameters on a record (FP64)
+# ---------------------------------------------------------------------------
+FIT_FIELDS = ("exposure_ev", "print_exposure", "filter_y_cc", "filter_m_cc", "film_gamma", "print_gamma", "dir_k")
+
+
+def dichroic_filters() -> np.ndarray:
+    """The enlarger's dichroic filter curves [81][C, M, Y] (print.rs:6-8,
+    :616-633), erf evaluated in FP64 on FP32 arguments."""
+    import math
+    edges, trans = (516.0, 500.0, 610.0, 607.0), (12.0, 8.0, 8.0, 8.0)
+    f = np.zeros((81, 3))
+    for s in range(81):
+        wl = float(np.float32(380.0 + 5.0 * s))
+        y = 0.5 * math.erf(float(np.float32((wl - edges[0]) / trans[0]))) + 0.5
+        m = (-math.erf(float(np.float32((wl - edges[1]) / trans[1]))) if wl <= 550.0
+             else math.erf(float(np.float32((wl - edges[2]) / trans[2]))))
+        c = -0.5 * math.erf(float(np.float32((wl - edges[3]) / trans[3]))) + 0.5
+        f[s] = (c, 0.5 * m + 0.5, y)
+    return f
+
+
+def filter_factor(f, cc):
+    """1 - (1 - f)(1 - 10^(-cc/100)) (print.rs:636-646)."""
+    return 1.0 - (1.0 - f) * (1.0 - 10.0 ** (-cc / 100.0))
+
+
+def compensated_axes(log_exposure, density_rgb, matrix_rgb, donor_k_rgb, density_ref_rgb):
+    """DIR-compensated curve axes of a negative stock (film/dir.rs:262-310):
+    coordinate = log E - sum_donor Langmuir(D; K, ref) * M[donor][receiver],
+    FP64 in the reference's order, narrowed to f32. Returns [3][N] (R, G, B)."""
+    n = len(log_exposure)
+    out = np.zeros((3, n), np.float32)
+    M = np.asarray(matrix_rgb, np.float64).reshape(3, 3)
+    for s in range(n):
+        donors = []
+        for d in range(3):
+            v, k, r = float(density_rgb[s * 3 + d]), float(donor_k_rgb[d]), float(density_ref_rgb[d])
+            donors.append(v * (k + r) / (k + v))
+        for rc in range(3):
+            corr = 0.0
+            for d in range(3):
+                corr += donors[d] * float(M[d, rc])
+            out[rc, s] = np.float32(float(log_exposure[s]) - corr)
+    return out
+
+
+def fit_record(prepared, theta):
+    """A copy of `prepared` (an fj.prepared.Prepared of the negative print
+    route) with the seven fit parameters applied as the reference's recipe
+    applies them: manual exposure EV, print exposure, film gamma (all three
+    channels), Y/M filtration added to the record's (the filtered illuminant
+    times the dichroic filter factors' ratio), donor K = dir_k x the record's
+    with the compensated axes recomputed. Returns (prepared copy, print_gamma)
+    for Plan.set_fit."""
+    import copy as _copy
+    from fj import abi
+    th = dict(zip(FIT_FIELDS, [float(v) for v in theta]))
+    q = _copy.copy(prepared)
+    q.record = abi.FjPreparedHostData.from_buffer_copy(prepared.record)
+    q.keep = list(prepared.keep)
+    r = q.record
+    r.film_exposure.manual_exposure_ev = th["exposure_ev"]
+    for c in range(3):
+        r.film_development.gamma_rgb[c] = th["film_gamma"]
+    r.print.exposure = th["print_exposure"]
+    pr = prepared.record.print
+    f = dichroic_filters()
+    ill = np.ctypeslib.as_array(pr.main_illuminant.data, shape=(81,)).astype(np.float64)
+    cm, cy = float(pr.main_cc_cmy[1]), float(pr.main_cc_cmy[2])
+    ratio = (filter_factor(f[:, 1], cm + th["filter_m_cc"]) * filter_factor(f[:, 2], cy + th["filter_y_cc"]) /
+             (filter_factor(f[:, 1], cm) * filter_factor(f[:, 2], cy)))
+    r.print.main_illuminant = q.fspan((ill * ratio).astype(np.float32))
+    dc = r.dir_couplers
+    k_rec = [float(prepared.record.dir_couplers.donor_k_rgb[d]) for d in range(3)]
+    for d in range(3):
+        dc.donor_k_rgb[d] = float(np.float32(k_rec[d])) if th["dir_k"] == 1.0 else float(np.float32(np.float32(th["dir_k"]) * np.float32(k_rec[d])))
+    dv = prepared.record.film_development
+    n = dv.log_exposure.count
+    axes = compensated_axes(np.ctypeslib.as_array(dv.log_exposure.data, shape=(n,)),
+                            np.ctypeslib.as_array(dv.density_rgb.data, shape=(3 * n,)),
+                            list(dc.matrix_rgb), [dc.donor_k_rgb[d] for d in range(3)],
+                            list(dc.density_ref_rgb))
+    for ch in range(3):
+        dc.compensated_axes_rgb[ch] = q.fspan(axes[ch])
+    return q, th["print_gamma"]
