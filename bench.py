@@ -171,8 +171,8 @@ def batched(reps: int = 2):
     import numpy as np
     import torch
 
-    from fj import descriptors, prepared, runtime, synth
-    rng = np.random.default_rng(20261203)
+    from fj import runtime, synth, workloads
+    rng = np.random.default_rng(workloads.MIXED_SEED)
     stream = torch.cuda.current_stream()
     sources = {}
     frames, records, dst_plain, dst_batch = [], [], [], []
@@ -180,16 +180,7 @@ def batched(reps: int = 2):
         if (w, h) not in sources:
             sources[(w, h)] = torch.from_numpy(synth.frame(w, h, seed=synth.DEFAULT_SEED + len(sources))).to("cuda:0")
         src = sources[(w, h)]
-        px = descriptors.pixel_size_for(w, h)
-        case = "PrintPreflash" if i % 2 == 0 else "NegativePrintHanatos"
-        p = prepared.simple_view(prepared.load(case), unsharp=True)
-        p.record.film_exposure.manual_exposure_ev = float(rng.uniform(-2.0, 2.0))
-        descriptors.spatial_dir(p.record, 20.0, 200.0, 0.03, px, w, h)
-        prepared.with_halation(p, w, h, px)
-        mist = descriptors.diffusion_recipe(1, 0.5)
-        prepared.with_diffusion(p, w, h, px, camera=mist, enlarger=mist)
-        seed = int(rng.integers(1, 2 ** 31))
-        prepared.with_grain(p, w, h, px, session_seed=seed, noise_shape=(512, 512, 256))
+        p, px, seed = workloads.mixed_frame(i, w, h, rng)
         records.append(p)
         d0, d1 = torch.empty_like(src), torch.empty_like(src)
         dst_plain.append(d0)
@@ -278,21 +269,23 @@ def differentiable(reps: int = 5, fit_steps: int = 200):
             fn()
         torch.cuda.synchronize()
         timed[name] = (time.perf_counter() - t0) / reps / b * 1e3
-    t0 = time.perf_counter()
-    got, losses = fit.fit(model, src, target, delta, theta0, steps=fit_steps)
-    torch.cuda.synchronize()
-    fit_s = time.perf_counter() - t0
-    spans = np.array([fit.RANGES[n][1] - fit.RANGES[n][0] for n in fit.PARAMS])
-    err = np.abs(np.array(got) - np.array(hidden)) / spans
-    return {"workload": "8 x 1024x1024 RGBA f32 print route (Mallett Portra 400 -> Portra Endura, preflash 0.01, "
+    result = {"workload": "8 x 1024x1024 RGBA f32 print route (Mallett Portra 400 -> Portra Endura, preflash 0.01, "
                         "spatial DIR 20/200 um at 0.03, LUT scanner, OkLch gamut, linear output), grain fixed "
                         "pathwise; gradients w.r.t. input RGB and 7 parameters",
             "forward_ms_per_frame": timed["forward"], "forward_backward_ms_per_frame": timed["forward_backward"],
-            "fwd_bwd_over_fwd": timed["forward_backward"] / timed["forward"],
-            "fit": {"steps": fit_steps, "optimizer": "Adam on the Gauss-Newton-preconditioned adjoint gradient",
-                    "seconds": fit_s, "loss_first": losses[0], "loss_last": losses[-1],
-                    "max_param_error_pct_of_range": float(100 * err.max()),
-                    "param_error_pct_of_range": dict(zip(fit.PARAMS, [float(100 * e) for e in err]))}}
+            "fwd_bwd_over_fwd": timed["forward_backward"] / timed["forward"]}
+    if fit_steps:
+        t0 = time.perf_counter()
+        got, losses = fit.fit(model, src, target, delta, theta0, steps=fit_steps)
+        torch.cuda.synchronize()
+        fit_s = time.perf_counter() - t0
+        spans = np.array([fit.RANGES[n][1] - fit.RANGES[n][0] for n in fit.PARAMS])
+        err = np.abs(np.array(got) - np.array(hidden)) / spans
+        result["fit"] = {"steps": fit_steps, "optimizer": "Adam on the Gauss-Newton-preconditioned adjoint gradient",
+                         "seconds": fit_s, "loss_first": losses[0], "loss_last": losses[-1],
+                         "max_param_error_pct_of_range": float(100 * err.max()),
+                         "param_error_pct_of_range": dict(zip(fit.PARAMS, [float(100 * e) for e in err]))}
+    return result
 
 
 def main():
@@ -303,10 +296,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-frame plain vs batched measurement")
     ap.add_argument("--no-fit", action="store_true", help="skip the differentiable-route measurement and fit")
+    ap.add_argument("--fit-only", action="store_true",
+                    help="only the differentiable route's forward / forward+backward timing (no fit), as JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB float32 buffer three times (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.fit_only:
+        print(json.dumps(differentiable(fit_steps=0)))
+        return
     if args.copy_calibration:
         import torch
         n = 64 * 1024 * 1024

@@ -47,3 +47,29 @@ def fit_print(width: int = 1024, height: int = 1024, *, grain: bool = False, see
     if grain:
         prepared.with_grain(p, width, height, pixel_um, session_seed=seed, noise_shape=(128, 128, 64))
     return p, pixel_um
+
+
+# The mixed-resolution batch: 16 x 1920x1080, 16 x UHD, 8 x 4096x1716,
+# 8 x 6144x3160 (377.4 Mpx).
+MIXED_SIZES = [(1920, 1080)] * 16 + [(3840, 2160)] * 16 + [(4096, 1716)] * 8 + [(6144, 3160)] * 8
+MIXED_SEED = 20261203
+
+
+def mixed_frame(index: int, width: int, height: int, rng, *, grain: bool = True, encode: bool = True):
+    """(prepared record, pixel size, grain session seed) of frame `index` of
+    the mixed batch: the UHD workload's stage set at width x height with its
+    own seeded exposure (EV in [-2, 2]) and grain session seed, the two
+    negative print routes of the prepared set alternating. `rng` is the
+    batch's numpy generator (two draws per frame, in frame order)."""
+    px = descriptors.pixel_size_for(width, height)
+    case = "PrintPreflash" if index % 2 == 0 else "NegativePrintHanatos"
+    p = prepared.simple_view(prepared.load(case), unsharp=True, encode=encode)
+    p.record.film_exposure.manual_exposure_ev = float(rng.uniform(-2.0, 2.0))
+    descriptors.spatial_dir(p.record, 20.0, 200.0, 0.03, px, width, height)
+    prepared.with_halation(p, width, height, px)
+    mist = descriptors.diffusion_recipe(1, 0.5)
+    prepared.with_diffusion(p, width, height, px, camera=mist, enlarger=mist)
+    seed = int(rng.integers(1, 2 ** 31))
+    if grain:
+        prepared.with_grain(p, width, height, px, session_seed=seed, noise_shape=(512, 512, 256))
+    return p, px, seed
