@@ -1,7 +1,8 @@
 """Frame-throughput benchmark of the MI355X render path (one GPU).
 
 Workload: UHD 3840x2160 RGBA float32 frame from the seeded synthetic
-generator, negative -> enlarger (filtered, preflash 0.01) -> paper print
+generator, negative -> enlarger (Y/M filtration drawn from U[-20, 20] CC with
+a fixed seed, print balance re-prepared; preflash 0.01) -> paper print
 route, Portra 400 capture on Portra Endura paper, scatter/halation on the
 camera carrier (still-film defaults: 65 um first bounce, 3 bounces,
 renormalized), camera and enlarger diffusion (Black Pro-Mist 1/2: PSF radius
@@ -422,8 +423,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "plain UHD print route, 3840x2160 RGBA f32: Portra 400 (Mallett) -> filtered "
-                        "enlarger + preflash 0.01 -> Portra Endura, spatial DIR 20 um core + 200 um tail "
+            "workload": "plain UHD print route, 3840x2160 RGBA f32: Portra 400 (Mallett) -> enlarger with "
+                        "Y %+.2f / M %+.2f CC (seeded U[-20, 20] draw, print balance re-prepared) + preflash "
+                        "0.01 -> Portra Endura, spatial DIR 20 um core + 200 um tail " % workloads.UHD_FILTRATION +
                         "weight 0.03 (DCT domain), scatter/halation (65 um, 3 bounces, renormalized) on the "
                         "camera carrier, camera + enlarger diffusion (Black Pro-Mist 1/2, PSF radius 834 px, "
                         "DCT domain), density-space grain (plug-in defaults: sub-layers, shared chroma, "

@@ -572,20 +572,28 @@ __global__ __launch_bounds__(kFitBlock) void fit_source_adjoint_kernel(FitArgs a
     block_partials<3>(part, slots, a.partials + static_cast<long long>(blockIdx.x) * kFitSlots);
 }
 
-// Sums `count` partial rows of kFitSlots floats in a fixed order into out[8].
-__global__ __launch_bounds__(kFitBlock) void fit_reduce_kernel(const float* partials, int count, float* out) {
-    __shared__ float red[kFitBlock];
+// Sums `count` partial rows of kFitSlots floats in a fixed order into out[8]:
+// 1024 threads, each summing every 128th row of its slot (eight independent
+// loads in flight per step), then a fixed-order tree over the 128 groups.
+constexpr int kReduceThreads = 1024;
+__global__ __launch_bounds__(kReduceThreads) void fit_reduce_kernel(const float* partials, int count, float* out) {
+    __shared__ float red[kReduceThreads];
     const int slot = threadIdx.x % kFitSlots, group = threadIdx.x / kFitSlots;
-    constexpr int kGroups = kFitBlock / kFitSlots;
-    float s = 0.0f;
-    for (int r = group; r < count; r += kGroups) s += partials[static_cast<long long>(r) * kFitSlots + slot];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x < kFitSlots) {
-        float t = 0.0f;
-        for (int g = 0; g < kGroups; ++g) t += red[g * kFitSlots + threadIdx.x];
-        out[threadIdx.x] = threadIdx.x < kFitUsed ? t : 0.0f;
+    constexpr int kGroups = kReduceThreads / kFitSlots;
+    float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int r = group;
+    for (; r + 7 * kGroups < count; r += 8 * kGroups) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] += partials[static_cast<long long>(r + u * kGroups) * kFitSlots + slot];
     }
+    for (; r < count; r += kGroups) acc[0] += partials[static_cast<long long>(r) * kFitSlots + slot];
+    red[threadIdx.x] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    __syncthreads();
+    for (int w = kGroups / 2; w > 0; w >>= 1) {
+        if (group < w) red[threadIdx.x] += red[threadIdx.x + w * kFitSlots];
+        __syncthreads();
+    }
+    if (threadIdx.x < kFitSlots) out[threadIdx.x] = threadIdx.x < kFitUsed ? red[threadIdx.x] : 0.0f;
 }
 
 // ---- launchers --------------------------------------------------------------
@@ -635,7 +643,7 @@ hipError_t launch_fit_adjoint(const FitArgs& a, int stage, hipStream_t stream) {
 }
 
 hipError_t launch_fit_reduce(const float* partials, int rows, float* out, hipStream_t stream) {
-    hipLaunchKernelGGL(fit_reduce_kernel, dim3(1), dim3(kFitBlock), 0, stream, partials, rows, out);
+    hipLaunchKernelGGL(fit_reduce_kernel, dim3(1), dim3(kReduceThreads), 0, stream, partials, rows, out);
     return hipGetLastError();
 }
 

@@ -427,3 +427,66 @@ def with_grain(p: Prepared, width: int, height: int, pixel_size_um: float, *, se
     nz.wang_colors = 2
     r.noise = nz
     return p
+
+
+DICHROIC_EDGES_NM = (516.0, 500.0, 610.0, 607.0)      # rust print.rs:7
+DICHROIC_TRANSITIONS_NM = (12.0, 8.0, 8.0, 8.0)       # rust print.rs:8
+
+
+def dichroic_filter_curve(channel: int) -> np.ndarray:
+    """One enlarger dichroic filter's transmission curve f (81 samples; 0 C,
+    1 M, 2 Y) of the reference's filter model (print.rs:616-633), read through
+    the library's fjx_prepare_print_illuminant: a unit source under a dense
+    filter of that channel alone transmits F = 1 - (1 - f)(1 - 0) = f."""
+    from . import runtime
+    params = abi.FjPrintIlluminantParams()
+    params.edges_nm[:] = DICHROIC_EDGES_NM
+    params.transitions_nm[:] = DICHROIC_TRANSITIONS_NM
+    params.cmy_cc[:] = [1.0e9 if c == channel else 0.0 for c in range(3)]
+    ones = np.ones(81, np.float32)
+    out, fail, err = abi.FjIlluminant(), abi.FjPrintIlluminantFailure(), runtime.ErrorBuffer()
+    st = runtime.lib().fjx_prepare_print_illuminant(C.byref(params),
+                                                    abi.FjFloatSpan(ones.ctypes.data_as(C.POINTER(C.c_float)), 81),
+                                                    C.byref(out), C.byref(fail), err.ptr())
+    if st.category != abi.FJ_STATUS_SUCCESS:
+        raise RuntimeError(f"dichroic filter curve: {err.text}")
+    return np.array(out.samples, np.float64)
+
+
+def with_filtration(p: Prepared, y_cc: float, m_cc: float) -> Prepared:
+    """Adds enlarger Y and M filtration (CC) to the record's print filtration
+    as the reference prepares it: the filtered illuminant is the source times
+    F(cc) = 1 - (1 - f)(1 - 10^(-cc/100)) per channel (print.rs:636-646), so
+    the record's illuminant is scaled by F(cc0 + x) / F(cc0) for M and Y;
+    then the print balance is recomputed with the library's
+    fjx_prepare_print_balance from the record's reference densities and
+    normalization mode (print.rs:435-522), which reproduces the record's own
+    normalizer bit for bit at zero filtration."""
+    from . import runtime
+    pr = p.record.print
+    if not p.is_print:
+        raise ValueError("filtration needs a print route")
+    ill = np.ctypeslib.as_array(pr.main_illuminant.data, shape=(81,)).astype(np.float64)
+    ratio = np.ones(81)
+    for ch, extra in ((1, m_cc), (2, y_cc)):
+        f = dichroic_filter_curve(ch)
+        cc0 = float(pr.main_cc_cmy[ch])
+        factor = lambda cc: 1.0 - (1.0 - f) * (1.0 - 10.0 ** (-cc / 100.0))  # noqa: E731
+        ratio *= factor(cc0 + extra) / factor(cc0)
+        pr.main_cc_cmy[ch] = float(np.float32(cc0 + extra))
+    pr.main_illuminant = p.fspan((ill * ratio).astype(np.float32))
+    rec = abi.FjPrintBalanceInput()
+    rec.film_base_density = pr.film_base_density
+    rec.film_channel_density_cmy = pr.film_density_cmy
+    rec.print_sensitivity_rgb = pr.sensitivity_cmy
+    rec.filtered_main_illuminant = pr.main_illuminant
+    rec.baseline_density_cmy[:] = [float(x) for x in p.arrays["film_reference.baseline_density_cmy"]]
+    rec.compensated_density_cmy[:] = [float(x) for x in p.arrays["film_reference.compensated_density_cmy"]]
+    rec.reference_hash = 1
+    rec.normalization_mode = int(p.scalars["print_normalization_mode"])
+    out, fail, err = abi.FjPrintBalance(), abi.FjPrintBalanceFailure(), runtime.ErrorBuffer()
+    st = runtime.lib().fjx_prepare_print_balance(C.byref(rec), C.byref(out), C.byref(fail), err.ptr())
+    if st.category != abi.FJ_STATUS_SUCCESS:
+        raise RuntimeError(f"print balance: {err.text}")
+    pr.normalizer = out.normalizer
+    return p

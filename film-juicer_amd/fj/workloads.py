@@ -10,18 +10,27 @@ unsharp 0.7/0.7; sRGB encode unless `encode` is false.
 """
 from __future__ import annotations
 
+import numpy as np
+
 from . import descriptors, prepared
 
 CASE = "PrintPreflash"
 GRAIN_SEED = 7
+# The UHD workload's enlarger Y and M filtration: one seeded draw from
+# U[-20, 20] CC each (added to the record's filtration).
+UHD_FILTRATION = tuple(float(v) for v in np.random.default_rng(20261203).uniform(-20.0, 20.0, 2))
 
 
 def uhd_print(width: int = 3840, height: int = 2160, *, encode: bool = True, grain: bool = True,
-              seed: int = GRAIN_SEED):
+              seed: int = GRAIN_SEED, filtration=UHD_FILTRATION):
     """(prepared record, pixel size in um) of the UHD print workload at
-    width x height (the pixel pitch is the 35 mm format's for that width)."""
+    width x height (the pixel pitch is the 35 mm format's for that width);
+    `filtration` = (Y, M) CC added to the record's, the print balance
+    re-prepared (None: the record's own)."""
     p = prepared.simple_view(prepared.load(CASE), unsharp=True, encode=encode)
     p.record.print.preflash_exposure = 0.01
+    if filtration is not None:
+        prepared.with_filtration(p, *filtration)
     pixel_um = descriptors.pixel_size_for(width, height)
     descriptors.spatial_dir(p.record, 20.0, 200.0, 0.03, pixel_um, width, height)
     prepared.with_halation(p, width, height, pixel_um)
@@ -58,9 +67,10 @@ MIXED_SEED = 20261203
 def mixed_frame(index: int, width: int, height: int, rng, *, grain: bool = True, encode: bool = True):
     """(prepared record, pixel size, grain session seed) of frame `index` of
     the mixed batch: the UHD workload's stage set at width x height with its
-    own seeded exposure (EV in [-2, 2]) and grain session seed, the two
+    own seeded exposure (EV in [-2, 2]), grain session seed and enlarger Y/M
+    filtration (each in [-20, 20] CC, print balance re-prepared), the two
     negative print routes of the prepared set alternating. `rng` is the
-    batch's numpy generator (two draws per frame, in frame order)."""
+    batch's numpy generator (four draws per frame, in frame order)."""
     px = descriptors.pixel_size_for(width, height)
     case = "PrintPreflash" if index % 2 == 0 else "NegativePrintHanatos"
     p = prepared.simple_view(prepared.load(case), unsharp=True, encode=encode)
@@ -70,6 +80,7 @@ def mixed_frame(index: int, width: int, height: int, rng, *, grain: bool = True,
     mist = descriptors.diffusion_recipe(1, 0.5)
     prepared.with_diffusion(p, width, height, px, camera=mist, enlarger=mist)
     seed = int(rng.integers(1, 2 ** 31))
+    prepared.with_filtration(p, *(float(v) for v in rng.uniform(-20.0, 20.0, 2)))
     if grain:
         prepared.with_grain(p, width, height, px, session_seed=seed, noise_shape=(512, 512, 256))
     return p, px, seed

@@ -192,3 +192,29 @@ def test_no_diagnostic_record_and_truncation():
     err = abi.FjErrorBuffer(C.cast(buf, C.c_char_p), 8, 99)
     st = lib().fjx_prepare_print_balance(C.byref(rec), C.byref(out), C.byref(fail), C.byref(err))
     assert err.length == 7 and buf.value == b"print b"
+
+
+@pytest.mark.parametrize("case", ["PrintPreflash", "NegativePrintHanatos", "PositivePrintHanatos"])
+def test_filtration_reprepares_the_record_balance(case):
+    """fj.prepared.with_filtration: at zero added filtration the re-prepared
+    balance gives the record's normalizer and illuminant bit for bit (the
+    library's balance on the record's own inputs); added Y/M filtration moves
+    the illuminant by the filter factors' ratio (checked against the oracle's
+    FP64 filter model) and the normalizer with it."""
+    from fj import prepared
+    from oracle import oracle
+    p = prepared.load(case)
+    n0 = p.record.print.normalizer
+    ill0 = np.ctypeslib.as_array(p.record.print.main_illuminant.data, shape=(81,)).copy()
+    prepared.with_filtration(p, 0.0, 0.0)
+    assert np.float32(p.record.print.normalizer).view(np.uint32) == np.float32(n0).view(np.uint32)
+    np.testing.assert_array_equal(np.ctypeslib.as_array(p.record.print.main_illuminant.data, shape=(81,)), ill0)
+    q = prepared.load(case)
+    cm, cy = float(q.record.print.main_cc_cmy[1]), float(q.record.print.main_cc_cmy[2])
+    prepared.with_filtration(q, 9.0, -6.0)
+    f = oracle.dichroic_filters()
+    want = ill0 * (oracle.filter_factor(f[:, 1], cm - 6.0) * oracle.filter_factor(f[:, 2], cy + 9.0) /
+                   (oracle.filter_factor(f[:, 1], cm) * oracle.filter_factor(f[:, 2], cy)))
+    got = np.ctypeslib.as_array(q.record.print.main_illuminant.data, shape=(81,))
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=0)
+    assert q.record.print.normalizer != n0
