@@ -140,6 +140,11 @@ struct PostParams {
     const float* weights;      // 2r+1 unsharp weights
     int radius;                // 0: no unsharp
     float amount;
+    // Gate weave: the output pixel (x, y) reads the planes at
+    // R (x - c, y - c) + c + (dx, dy) through a Mitchell filter; c is the
+    // window centre. Set only with radius 0 (the unsharp is materialized).
+    int weave;
+    float weave_dx, weave_dy, weave_cos, weave_sin;
 };
 
 }  // namespace fjx

@@ -503,8 +503,37 @@ bool check_supported(const FjPreparedHostData& p, const FjFrame& f, Diag& d) {
         std::string field;
         if (!grain_decode(p.grain, field)) return d.malformed(field.c_str());
     }
-    if (p.effects.flags != 0 || p.effects.hash != 0) return d.unsupported("film/gate effects");
+    {
+        // Effects descriptor (native/juicer_cuda_descriptors.cpp:466-482);
+        // of its families this build renders the gate weave alone.
+        const FjFilmEffects& fx = p.effects;
+        if (fx.flags & ~(FJ_EFFECTS_WEAVE | FJ_EFFECTS_FILM | FJ_EFFECTS_GATE_TRANSMITTANCE | FJ_EFFECTS_GATE_OUTPUT |
+                         FJ_EFFECTS_FULL_FRAME))
+            return d.malformed("effects.flags");
+        if ((fx.flags & (FJ_EFFECTS_FILM | FJ_EFFECTS_GATE_OUTPUT)) == 0)
+            return (fx.flags == 0 && fx.hash == 0 && fx.recipe_hash == 0) || d.malformed("effects.inactive");
+        if (fx.hash == 0 || fx.recipe_hash == 0) return d.malformed("effects.identity");
+        if (fx.flags & (FJ_EFFECTS_FILM | FJ_EFFECTS_GATE_TRANSMITTANCE))
+            return d.unsupported("film/gate dust and scratches");
+        if (fx.flags & FJ_EFFECTS_FULL_FRAME) {
+            // film_juicer_effects_full_frame_preflight (JuicerCudaExecutor.cpp:751-769).
+            const FjExtent& e = fx.full_frame_extent;
+            auto full = [&e](const FjRect& r) {
+                return r.x1 == e.x && r.y1 == e.y && r.x2 - r.x1 == e.width && r.y2 - r.y1 == e.height;
+            };
+            if (!full(f.render_window) || !full(f.source.bounds) || !full(f.full_frame_extent))
+                return d.unsupported("film effects require the complete frame domain");
+        }
+    }
     return true;
+}
+
+// The gate weave runs with gate output and the weave flag, unless a grain
+// debug view replaces the image (JuicerCudaExecutor.cpp:2582-2589, :1769-1773).
+bool weave_active(const FjPreparedHostData& p) {
+    const bool gate_output = (p.effects.flags & FJ_EFFECTS_GATE_OUTPUT) != 0 && p.effects.hash != 0;
+    const bool grain_debug = (p.grain.flags & FJ_GRAIN_ACTIVE) != 0 && p.grain.debug_view != 0;
+    return gate_output && (p.effects.flags & FJ_EFFECTS_WEAVE) != 0 && !grain_debug;
 }
 
 bool same_extent(const FjExtent& e, const FjRect& r) {
@@ -1124,7 +1153,12 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     };
     const int cus = dev.compute_units;
     const bool spatial_dir = prepared.spatial_dir.hash != 0;
-    const bool post = post_active(prepared.scanner_post);
+    const bool weave = weave_active(prepared);
+    const bool post = post_active(prepared.scanner_post) || weave;
+    // The weave resamples the unsharp mask's result, so the mask runs as its
+    // own pass first.
+    const bool unsharp_apart =
+        unsharp_separate(prepared.scanner_post) || (weave && unsharp_active(prepared.scanner_post));
     const bool halation = (prepared.optics.flags & FJ_OPTICS_SCATTER_HALATION) != 0;
     const bool camera_diffusion = (prepared.optics.flags & FJ_OPTICS_CAMERA_DIFFUSION) != 0;
     const bool enlarger_diffusion = pp.print && (prepared.optics.flags & FJ_OPTICS_ENLARGER_DIFFUSION) != 0;
@@ -1221,7 +1255,7 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
     long long post_off = -1;
     if (post) post_off = spatial_dir ? dir_off + 9 * plane : (q_off >= 0 ? q_off : take(3 * plane));
     const long long lens_off =
-        (post && (lens_active(prepared.scanner_post) || unsharp_separate(prepared.scanner_post))) ? take(plane) : -1;
+        (post && (lens_active(prepared.scanner_post) || unsharp_apart)) ? take(plane) : -1;
     const long long pad_off = padded_floats ? take(padded_floats) : -1;
     const long long glare_off = glare ? take(plane) : -1;
     const long long glare_tmp_off = glare_w.empty() ? -1 : take(plane);
@@ -1498,8 +1532,8 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         const void* wptr = nullptr;
         e = dev.table(hash_bytes(w.data(), w.size() * 4, 0x5057ull), w.data(), w.size() * 4, &wptr);
         if (e != hipSuccess) return outcome_hip(e, "unsharp weights", error);
-        const bool fused_unsharp = unsharp_active(sp) && !unsharp_separate(sp);
-        if (unsharp_separate(sp)) {
+        const bool fused_unsharp = unsharp_active(sp) && !unsharp_apart;
+        if (unsharp_apart) {
             // x + a (x - G * x) per plane: horizontal blur into the scratch
             // plane, then the vertical blur with the unsharp combine in place.
             mark("post_unsharp_wide");
@@ -1525,6 +1559,13 @@ FjRenderOutcome render_frame(DeviceState& dev, const FjFrame& frame, const FjSub
         po.weights = static_cast<const float*>(wptr);
         po.radius = fused_unsharp ? sp.unsharp_radius : 0;
         po.amount = fused_unsharp ? sp.unsharp_amount : 0.0f;
+        if (weave) {
+            po.weave = 1;
+            po.weave_dx = prepared.effects.weave_dx_px;
+            po.weave_dy = prepared.effects.weave_dy_px;
+            po.weave_cos = prepared.effects.weave_cos_rot;
+            po.weave_sin = prepared.effects.weave_sin_rot;
+        }
         mark("post_unsharp_encode");
         e = launch_post(po, stream);
         if (e != hipSuccess) return outcome_hip(e, "post stage", error);

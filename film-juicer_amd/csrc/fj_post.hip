@@ -4,7 +4,9 @@
 // Reference behaviour (JuicerCudaScanPipeline.cu:316-452, :586-660,
 // :2096-2182): horizontal Gaussian blur, vertical blur, v + a*(v - blur),
 // non-finite intermediate values replaced by zero, half-sample reflected
-// borders, then the output CCTF, a [0,1] clamp and the source alpha.
+// borders, then the output CCTF, a [0,1] clamp and the source alpha. With a
+// gate weave the unsharp mask has run before (the weave resamples its
+// result) and this pass samples the planes at the woven positions.
 #include <hip/hip_runtime.h>
 
 #include "fj_batch.h"
@@ -23,6 +25,48 @@ __device__ __forceinline__ int reflect_half(int i, int n) {
     if (n <= 1) return 0;
     while (i < 0 || i >= n) i = i < 0 ? -i - 1 : 2 * n - i - 1;
     return i;
+}
+
+// Mitchell-Netravali cubic with B = C = 1/3 (Mitchell & Netravali, 1988):
+// 7|t|^3 - 12t^2 + 16/3 inside |t| < 1, -7/3|t|^3 + 12t^2 - 20|t| + 32/3 in
+// 1 <= |t| < 2, both over 6.
+__device__ __forceinline__ float mitchell(float t) {
+    const float a = fabsf(t);
+    if (a < 1.0f) return ((7.0f * a - 12.0f) * a * a + 16.0f / 3.0f) * (1.0f / 6.0f);
+    if (a < 2.0f) return (((-7.0f / 3.0f) * a + 12.0f) * a - 20.0f) * a * (1.0f / 6.0f) + 32.0f / 18.0f;
+    return 0.0f;
+}
+
+// One plane at the woven source position of output pixel (x, y): rotation
+// about the window centre, then the shift; the position is clamped to the
+// plane and the 4 x 4 taps clamp their indices (edge replication); the 16
+// products are summed in FP64 (JuicerCudaScanPipeline.cu:1514-1558,
+// :2122-2146). A non-finite position clamps to 0 (fmaxf drops the NaN).
+__device__ __forceinline__ float weave_sample(const PostParams& p, const float* plane, int x, int y) {
+    const int W = p.io.width, H = p.io.height;
+    const float cx = 0.5f * static_cast<float>(W - 1), cy = 0.5f * static_cast<float>(H - 1);
+    const float ox = static_cast<float>(x) - cx, oy = static_cast<float>(y) - cy;
+    float sx = p.weave_cos * ox - p.weave_sin * oy + cx + p.weave_dx;
+    float sy = p.weave_sin * ox + p.weave_cos * oy + cy + p.weave_dy;
+    sx = fminf(fmaxf(sx, 0.0f), static_cast<float>(W - 1));
+    sy = fminf(fmaxf(sy, 0.0f), static_cast<float>(H - 1));
+    const int ix = static_cast<int>(floorf(sx)), iy = static_cast<int>(floorf(sy));
+    const float fx = sx - static_cast<float>(ix), fy = sy - static_cast<float>(iy);
+    float wx[4], wy[4];
+    for (int k = 0; k < 4; ++k) {
+        wx[k] = mitchell(fx + 1.0f - static_cast<float>(k));
+        wy[k] = mitchell(fy + 1.0f - static_cast<float>(k));
+    }
+    double acc = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        const int yy = min(max(iy + j - 1, 0), H - 1);
+        const float* row = plane + static_cast<long long>(yy) * W;
+        for (int i = 0; i < 4; ++i) {
+            const int xx = min(max(ix + i - 1, 0), W - 1);
+            acc = fma(static_cast<double>(wy[j]) * static_cast<double>(wx[i]), static_cast<double>(row[xx]), acc);
+        }
+    }
+    return static_cast<float>(acc);
 }
 
 }  // namespace
@@ -68,6 +112,11 @@ __global__ __launch_bounds__(256) void post_kernel(PostParams p) {
                 const float v0 = s_in[(ly + r) * in_w + tx + r];
                 const float v = fmaf(p.amount, v0 - blurred, v0);
                 out[c][q] = isfinite(v) ? v : 0.0f;
+            }
+        } else if (p.weave) {
+            for (int q = 0; q < 4; ++q) {
+                const int gx = x0 + tx, gy = y0 + ty * 4 + q;
+                out[c][q] = (gx < W && gy < H) ? weave_sample(p, src, gx, gy) : 0.0f;
             }
         } else {
             for (int q = 0; q < 4; ++q) {

@@ -72,6 +72,11 @@ FJ_DIR_OPERATOR_IDENTITY = 1
 FJ_DIR_OPERATOR_FIR_REFLECT = 2
 FJ_DIR_OPERATOR_YVV_REFLECT = 3
 FJ_GRAIN_ACTIVE = 1
+FJ_EFFECTS_WEAVE = 1
+FJ_EFFECTS_FILM = 2
+FJ_EFFECTS_GATE_TRANSMITTANCE = 4
+FJ_EFFECTS_GATE_OUTPUT = 8
+FJ_EFFECTS_FULL_FRAME = 16
 FJ_GRAIN_SUBLAYERS = 2
 FJ_GRAIN_FULL_FRAME = 4
 FJ_GRAIN_AXIS_FINITE_C = 8

@@ -490,3 +490,19 @@ def with_filtration(p: Prepared, y_cc: float, m_cc: float) -> Prepared:
         raise RuntimeError(f"print balance: {err.text}")
     pr.normalizer = out.normalizer
     return p
+
+
+def with_gate_weave(p: Prepared, dx_px: float, dy_px: float, rotation_deg: float, flags: int = 0) -> Prepared:
+    """Binds a gate-output effects descriptor whose only family is the gate
+    weave (a shift of (dx, dy) pixels and a rotation about the window centre,
+    FjFilmEffects, native/juicer_cuda_api.h:700-728); `flags` adds further
+    effects flags."""
+    fx = p.record.effects
+    fx.flags = abi.FJ_EFFECTS_GATE_OUTPUT | abi.FJ_EFFECTS_WEAVE | flags
+    fx.weave_dx_px = dx_px
+    fx.weave_dy_px = dy_px
+    fx.weave_cos_rot = float(np.float32(np.cos(np.radians(rotation_deg))))
+    fx.weave_sin_rot = float(np.float32(np.sin(np.radians(rotation_deg))))
+    fx.hash = 0x57EA5E0000000001
+    fx.recipe_hash = 0x57EA5E0000000002
+    return p

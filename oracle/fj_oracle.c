@@ -1602,6 +1602,39 @@ void fjo_glare_plane(const FjScannerPostEffects* sp, uint64_t seed, uint32_t med
                                                       sp->glare_percent, sp->glare_roughness);
 }
 
+/* Gate weave (JuicerCudaScanPipeline.cu:1514-1558, :2122-2146): output pixel
+ * (x, y) samples the interleaved RGB planes at R (x - c) + c + (dx, dy), c the
+ * window centre ((W-1)/2, (H-1)/2); the position is clamped to the plane and
+ * the 4 x 4 Mitchell taps clamp their indices. */
+static void weave_planes(const FjFilmEffects* fx, const double* in, double* out, int W, int H) {
+    const double cx = 0.5 * (W - 1), cy = 0.5 * (H - 1);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const double ox = x - cx, oy = y - cy;
+            double sx = (double)fx->weave_cos_rot * ox - (double)fx->weave_sin_rot * oy + cx + fx->weave_dx_px;
+            double sy = (double)fx->weave_sin_rot * ox + (double)fx->weave_cos_rot * oy + cy + fx->weave_dy_px;
+            sx = isnan(sx) ? 0.0 : fmin(fmax(sx, 0.0), W - 1.0);
+            sy = isnan(sy) ? 0.0 : fmin(fmax(sy, 0.0), H - 1.0);
+            const int ix = (int)floor(sx), iy = (int)floor(sy);
+            double wx[4], wy[4];
+            for (int k = 0; k < 4; ++k) {
+                wx[k] = mitchell(sx - ix + 1.0 - k);
+                wy[k] = mitchell(sy - iy + 1.0 - k);
+            }
+            double acc[3] = {0.0, 0.0, 0.0};
+            for (int j = 0; j < 4; ++j) {
+                int yy = iy + j - 1;
+                yy = yy < 0 ? 0 : (yy > H - 1 ? H - 1 : yy);
+                for (int i = 0; i < 4; ++i) {
+                    int xx = ix + i - 1;
+                    xx = xx < 0 ? 0 : (xx > W - 1 ? W - 1 : xx);
+                    for (int c = 0; c < 3; ++c) acc[c] += wy[j] * wx[i] * in[((size_t)yy * W + xx) * 3 + c];
+                }
+            }
+            for (int c = 0; c < 3; ++c) out[((size_t)y * W + x) * 3 + c] = acc[c];
+        }
+}
+
 static void unsharp_plane(double* v, int W, int H, const float* w, int r, double amount) {
     const size_t N = (size_t)W * H;
     double* h = (double*)malloc(sizeof(double) * N);
@@ -1987,6 +2020,14 @@ int fjo_render(const FjoPlan* plan, const float* src, int width, int height, int
             }
             free(pl3);
         }
+    }
+    if (rc == 0 && (p->effects.flags & FJ_EFFECTS_GATE_OUTPUT) && (p->effects.flags & FJ_EFFECTS_WEAVE) &&
+        p->effects.hash != 0 && !(grain && p->grain.debug_view != 0)) {
+        /* gate weave on the post-unsharp planes (JuicerCudaScanPipeline.cu:2122-2146) */
+        double* woven = (double*)malloc(sizeof(double) * N * 3);
+        weave_planes(&p->effects, f.linear, woven, width, height);
+        free(f.linear);
+        f.linear = woven;
     }
     if (rc == 0) {
         if (linear) memcpy(linear, f.linear, sizeof(double) * N * 3);
