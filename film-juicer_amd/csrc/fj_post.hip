@@ -6,7 +6,8 @@
 // non-finite intermediate values replaced by zero, half-sample reflected
 // borders, then the output CCTF, a [0,1] clamp and the source alpha. With a
 // gate weave the unsharp mask has run before (the weave resamples its
-// result) and this pass samples the planes at the woven positions.
+// result) and weave_encode_kernel samples the planes at the woven positions
+// in place of this pass.
 #include <hip/hip_runtime.h>
 
 #include "fj_batch.h"
@@ -113,11 +114,6 @@ __global__ __launch_bounds__(256) void post_kernel(PostParams p) {
                 const float v = fmaf(p.amount, v0 - blurred, v0);
                 out[c][q] = isfinite(v) ? v : 0.0f;
             }
-        } else if (p.weave) {
-            for (int q = 0; q < 4; ++q) {
-                const int gx = x0 + tx, gy = y0 + ty * 4 + q;
-                out[c][q] = (gx < W && gy < H) ? weave_sample(p, src, gx, gy) : 0.0f;
-            }
         } else {
             for (int q = 0; q < 4; ++q) {
                 const int gx = x0 + tx, gy = y0 + ty * 4 + q;
@@ -140,8 +136,33 @@ __global__ __launch_bounds__(256) void post_kernel(PostParams p) {
     }
 }
 
+// The weave's own pass, one pixel per thread: kept apart from post_kernel,
+// whose tiles would otherwise carry the FP64 taps' registers.
+__global__ __launch_bounds__(256) void weave_encode_kernel(PostParams p) {
+    const long long n = static_cast<long long>(p.io.width) * p.io.height;
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int gy = static_cast<int>(i / p.io.width), gx = static_cast<int>(i - static_cast<long long>(gy) * p.io.width);
+    float rgb[3];
+    for (int c = 0; c < 3; ++c) rgb[c] = weave_sample(p, p.planes + c * n, gx, gy);
+    encode_output(p.co, rgb);
+    const float* sp = reinterpret_cast<const float*>(p.io.src + gy * p.io.src_row_bytes) + gx * p.io.comps;
+    float* dp = reinterpret_cast<float*>(p.io.dst + gy * p.io.dst_row_bytes) + gx * p.io.comps;
+    if (p.io.comps == 4) {
+        *reinterpret_cast<float4*>(dp) = make_float4(rgb[0], rgb[1], rgb[2], sp[3]);
+    } else {
+        dp[0] = rgb[0]; dp[1] = rgb[1]; dp[2] = rgb[2];
+    }
+}
+
 hipError_t launch_post(const PostParams& p, hipStream_t stream) {
     if (p.radius < 0 || p.radius > kMaxPostRadius) return hipErrorInvalidValue;
+    if (p.weave) {
+        if (p.radius != 0) return hipErrorInvalidValue;
+        const long long n = static_cast<long long>(p.io.width) * p.io.height;
+        hipLaunchKernelGGL(weave_encode_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, p);
+        return hipGetLastError();
+    }
     const dim3 grid((p.io.width + kTileW - 1) / kTileW, (p.io.height + kTileH - 1) / kTileH);
     hipLaunchKernelGGL(post_kernel, grid, dim3(256), 0, stream, p);
     return hipGetLastError();

@@ -113,14 +113,19 @@ def test_uhd_sampled_parity():
 
 
 def test_unsupported_family_is_rejected_loudly():
-    p = prepared.simple_view(prepared.load("NegativePrintHanatos"))
-    p.record.effects.hash = 0x5EED  # film/gate dust, scratches and weave are not rendered
-    src = synth.frame(16, 16, seed=1)
-    s = torch.from_numpy(src).cuda()
-    d = torch.zeros_like(s)
-    f = runtime.frame_for(s.data_ptr(), d.data_ptr(), 16, 16, 4, stream=torch.cuda.current_stream().cuda_stream)
-    out, text = runtime.shared_renderer().render_raw(f, p.record)
-    assert out.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT and "film/gate effects" in text
+    # film dust and scratches are not rendered; an effects identity without
+    # an active family is malformed (native/juicer_cuda_descriptors.cpp:476-479)
+    for flags, want in ((abi.FJ_EFFECTS_FILM, "film/gate dust and scratches"), (0, "field=effects.inactive")):
+        p = prepared.simple_view(prepared.load("NegativePrintHanatos"))
+        p.record.effects.flags = flags
+        p.record.effects.hash = 0x5EED
+        p.record.effects.recipe_hash = 0x5EED
+        src = synth.frame(16, 16, seed=1)
+        s = torch.from_numpy(src).cuda()
+        d = torch.zeros_like(s)
+        f = runtime.frame_for(s.data_ptr(), d.data_ptr(), 16, 16, 4, stream=torch.cuda.current_stream().cuda_stream)
+        out, text = runtime.shared_renderer().render_raw(f, p.record)
+        assert out.status.category == abi.FJ_STATUS_UNSUPPORTED_INPUT and want in text, text
 
 
 def test_malformed_span_and_identity():
